@@ -1,0 +1,160 @@
+/*
+ * loopstructural_amd: MI355X-native discrete implicit interpolation.
+ *
+ * C ABI of libloopstructural_amd.so.  Plain pointers and sizes only.  Every
+ * array argument is DEVICE memory unless stated, every call takes the HIP
+ * stream to run on (NULL = default stream) and returns an int status
+ * (LSAMD_OK = 0).  The message of the last failing call on the calling thread
+ * is returned by lsamd_last_error().  The caller owns every buffer it passes;
+ * workspace lives behind an explicit create/destroy handle (lsamd_fd_system,
+ * lsamd_batch).  Node ids are int64 at the ABI and F-order (x fastest):
+ * g = i + nx*j + nx*ny*k.
+ *
+ * Reference interfaces replaced (paths relative to the LoopStructural
+ * checkout, LI = packages/loop_interpolation/src/loop_interpolation,
+ * LC = packages/loop_common/src/loop_common):
+ *   lsamd_fd_locate        LC/supports/_3d_base_structured.py:251-293,386-412
+ *                          LC/supports/_3d_structured_grid.py:90-162,393-454
+ *   lsamd_fd_build_rows    LI/_finite_difference_interpolator.py:236-269,342-476,478-539
+ *                          LI/_discrete_interpolator.py:316-389 (row normalisation)
+ *                          LC/math/_maths.py:276-332 (strike/dip basis)
+ *   lsamd_fd_system_*      LI/_finite_difference_interpolator.py:643-692,1052-1164 (regulariser)
+ *                          LI/_discrete_interpolator.py:684-755,815-833 (build_matrix, scaling)
+ *                          LI/_solver_pipeline.py:86-124 (ridge, column scaling)
+ *                          LI/_solver_strategy.py:48-71 (cg on A.T@A)
+ *   lsamd_fd_evaluate      LC/supports/_3d_structured_grid.py:295-391
+ *                          LI/_discrete_interpolator.py:1342-1379
+ *   lsamd_rows_adjoint     no reference counterpart (gradient of a held-out
+ *                          misfit w.r.t. constraint values, weights, normals)
+ */
+#ifndef LOOPSTRUCTURAL_AMD_H
+#define LOOPSTRUCTURAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSAMD_OK 0
+#define LSAMD_ERR_ARG 1        /* invalid argument / shape */
+#define LSAMD_ERR_HIP 2        /* HIP runtime error */
+#define LSAMD_ERR_ALLOC 3      /* device allocation failed */
+#define LSAMD_ERR_INJECTED 9   /* fault injected through LSAMD_FAULT_INJECT */
+
+/* Precision of the stencil/row operator. */
+#define LSAMD_FP64 0
+#define LSAMD_FP32 1
+
+/* Row families for lsamd_fd_build_rows. */
+#define LSAMD_ROWS_VALUE 0    /* 1 row / point: trilinear N, b = v            */
+#define LSAMD_ROWS_NORMAL 1   /* 3 rows / point: T_d, b = n_d (d = 0,1,2)     */
+#define LSAMD_ROWS_GRADIENT 2 /* 2 rows / point: strike.T, dip.T, b = 0       */
+#define LSAMD_ROWS_TANGENT 3  /* 1 row / point: t_hat.T, b = 0                */
+
+/* Structured grid: nnodes = nsteps_cells + 1 per axis. */
+typedef struct lsamd_grid {
+  double origin[3];
+  double step[3];
+  int64_t nnodes[3];
+} lsamd_grid;
+
+/* Regulariser weights, all already squared: w2_xx = dxx^2, w2_xy = (dxy/4)^2,
+ * w2_bx = dx^2 (border families, 0 when dx == 0 or nnodes[0] < 3),
+ * ridge2 = ridge_factor^2 (0 when ridge is off). */
+typedef struct lsamd_fd_reg {
+  double w2_xx, w2_yy, w2_zz;
+  double w2_xy, w2_xz, w2_yz;
+  double w2_bx, w2_by, w2_bz;
+  double ridge2;
+} lsamd_fd_reg;
+
+/* PCG options.  The stopping test is scipy's on the column-scaled system:
+ * |S r| < max(atol, rtol |S rhs|), S = diag(N)^-1/2 (Jacobi), checked before
+ * each iteration; maxiter <= 0 means 10 * dof. */
+typedef struct lsamd_solve_opts {
+  double rtol;
+  double atol;
+  int64_t maxiter;
+  int32_t precision;    /* LSAMD_FP64: plain FP64 PCG.
+                           LSAMD_FP32: FP32 operator/vectors, FP64 dots, FP64
+                           residual recomputation + outer refinement. */
+  int32_t check_every;  /* iterations per captured graph chunk (<=0: 64)   */
+  double inner_rtol;    /* FP32 inner solve tolerance (<=0: 1e-4)          */
+  int64_t max_outer;    /* refinement sweeps (<=0: 50)                     */
+} lsamd_solve_opts;
+
+typedef struct lsamd_solve_info {
+  int64_t iterations;   /* total (inner) CG iterations                      */
+  int64_t outer;        /* refinement sweeps (0 for LSAMD_FP64)             */
+  int32_t converged;    /* 1 if the stopping test was met                   */
+  int32_t pad;
+  double residual;      /* final |S r| (FP64)                               */
+  double threshold;     /* max(atol, rtol |S rhs|)                          */
+} lsamd_solve_info;
+
+typedef struct lsamd_fd_system lsamd_fd_system;
+
+const char* lsamd_last_error(void);
+int lsamd_version(void);
+int lsamd_device_count(int* count);
+
+/* A2-A4: locate points in the grid.  pts (n,3) row-major FP64.  Outputs
+ * (any may be NULL): idc (n,8) int64 corner node ids (-1 outside), w (n,8)
+ * trilinear weights, T (n,3,8) shape-function gradients, inside (n) uint8.
+ * Bit-identical to the reference's numpy arithmetic. */
+int lsamd_fd_locate(const lsamd_grid* g, const double* pts, int64_t n, int64_t* idc,
+                    double* w, double* T, uint8_t* inside, void* stream);
+
+/* A5-A7: least-squares rows for n points that are INSIDE the grid.
+ * pts (n, ld) FP64 row-major: x y z | value or vector ... | weight.
+ * Writes rows_per_point*n rows of 8 entries, family-major (row = r*n + i):
+ * cols (rows,8) int32, coef (rows,8) FP64 normalised, b (rows) normalised,
+ * w (rows) = wscale * (use_point_weight ? pts[i, weight_col] : 1),
+ * norm (rows) = |a| before normalisation.  GRADIENT/TANGENT normalise the
+ * vector columns of pts IN PLACE (the reference does the same). */
+int lsamd_fd_build_rows(const lsamd_grid* g, int32_t kind, double* pts, int64_t n, int64_t ld,
+                        int32_t weight_col, double wscale, int32_t* cols, double* coef,
+                        double* b, double* w, double* norm, void* stream);
+
+/* A8-A12: the normal-equations system N = A^T W^2 A + R^T W R + ridge2 I of
+ * one structured grid.  scale2 (nnodes, nullable) multiplies every
+ * regulariser row weight^2 at its centre node. */
+int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const double* scale2,
+                           lsamd_fd_system** out);
+int lsamd_fd_system_destroy(lsamd_fd_system* s);
+
+/* Data rows in CSR form: row_ptr (nrows+1) int64, cols int32, coef FP64,
+ * b (nrows), w (nrows) (weights, not squared).  Copied into the handle;
+ * builds the node-major transpose, rhs and Jacobi diagonal. */
+int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
+                             const int64_t* row_ptr, const int32_t* cols, const double* coef,
+                             const double* b, const double* w, void* stream);
+
+/* y = N x in FP64 (x, y: nnodes FP64) or FP32 (x, y: nnodes float). */
+int lsamd_fd_system_apply(lsamd_fd_system* s, int32_t precision, const void* x, void* y,
+                          void* stream);
+/* rhs = A^T W^2 b ; diag = diag(N). */
+int lsamd_fd_system_rhs(lsamd_fd_system* s, double* rhs, void* stream);
+int lsamd_fd_system_diag(lsamd_fd_system* s, double* diag, void* stream);
+/* Solve N c = rhs (rhs NULL: the system's own).  c (nnodes) FP64, output. */
+int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, const double* rhs,
+                          double* c, lsamd_solve_info* info, void* stream);
+
+/* A14: evaluate c at n points.  values (n) or grads (n,3), either NULL.
+ * Values: outside -> NaN, NaN coordinate -> 0.  Gradients: outside -> 0. */
+int lsamd_fd_evaluate(const lsamd_grid* g, const double* c, const double* pts, int64_t n,
+                      double* values, double* grads, void* stream);
+
+/* A20: per data row r of the system (set_rows order), with u = lambda . a_r
+ * and the row's normalised coefficients a_r, rhs b_r, weight w_r:
+ *   g_b[r] = w_r^2 u            (d loss / d b_hat_r)
+ *   g_w[r] = 2 w_r u (b_r - a_r . c)   (d loss / d w_r)
+ * for the loss L with N lambda = dL/dc.  Either output may be NULL. */
+int lsamd_rows_adjoint(lsamd_fd_system* s, const double* c, const double* lambda, double* g_b,
+                       double* g_w, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+// Shared device/host helpers for libloopstructural_amd (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/loopstructural_amd.h"
+
+namespace lsamd {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+// Returns LSAMD_ERR_INJECTED when the LSAMD_FAULT_INJECT environment
+// variable names `site` (comma separated list), so host tests can force the
+// error paths of every entry point without a faulting kernel.
+int injected(const char* site);
+
+#define LSAMD_HIP(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::lsamd::fail(LSAMD_ERR_HIP, std::string(#expr) + ": " +                \
+                                              hipGetErrorString(e_));                \
+  } while (0)
+
+#define LSAMD_CHECK_LAUNCH(name)                                                     \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess)                                                            \
+      return ::lsamd::fail(LSAMD_ERR_HIP, std::string(name) + ": " +                 \
+                                              hipGetErrorString(e_));                \
+  } while (0)
+
+constexpr int kWave = 64;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Device view of a structured grid.
+struct GridDev {
+  double ox, oy, oz;
+  double hx, hy, hz;
+  int nx, ny, nz;  // nodes per axis
+};
+
+inline GridDev to_dev(const lsamd_grid& g) {
+  GridDev d;
+  d.ox = g.origin[0];
+  d.oy = g.origin[1];
+  d.oz = g.origin[2];
+  d.hx = g.step[0];
+  d.hy = g.step[1];
+  d.hz = g.step[2];
+  d.nx = (int)g.nnodes[0];
+  d.ny = (int)g.nnodes[1];
+  d.nz = (int)g.nnodes[2];
+  return d;
+}
+
+int check_grid(const lsamd_grid* g);
+
+// ------------------------------------------------ deterministic reductions --
+// Fixed-order wave64 butterfly sum.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum in a fixed order (waves reduced by lane 0 of wave 0 in index
+// order).  `sh` must hold blockDim.x/64 doubles.  Result valid in thread 0.
+__device__ inline double block_sum(double v, double* sh) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += sh[i];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Device-coherent load of a value written by another workgroup.
+__device__ inline double coherent_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int coherent_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// "Last block done": every block publishes its partial, then one ticket is
+// drawn; the block holding the final ticket sees every partial and reduces
+// them in index order (deterministic).  Returns true in all threads of the
+// last block.  `counter` is reset to 0 by the last block.
+__device__ inline bool last_block(int* counter, int nblocks, int* sh_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = atomicAdd(counter, 1);
+    *sh_flag = (t == nblocks - 1);
+  }
+  __syncthreads();
+  bool last = *sh_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Sum `n` partials in fixed order with the whole block; result in thread 0.
+__device__ inline double reduce_partials(const double* part, int n, double* sh) {
+  double v = 0.0;
+  // each thread sums a contiguous, fixed chunk
+  const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b = threadIdx.x * per;
+  for (int i = 0; i < per; ++i) {
+    int k = b + i;
+    if (k < n) v += coherent_load(part + k);
+  }
+  return block_sum(v, sh);
+}
+
+}  // namespace lsamd

@@ -1,0 +1,228 @@
+"""Device-side finite-difference least-squares system (thin owner of the C handle).
+
+``FdRows`` is one family of least-squares rows held in HBM (CSR with int32
+columns, normalised FP64 coefficients, normalised rhs, weights and the
+pre-normalisation row norms).  ``FdSystem`` owns an ``lsamd_fd_system``
+handle: the matrix-free normal operator of the regulariser plus the data rows,
+its Jacobi diagonal, rhs and the PCG driver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+F64 = torch.float64
+
+
+@dataclass
+class FdRows:
+    """A block of least-squares rows on the device (one reference family).
+
+    Mirrors ``constraints[name] = {matrix, b, w}``
+    (LI/_discrete_interpolator.py:383-389); coefficients and ``b`` are
+    already row-normalised (:361-366) and ``w`` is not yet applied.
+    """
+
+    row_ptr: torch.Tensor  # (R+1,) int64
+    cols: torch.Tensor     # (nnz,) int32
+    coef: torch.Tensor     # (nnz,) float64
+    b: torch.Tensor        # (R,) float64
+    w: torch.Tensor        # (R,) float64
+    norm: torch.Tensor     # (R,) float64, |a| before normalisation
+    dof: int
+
+    @property
+    def nrows(self) -> int:
+        return int(self.b.shape[0])
+
+    @property
+    def shape(self):
+        return (self.nrows, self.dof)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.cols.shape[0])
+
+    def to_scipy(self):
+        """Materialise the normalised rows as a scipy CSR matrix (inspection only)."""
+        from scipy import sparse
+
+        return sparse.csr_matrix((self.coef.cpu().numpy(), self.cols.cpu().numpy().astype(np.int64),
+                                  self.row_ptr.cpu().numpy()), shape=self.shape)
+
+
+def fixed_width_rows(cols, coef, b, w, norm, dof) -> FdRows:
+    """Wrap (R,k) column/coefficient tensors as an FdRows block."""
+    R, k = cols.shape
+    dev = cols.device
+    row_ptr = torch.arange(0, (R + 1) * k, k, dtype=torch.int64, device=dev)
+    return FdRows(row_ptr, cols.reshape(-1).contiguous(), coef.reshape(-1).contiguous(),
+                  b.contiguous(), w.contiguous(), norm.contiguous(), dof)
+
+
+def concat_rows(blocks, dof) -> FdRows | None:
+    blocks = [b for b in blocks if b is not None and b.nrows > 0]
+    if not blocks:
+        return None
+    ptrs, off = [], 0
+    for i, b in enumerate(blocks):
+        p = b.row_ptr[:-1] + off if i < len(blocks) - 1 else b.row_ptr + off
+        ptrs.append(p)
+        off += b.nnz
+    return FdRows(torch.cat(ptrs), torch.cat([b.cols for b in blocks]),
+                  torch.cat([b.coef for b in blocks]), torch.cat([b.b for b in blocks]),
+                  torch.cat([b.w for b in blocks]), torch.cat([b.norm for b in blocks]), dof)
+
+
+def reg_struct(w: dict, nnodes, ridge: float | None) -> _lib.FdReg:
+    """Squared regulariser weights (LC/supports/_3d_structured_grid.py:489-510:
+    cross families use dxy/4; LI/_finite_difference_interpolator.py:1141-1143:
+    border families skipped when their weight is 0 or the axis has < 3 nodes)."""
+    r = _lib.FdReg()
+    r.w2_xx = float(w["dxx"]) ** 2
+    r.w2_yy = float(w["dyy"]) ** 2
+    r.w2_zz = float(w["dzz"]) ** 2
+    r.w2_xy = (float(w["dxy"]) / 4) ** 2
+    r.w2_xz = (float(w["dxz"]) / 4) ** 2
+    r.w2_yz = (float(w["dyz"]) / 4) ** 2
+    for ax, key, name in ((0, "dx", "w2_bx"), (1, "dy", "w2_by"), (2, "dz", "w2_bz")):
+        bw = float(w.get(key, 0.0))
+        setattr(r, name, bw * bw if (bw != 0 and int(nnodes[ax]) >= 3) else 0.0)
+    r.ridge2 = float(ridge) ** 2 if ridge else 0.0
+    return r
+
+
+class FdSystem:
+    """Owner of one ``lsamd_fd_system`` handle."""
+
+    def __init__(self, grid: _lib.Grid, reg: _lib.FdReg, scale2: torch.Tensor | None = None):
+        self.dev = _lib.device()
+        self.grid = grid
+        self.reg = reg
+        self.n = int(grid.nnodes[0] * grid.nnodes[1] * grid.nnodes[2])
+        self._scale2 = None
+        if scale2 is not None:
+            self._scale2 = scale2.to(self.dev, F64).contiguous()
+        h = C.c_void_p()
+        _lib.call("lsamd_fd_system_create", C.byref(grid), C.byref(reg), _lib.ptr(self._scale2),
+                  C.byref(h))
+        self._h = h
+        self.rows: FdRows | None = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.load().lsamd_fd_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rows(self, rows: FdRows | None):
+        self.rows = rows
+        s = _lib.stream()
+        if rows is None or rows.nrows == 0:
+            _lib.call("lsamd_fd_system_set_rows", self._h, 0, 0, None, None, None, None, None, s)
+            return
+        _lib.call("lsamd_fd_system_set_rows", self._h, rows.nrows, rows.nnz,
+                  _lib.ptr(rows.row_ptr), _lib.ptr(rows.cols), _lib.ptr(rows.coef),
+                  _lib.ptr(rows.b), _lib.ptr(rows.w), s)
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        prec = _lib.FP32 if x.dtype == torch.float32 else _lib.FP64
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        _lib.call("lsamd_fd_system_apply", self._h, prec, _lib.ptr(x), _lib.ptr(y), _lib.stream())
+        return y
+
+    def rhs(self) -> torch.Tensor:
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_rhs", self._h, _lib.ptr(out), _lib.stream())
+        return out
+
+    def diag(self) -> torch.Tensor:
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_diag", self._h, _lib.ptr(out), _lib.stream())
+        return out
+
+    def solve(self, rtol=1e-5, atol=0.0, maxiter=None, precision="fp64", rhs=None,
+              check_every=64, inner_rtol=1e-4, max_outer=50):
+        o = _lib.SolveOpts()
+        o.rtol = float(rtol)
+        o.atol = float(atol)
+        o.maxiter = int(maxiter) if maxiter else 0
+        o.precision = _lib.FP32 if precision == "fp32" else _lib.FP64
+        o.check_every = int(check_every)
+        o.inner_rtol = float(inner_rtol)
+        o.max_outer = int(max_outer)
+        info = _lib.SolveInfo()
+        c = torch.empty(self.n, dtype=F64, device=self.dev)
+        if rhs is not None:
+            rhs = rhs.to(self.dev, F64).contiguous()
+        _lib.call("lsamd_fd_system_solve", self._h, C.byref(o), _lib.ptr(rhs), _lib.ptr(c),
+                  C.byref(info), _lib.stream())
+        return c, {"iterations": int(info.iterations), "outer": int(info.outer),
+                   "converged": bool(info.converged), "residual": float(info.residual),
+                   "threshold": float(info.threshold)}
+
+    def adjoint_rows(self, c: torch.Tensor, lam: torch.Tensor):
+        R = self.rows.nrows if self.rows is not None else 0
+        gb = torch.empty(R, dtype=F64, device=self.dev)
+        gw = torch.empty(R, dtype=F64, device=self.dev)
+        if R:
+            _lib.call("lsamd_rows_adjoint", self._h, _lib.ptr(c.contiguous()),
+                      _lib.ptr(lam.contiguous()), _lib.ptr(gb), _lib.ptr(gw), _lib.stream())
+        return gb, gw
+
+
+# ------------------------------------------------------- kernel wrappers ----
+def locate(grid: _lib.Grid, pts: torch.Tensor, want_w=True, want_T=False):
+    """A2-A4 on the device: returns (idc int64 (n,8), inside bool, w (n,8), T (n,3,8))."""
+    pts = pts.to(_lib.device(), F64).contiguous()
+    n = pts.shape[0]
+    idc = torch.empty((n, 8), dtype=torch.int64, device=pts.device)
+    ins = torch.empty(n, dtype=torch.uint8, device=pts.device)
+    w = torch.empty((n, 8), dtype=F64, device=pts.device) if want_w else None
+    T = torch.empty((n, 3, 8), dtype=F64, device=pts.device) if want_T else None
+    _lib.call("lsamd_fd_locate", C.byref(grid), _lib.ptr(pts), n, _lib.ptr(idc), _lib.ptr(w),
+              _lib.ptr(T), _lib.ptr(ins), _lib.stream())
+    return idc, ins.bool(), w, T
+
+
+def build_rows(grid: _lib.Grid, kind: int, pts: torch.Tensor, weight_col: int, wscale: float,
+               dof: int) -> FdRows:
+    """A5-A7 rows for points already known to be inside.  ``pts`` is updated in
+    place for GRADIENT/TANGENT (normalised vectors), as in the reference."""
+    n, ld = pts.shape
+    per = {_lib.ROWS_VALUE: 1, _lib.ROWS_NORMAL: 3, _lib.ROWS_GRADIENT: 2,
+           _lib.ROWS_TANGENT: 1}[kind]
+    R = per * n
+    dev = pts.device
+    cols = torch.empty((R, 8), dtype=torch.int32, device=dev)
+    coef = torch.empty((R, 8), dtype=F64, device=dev)
+    b = torch.empty(R, dtype=F64, device=dev)
+    w = torch.empty(R, dtype=F64, device=dev)
+    norm = torch.empty(R, dtype=F64, device=dev)
+    if n:
+        _lib.call("lsamd_fd_build_rows", C.byref(grid), kind, _lib.ptr(pts), n, ld, weight_col,
+                  float(wscale), _lib.ptr(cols), _lib.ptr(coef), _lib.ptr(b), _lib.ptr(w),
+                  _lib.ptr(norm), _lib.stream())
+    return fixed_width_rows(cols, coef, b, w, norm, dof)
+
+
+def evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True, gradients=False):
+    pts = pts.to(_lib.device(), F64).contiguous()
+    c = c.to(pts.device, F64).contiguous()
+    n = pts.shape[0]
+    v = torch.empty(n, dtype=F64, device=pts.device) if values else None
+    g = torch.empty((n, 3), dtype=F64, device=pts.device) if gradients else None
+    _lib.call("lsamd_fd_evaluate", C.byref(grid), _lib.ptr(c), _lib.ptr(pts), n, _lib.ptr(v),
+              _lib.ptr(g), _lib.stream())
+    return v, g

@@ -1,0 +1,35 @@
+"""CPU oracle for the discrete implicit interpolation path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``loopstructural_amd`` imports this
+package; only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker or as the
+timed CPU baseline, never as a product code path.
+
+It restates, in plain numpy/scipy, the reference algorithm of
+LoopStructural's ``FiniteDifferenceInterpolator`` / ``P1Interpolator`` path:
+explicit sparse assembly with the reference's row normalisation, family order,
+weights, border families, ridge and column scaling, then
+``scipy.sparse.linalg.cg(A.T @ A, A.T @ b)``.  Each function cites the
+reference file:line it follows (paths relative to the reference checkout;
+``LI/`` = ``packages/loop_interpolation/src/loop_interpolation/``,
+``LC/`` = ``packages/loop_common/src/loop_common/``).
+
+Pinning (what checks each function, see ``tests/test_oracle_*.py``):
+  * grid geometry, cell lookup, trilinear weights/gradients, evaluation:
+    the reference's own known-answer tests (local coords (0.2,0.5,0.7),
+    linear-field value/gradient exactness, max-face and corner evaluation,
+    outside handling, border row counts) plus closed forms;
+  * regulariser rows: the reference's mask KATs, row sums = 0, annihilation of
+    linear fields, symmetric Gram, brute-force comparison against an
+    independent dense finite-difference construction on tiny grids;
+  * data rows: closed forms (partition of unity, gradient of linear fields),
+    strike/dip orthonormality against the normal;
+  * solve: the unique least-squares minimiser via ``scipy.sparse.linalg.spsolve``
+    on the normal equations (library routine) and ``numpy.linalg.lstsq`` on
+    tiny dense systems;
+  * P1: closed-form barycentric coordinates, partition of unity, exact
+    gradients of linear fields, face-pair counts, brute-force face search.
+Functions whose exact floating output only an import of the reference could
+pin bit-for-bit (``get_vectors`` trig rounding, the P1 face-table order) are
+pinned at the level of the rows they produce after normalisation.
+"""
