@@ -82,6 +82,10 @@ typedef struct lsamd_solve_opts {
   int32_t check_every;  /* iterations per captured graph chunk (<=0: 64)   */
   double inner_rtol;    /* FP32 inner solve tolerance (<=0: 1e-4)          */
   int64_t max_outer;    /* refinement sweeps (<=0: 50)                     */
+  int32_t jacobi;       /* 1: Jacobi preconditioner = the reference's column
+                           scaling (apply_scaling_matrix=True); 0: plain CG
+                           with the 2-norm stopping test                   */
+  int32_t pad;
 } lsamd_solve_opts;
 
 typedef struct lsamd_solve_info {
@@ -117,6 +121,13 @@ int lsamd_fd_build_rows(const lsamd_grid* g, int32_t kind, double* pts, int64_t 
                         int32_t weight_col, double wscale, int32_t* cols, double* coef,
                         double* b, double* w, double* norm, void* stream);
 
+/* A6: row normalisation of nrows rows of width k (k <= 128) in place:
+ * a /= |a|, b /= |a| where |a| > 0, with |a| summed in numpy's pairwise
+ * order (LI/_discrete_interpolator.py:361-366); norm (nullable) gets |a|.
+ * Used for interface rows (k = 16) and caller-supplied rows. */
+int lsamd_rows_normalise(int64_t nrows, int32_t k, double* coef, double* b, double* norm,
+                         void* stream);
+
 /* A8-A12: the normal-equations system N = A^T W^2 A + R^T W R + ridge2 I of
  * one structured grid.  scale2 (nnodes, nullable) multiplies every
  * regulariser row weight^2 at its centre node. */
@@ -141,10 +152,29 @@ int lsamd_fd_system_diag(lsamd_fd_system* s, double* diag, void* stream);
 int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, const double* rhs,
                           double* c, lsamd_solve_info* info, void* stream);
 
+/* Measurement: launch the regulariser stencil kernel alone `iters` times
+ * (FP64 or FP32, on the handle's work vectors) bracketed by HIP events on
+ * `stream`; *avg_ms = mean kernel duration. */
+int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
+                                 double* avg_ms, void* stream);
+
+/* Autotuner result: z-chunk of the plane-streaming stencil chosen for the
+ * precision (0 = exact per-node kernel: an axis has < 7 nodes or a per-node
+ * weight scale is set; also 0 before the first apply/solve), and the timed
+ * candidates 8/16/32 in microseconds (tune_us[3], nullable, -1 = skipped). */
+int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,
+                                 float* tune_us);
+
 /* A14: evaluate c at n points.  values (n) or grads (n,3), either NULL.
  * Values: outside -> NaN, NaN coordinate -> 0.  Gradients: outside -> 0. */
 int lsamd_fd_evaluate(const lsamd_grid* g, const double* c, const double* pts, int64_t n,
                       double* values, double* grads, void* stream);
+
+/* A7 (use_regularisation_weight_scale): squared distance from every node
+ * (np.linspace coordinates) to the nearest of m points (m,3); d2 (nnodes).
+ * Replaces the KDTree query of LI/_finite_difference_interpolator.py:66-88. */
+int lsamd_fd_nearest_d2(const lsamd_grid* g, const double* pts, int64_t m, double* d2,
+                        void* stream);
 
 /* A20: per data row r of the system (set_rows order), with u = lambda . a_r
  * and the row's normalised coefficients a_r, rhs b_r, weight w_r:

@@ -44,7 +44,8 @@ class FdReg(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("maxiter", C.c_int64),
                 ("precision", C.c_int32), ("check_every", C.c_int32),
-                ("inner_rtol", C.c_double), ("max_outer", C.c_int64)]
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int64),
+                ("jacobi", C.c_int32), ("pad", C.c_int32)]
 
 
 class SolveInfo(C.Structure):
@@ -73,6 +74,10 @@ SIGNATURES = {
     "lsamd_fd_system_solve": [_P, _P, _P, _P, _P, _P],
     "lsamd_fd_evaluate": [_P, _P, _P, _I64, _P, _P, _P],
     "lsamd_rows_adjoint": [_P, _P, _P, _P, _P, _P],
+    "lsamd_rows_normalise": [_I64, _I32, _P, _P, _P, _P],
+    "lsamd_fd_nearest_d2": [_P, _P, _I64, _P, _P],
+    "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
+    "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],
 }
 _RESTYPE = {"lsamd_last_error": C.c_char_p}
 

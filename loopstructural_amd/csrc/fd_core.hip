@@ -55,110 +55,244 @@ int check_grid(const lsamd_grid* g) {
 
 // ------------------------------------------------------- CG control block --
 struct CgCtl {
-  double rho, rho_prev, pq_st, pq_rows, thr, bnorm, acc;
+  double rho, rho_prev, pq_st, pq_rows, thr, bnorm, acc, alpha;
   int iter, done, maxiter, converged;
-  int cnt_st, cnt_rows, cnt_upd, cnt_misc;
 };
 
 // ------------------------------------------------------------- stencil ----
+// y = (R^T W R + ridge2 I) x with per-block partials of x.y.
+//
+// Regular grids (>= 7 nodes per axis, uniform row weights) use three
+// launches over disjoint node sets, all plane-streaming 64x8 xy tiles
+// through a double-buffered LDS plane with D planes of global loads kept in
+// flight in registers:
+//   * k_st_bulk<UNI=true>  tiles interior in x and y, output planes
+//     [3, nz-4]: one coefficient set, passed as kernel arguments;
+//   * k_st_bulk<UNI=false> edge tiles, same planes: per-lane coefficients
+//     of the node's (x, y) class, loaded once before the first prefetch;
+//   * k_st_zb              output planes [0,3) and [nz-3,nz): coefficients
+//     re-read per plane.
+// Other grids (an axis with < 7 nodes, or a per-node weight scale) use
+// k_st_general: one node per thread, exact two-pass formula.
 constexpr int TX = 64, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 4, PY = TY + 4, PSZ = PX * PY;
 
 template <typename T>
-struct StencilArgs {
-  GridDev g;
-  Composite<T> k;
-  RegW<T> w;
-  const double* scale2;
-  Shell sh;
-  int nbx, nby, zc;
-  int64_t nbulk;  // number of bulk blocks (shell blocks follow)
+struct Coef25 {
+  T k[kTaps];
 };
 
-// y = (R^T W R + ridge2 I) x ; per-block partial of x.y, reduced by the last
-// block into ctl->pq_st.  Bulk blocks: 64x8 xy tile marching over zc planes
-// with the 25-tap composite, stores only nodes at distance >= 3 from every
-// face.  Shell blocks: one node per thread, exact two-pass formula.
+// 25-tap update of one output node from its plane sums; see the tap order
+// in fd_operator.hpp.  b points at the node in the current LDS plane.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_stencil(const T* __restrict__ x, T* __restrict__ y,
-                                                StencilArgs<T> a, double* __restrict__ part,
-                                                CgCtl* ctl) {
-  if (coherent_load(&ctl->done)) return;
+__device__ __forceinline__ T inplane(const T* k, const T* b) {
+  return k[0] * b[0] + k[1] * b[1] + k[2] * b[-1] + k[3] * b[PX] + k[4] * b[-PX] + k[5] * b[2] +
+         k[6] * b[-2] + k[7] * b[2 * PX] + k[8] * b[-2 * PX] + k[9] * b[2 * PX + 2] +
+         k[10] * b[2 * PX - 2] + k[11] * b[-2 * PX + 2] + k[12] * b[-2 * PX - 2];
+}
+
+template <typename T, int ZC, int D, bool UNI>
+__global__ __launch_bounds__(NT) void k_st_bulk(const T* __restrict__ x, T* __restrict__ y,
+                                                GridDev g, const T* __restrict__ tab,
+                                                Coef25<T> ku, const int2* __restrict__ tiles,
+                                                int ntile, double* __restrict__ part,
+                                                const CgCtl* ctl) {
+  if (ctl->done) return;
+  constexpr int NS = ZC + 4;
   __shared__ T pl[2][PSZ];
-  __shared__ double sh[NT / 64];
-  __shared__ int flag;
-  const GridDev& g = a.g;
-  double dot = 0.0;
-  const int64_t bid = blockIdx.x;
-  if (bid < a.nbulk) {
-    const int bx = (int)(bid % a.nbx);
-    const int by = (int)((bid / a.nbx) % a.nby);
-    const int bz = (int)(bid / ((int64_t)a.nbx * a.nby));
-    const int x0 = bx * TX, y0 = by * TY, z0 = bz * a.zc;
-    const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
-    const int ix = x0 + lx, iy = y0 + ly;
-    const bool col_bulk = ix >= 3 && ix <= g.nx - 4 && iy >= 3 && iy <= g.ny - 4;
-    const int zend = min(z0 + a.zc, g.nz);
-    const Composite<T> K = a.k;
-    T xc3 = 0, xc2 = 0, xc1 = 0, xc0 = 0, v2 = 0, v1 = 0, v0 = 0;
-    T u4 = 0, u3 = 0, u2 = 0, u1 = 0, u0 = 0;
-    for (int p = z0 - 2; p < zend + 2; ++p) {
-      T* buf = pl[p & 1];
-      const bool pin = p >= 0 && p < g.nz;
-      for (int e = threadIdx.x; e < PSZ; e += NT) {
-        const int ey = e / PX, ex = e - ey * PX;
-        const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
-        T val = (T)0;
-        if (pin && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) val = x[nid(g, gx, gy, p)];
-        buf[e] = val;
-      }
-      __syncthreads();
-      xc3 = xc2;
-      xc2 = xc1;
-      xc1 = xc0;
-      v2 = v1;
-      v1 = v0;
-      u4 = u3;
-      u3 = u2;
-      u2 = u1;
-      u1 = u0;
-      const T* b = buf + (ly + 2) * PX + (lx + 2);
-      const T c = b[0];
-      const T sx1 = b[1] + b[-1];
-      const T sy1 = b[PX] + b[-PX];
-      const T sx2 = b[2] + b[-2];
-      const T sy2 = b[2 * PX] + b[-2 * PX];
-      const T sd = (b[2 * PX + 2] + b[2 * PX - 2]) + (b[-2 * PX + 2] + b[-2 * PX - 2]);
-      xc0 = c;
-      v0 = K.c0 * c + K.cx1 * sx1 + K.cy1 * sy1 + K.cx2 * sx2 + K.cy2 * sy2 + K.cxy * sd;
-      u0 = K.cz2 * c + K.cxz * sx2 + K.cyz * sy2;
-      const int kz = p - 2;
-      if (col_bulk && kz >= z0 && kz < zend && kz >= 3 && kz <= g.nz - 4) {
-        const T yv = v2 + K.cz1 * (xc3 + xc1) + u4 + u0;
-        y[nid(g, ix, iy, kz)] = yv;
-        dot += (double)xc2 * (double)yv;
-      }
-    }
+  __shared__ double shm[NT / 64];
+  const int txy = blockIdx.x % ntile, bz = blockIdx.x / ntile;
+  const int2 tb = tiles[txy];
+  const int x0 = tb.x * TX, y0 = tb.y * TY, z0 = 3 + bz * ZC;
+  const int zend = min(z0 + ZC, g.nz - 3);
+  const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
+  const int ix = x0 + lx, iy = y0 + ly;
+  const bool colok = ix < g.nx && iy < g.ny;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  T k[kTaps];
+  if constexpr (UNI) {
+#pragma unroll
+    for (int q = 0; q < kTaps; ++q) k[q] = ku.k[q];
   } else {
-    const int64_t t = (bid - a.nbulk) * NT + threadIdx.x;
-    if (t < a.sh.count) {
-      int i = 0, j = 0, k = 0;
-      shell_decode(g, a.sh, t, i, j, k);
-      const T yv = general_node(x, g, a.w, a.scale2, i, j, k);
-      const int64_t id = nid(g, i, j, k);
-      y[id] = yv;
-      dot += (double)x[id] * (double)yv;
-    }
+    const int cx = cls7(min(ix, g.nx - 1), g.nx), cy = cls7(min(iy, g.ny - 1), g.ny);
+    const T* tc = tab + (int64_t)((3 * 7 + cy) * 7 + cx) * kTaps;
+#pragma unroll
+    for (int q = 0; q < kTaps; ++q) k[q] = tc[q];
   }
-  const double bs = block_sum(dot, sh);
-  if (threadIdx.x == 0) part[bid] = bs;
-  if (last_block(&ctl->cnt_st, (int)gridDim.x, &flag)) {
-    const double tot = reduce_partials(part, (int)gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ctl->pq_st = tot;
-      ctl->cnt_st = 0;
-    }
+  // two load slots per thread in the (64+4) x (8+4) plane tile; addresses
+  // clamped into the grid, out-of-grid values zeroed after the load
+  const int e1 = threadIdx.x + NT;
+  const bool has1 = e1 < PSZ;
+  int64_t go0, go1;
+  bool ok0, ok1;
+  {
+    const int ey = threadIdx.x / PX, ex = threadIdx.x - ey * PX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok0 = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go0 = (int64_t)min(max(gx, 0), g.nx - 1) + (int64_t)min(max(gy, 0), g.ny - 1) * g.nx;
   }
+  {
+    const int ee = has1 ? e1 : 0;
+    const int ey = ee / PX, ex = ee - ey * PX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok1 = has1 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go1 = (int64_t)min(max(gx, 0), g.nx - 1) + (int64_t)min(max(gy, 0), g.ny - 1) * g.nx;
+  }
+  const int pbeg = z0 - 2;  // >= 1
+  T qa[D], qb[D];
+  auto ld = [&](int p, T& va, T& vb) {
+    const int64_t po = (int64_t)min(p, g.nz - 1) * nxy;
+    const T a = x[go0 + po], b = x[go1 + po];
+    va = ok0 ? a : (T)0;
+    vb = ok1 ? b : (T)0;
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) ld(pbeg + d, qa[d], qb[d]);
+  // ring of the four previous planes: c, x(+2,0), x(-2,0), x(0,+2), x(0,-2)
+  T r4[5] = {0, 0, 0, 0, 0}, r3[5] = {0, 0, 0, 0, 0}, r2[5] = {0, 0, 0, 0, 0},
+    r1[5] = {0, 0, 0, 0, 0};
+  T v2 = 0, v1 = 0;
+  double dot = 0.0;
+  pl[0][threadIdx.x] = qa[0];
+  if (has1) pl[0][e1] = qb[0];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (s + D < NS) ld(pbeg + s + D, qa[s % D], qb[s % D]);
+    const T* b = pl[s & 1] + (ly + 2) * PX + lx + 2;
+    const T c = b[0], xp2 = b[2], xm2 = b[-2], yp2 = b[2 * PX], ym2 = b[-2 * PX];
+    const T v0 = inplane(k, b);
+    if (s >= 4) {
+      const int z = pbeg + s - 2;
+      const T yv = v2 + k[13] * r3[0] + k[14] * r1[0] + k[15] * r4[0] + k[16] * r4[1] +
+                   k[17] * r4[2] + k[18] * r4[3] + k[19] * r4[4] + k[20] * c + k[21] * xp2 +
+                   k[22] * xm2 + k[23] * yp2 + k[24] * ym2;
+      if (colok && z < zend) {
+        y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy] = yv;
+        dot += (double)r2[0] * (double)yv;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      r4[q] = r3[q];
+      r3[q] = r2[q];
+      r2[q] = r1[q];
+    }
+    r1[0] = c;
+    r1[1] = xp2;
+    r1[2] = xm2;
+    r1[3] = yp2;
+    r1[4] = ym2;
+    v2 = v1;
+    v1 = v0;
+    if (s + 1 < NS) {
+      pl[(s + 1) & 1][threadIdx.x] = qa[(s + 1) % D];
+      if (has1) pl[(s + 1) & 1][e1] = qb[(s + 1) % D];
+    }
+    __syncthreads();
+  }
+  const double bs = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// output planes [0,3) (bz = 0) and [nz-3, nz) (bz = 1) of every xy tile
+template <typename T>
+__global__ __launch_bounds__(NT) void k_st_zb(const T* __restrict__ x, T* __restrict__ y,
+                                              GridDev g, const T* __restrict__ tab, int nbx,
+                                              int nby, double* __restrict__ part,
+                                              const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ T pl[2][PSZ];
+  __shared__ double shm[NT / 64];
+  const int tile = blockIdx.x % (nbx * nby), bz = blockIdx.x / (nbx * nby);
+  const int x0 = (tile % nbx) * TX, y0 = (tile / nbx) * TY, z0 = bz == 0 ? 0 : g.nz - 3;
+  const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
+  const int ix = x0 + lx, iy = y0 + ly;
+  const bool colok = ix < g.nx && iy < g.ny;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int e1 = threadIdx.x + NT;
+  int64_t go0, go1;
+  bool ok0, ok1;
+  {
+    const int ey = threadIdx.x / PX, ex = threadIdx.x - ey * PX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok0 = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go0 = ok0 ? (int64_t)gx + (int64_t)gy * g.nx : 0;
+  }
+  {
+    const int ey = e1 / PX, ex = e1 - ey * PX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok1 = e1 < PSZ && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go1 = ok1 ? (int64_t)gx + (int64_t)gy * g.nx : 0;
+  }
+  const int cx = cls7(min(ix, g.nx - 1), g.nx), cy = cls7(min(iy, g.ny - 1), g.ny);
+  const T* tc = tab + (int64_t)(cy * 7 + cx) * kTaps;
+  T r4[5] = {0, 0, 0, 0, 0}, r3[5] = {0, 0, 0, 0, 0}, r2[5] = {0, 0, 0, 0, 0},
+    r1[5] = {0, 0, 0, 0, 0};
+  T v2 = 0, v1 = 0;
+  double dot = 0.0;
+  for (int s = 0; s < 7; ++s) {
+    const int p = z0 - 2 + s;
+    const bool pin = p >= 0 && p < g.nz;
+    const int64_t po = (int64_t)(pin ? p : 0) * nxy;
+    pl[s & 1][threadIdx.x] = (pin && ok0) ? x[go0 + po] : (T)0;
+    if (e1 < PSZ) pl[s & 1][e1] = (pin && ok1) ? x[go1 + po] : (T)0;
+    __syncthreads();
+    const T* b = pl[s & 1] + (ly + 2) * PX + lx + 2;
+    const T* t = tc + (int64_t)(pin ? cls7(p, g.nz) : 3) * 49 * kTaps;
+    const T c = b[0], xp2 = b[2], xm2 = b[-2], yp2 = b[2 * PX], ym2 = b[-2 * PX];
+    const T v0 = inplane(t, b);
+    const int z = p - 2;
+    if (s >= 4 && z >= 0 && z < g.nz) {
+      const T* o = tc + (int64_t)cls7(z, g.nz) * 49 * kTaps;
+      const T yv = v2 + o[13] * r3[0] + o[14] * r1[0] + o[15] * r4[0] + o[16] * r4[1] +
+                   o[17] * r4[2] + o[18] * r4[3] + o[19] * r4[4] + o[20] * c + o[21] * xp2 +
+                   o[22] * xm2 + o[23] * yp2 + o[24] * ym2;
+      if (colok) {
+        y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy] = yv;
+        dot += (double)r2[0] * (double)yv;
+      }
+    }
+    for (int q = 0; q < 5; ++q) {
+      r4[q] = r3[q];
+      r3[q] = r2[q];
+      r2[q] = r1[q];
+    }
+    r1[0] = c;
+    r1[1] = xp2;
+    r1[2] = xm2;
+    r1[3] = yp2;
+    r1[4] = ym2;
+    v2 = v1;
+    v1 = v0;
+    __syncthreads();
+  }
+  const double bs = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// any grid: one node per thread, exact two-pass formula
+template <typename T>
+__global__ __launch_bounds__(256) void k_st_general(const T* __restrict__ x, T* __restrict__ y,
+                                                    GridDev g, RegW<T> w, const double* scale2,
+                                                    double* __restrict__ part, const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double shm[4];
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dot = 0.0;
+  if (id < n) {
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const int k = (int)(id / nxy);
+    const int64_t r = id - (int64_t)k * nxy;
+    const int j = (int)(r / g.nx);
+    const int i = (int)(r - (int64_t)j * g.nx);
+    const T yv = general_node(x, g, w, scale2, i, j, k);
+    y[id] = yv;
+    dot = (double)x[id] * (double)yv;
+  }
+  const double bs = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // ----------------------------------------------------------- data rows ----
@@ -180,9 +314,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_rows_t(const T* __restrict__ x, RowsDev R,
                                                 T* __restrict__ t, double* __restrict__ part,
                                                 CgCtl* ctl) {
-  if (coherent_load(&ctl->done)) return;
+  if (ctl->done) return;
   __shared__ double sh[4];
-  __shared__ int flag;
   const T* coef = (const T*)R.coef;
   const T* w2 = (const T*)R.w2;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,33 +329,44 @@ __global__ __launch_bounds__(256) void k_rows_t(const T* __restrict__ x, RowsDev
   }
   const double bs = block_sum(dot, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&ctl->cnt_rows, (int)gridDim.x, &flag)) {
-    const double tot = reduce_partials(part, (int)gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ctl->pq_rows = tot;
-      ctl->cnt_rows = 0;
-    }
-  }
 }
 
 // y[node] += sum over the node's entries of coef_e t[row_e]  (node-major,
 // fixed order: deterministic, no atomics).  accumulate=0 overwrites.
+// With fin != 0, block 0 then reduces the stencil and row partials (written
+// by the two previous kernels of the stream) into pq and alpha = rho / pq.
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void k_rows_scatter(RowsDev R, const T* __restrict__ t,
                                                       TO* __restrict__ y, int accumulate,
-                                                      CgCtl* ctl) {
-  if (ctl && coherent_load(&ctl->done)) return;
+                                                      CgCtl* ctl, int fin,
+                                                      const double* __restrict__ part_st,
+                                                      int n_st,
+                                                      const double* __restrict__ part_rows,
+                                                      int n_rows) {
+  if (ctl && ctl->done) return;
   const T* coef = (const T*)R.coef;
   const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= R.ndn) return;
-  T acc = 0;
-  for (int64_t s = R.seg_ptr[d]; s < R.seg_ptr[d + 1]; ++s)
-    acc += coef[R.seg_entry[s]] * t[R.seg_row[s]];
-  const int64_t node = R.dnode[d];
-  if (accumulate)
-    y[node] += (TO)acc;
-  else
-    y[node] = (TO)acc;
+  if (d < R.ndn) {
+    T acc = 0;
+    for (int64_t s = R.seg_ptr[d]; s < R.seg_ptr[d + 1]; ++s)
+      acc += coef[R.seg_entry[s]] * t[R.seg_row[s]];
+    const int64_t node = R.dnode[d];
+    if (accumulate)
+      y[node] += (TO)acc;
+    else
+      y[node] = (TO)acc;
+  }
+  if (fin && blockIdx.x == 0) {
+    __shared__ double sh[4];
+    const double a = reduce_partials(part_st, n_st, sh);
+    const double b = reduce_partials(part_rows, n_rows, sh);
+    if (threadIdx.x == 0) {
+      const double pq = a + b;
+      ctl->pq_st = a;
+      ctl->pq_rows = b;
+      ctl->alpha = ctl->rho / pq;
+    }
+  }
 }
 
 // data-term diagonal: sum over rows of w_r^2 (merged coefficient of node)^2
@@ -295,6 +439,13 @@ __global__ void k_minv(const double* __restrict__ diag, double* __restrict__ m64
   m64[i] = m;
   m32[i] = (float)m;
 }
+__global__ void k_fill_one(double* a, float* b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 1.0;
+    b[i] = 1.0f;
+  }
+}
 __global__ void k_zero_d(double* v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = 0.0;
@@ -319,15 +470,12 @@ __global__ void k_adjoint(RowsDev R, const double* __restrict__ coef, const doub
 // ------------------------------------------------------------------ PCG ----
 constexpr int VB = 256;  // vector kernel block size
 
-// x = 0 ; r = rhs ; rho = r.(minv r) ; thr = max(atol, rtol sqrt(rho))
+// x = 0 ; r = rhs ; partials of r.(minv r)
 template <typename T>
 __global__ __launch_bounds__(VB) void k_cg_init(const double* __restrict__ rhs,
                                                 const T* __restrict__ minv, T* __restrict__ x,
-                                                T* __restrict__ r, int64_t n, double rtol,
-                                                double atol, int maxiter, double* part,
-                                                CgCtl* ctl) {
+                                                T* __restrict__ r, int64_t n, double* part) {
   __shared__ double sh[VB / 64];
-  __shared__ int flag;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) {
     const T rv = (T)rhs[i];
@@ -337,21 +485,26 @@ __global__ __launch_bounds__(VB) void k_cg_init(const double* __restrict__ rhs,
   }
   const double bs = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&ctl->cnt_upd, (int)gridDim.x, &flag)) {
-    const double tot = reduce_partials(part, (int)gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      const double bn = sqrt(tot);
-      const double thr = fmax(atol, rtol * bn);
-      ctl->rho = tot;
-      ctl->rho_prev = 1.0;
-      ctl->bnorm = bn;
-      ctl->thr = thr;
-      ctl->iter = 0;
-      ctl->maxiter = maxiter;
-      ctl->converged = (sqrt(tot) < thr) || bn == 0.0;
-      ctl->done = ctl->converged;
-      ctl->cnt_upd = 0;
-    }
+}
+
+// one block: rho = sum(part); thr = max(atol, rtol sqrt(rho)); iter = 0
+__global__ __launch_bounds__(VB) void k_cg_init_fin(const double* __restrict__ part, int np,
+                                                    double rtol, double atol, int maxiter,
+                                                    CgCtl* ctl) {
+  __shared__ double sh[VB / 64];
+  const double tot = reduce_partials(part, np, sh);
+  if (threadIdx.x == 0) {
+    const double bn = sqrt(tot);
+    const double thr = fmax(atol, rtol * bn);
+    ctl->rho = tot;
+    ctl->rho_prev = 1.0;
+    ctl->bnorm = bn;
+    ctl->thr = thr;
+    ctl->alpha = 0.0;
+    ctl->iter = 0;
+    ctl->maxiter = maxiter;
+    ctl->converged = (sqrt(tot) < thr) || bn == 0.0;
+    ctl->done = ctl->converged;
   }
 }
 
@@ -359,27 +512,24 @@ __global__ __launch_bounds__(VB) void k_cg_init(const double* __restrict__ rhs,
 template <typename T>
 __global__ __launch_bounds__(VB) void k_cg_dir(const T* __restrict__ r, const T* __restrict__ minv,
                                                T* __restrict__ p, int64_t n, CgCtl* ctl) {
-  if (coherent_load(&ctl->done)) return;
-  const int first = coherent_load(&ctl->iter) == 0;
-  const T beta = first ? (T)0 : (T)(coherent_load(&ctl->rho) / coherent_load(&ctl->rho_prev));
+  if (ctl->done) return;
+  const int first = ctl->iter == 0;
+  const T beta = first ? (T)0 : (T)(ctl->rho / ctl->rho_prev);
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) {
     const T z = minv[i] * r[i];
     p[i] = first ? z : z + beta * p[i];
   }
 }
 
-// alpha = rho / p.q ; x += alpha p ; r -= alpha q ; rho' = r.(minv r)
+// x += alpha p ; r -= alpha q ; partials of r.(minv r)
 template <typename T>
 __global__ __launch_bounds__(VB) void k_cg_upd(T* __restrict__ x, T* __restrict__ r,
                                                const T* __restrict__ p, const T* __restrict__ q,
                                                const T* __restrict__ minv, int64_t n,
                                                double* part, CgCtl* ctl) {
-  if (coherent_load(&ctl->done)) return;
+  if (ctl->done) return;
   __shared__ double sh[VB / 64];
-  __shared__ int flag;
-  const double pq = coherent_load(&ctl->pq_st) + coherent_load(&ctl->pq_rows);
-  const double rho = coherent_load(&ctl->rho);
-  const T alpha = (T)(rho / pq);
+  const T alpha = (T)ctl->alpha;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) {
     x[i] += alpha * p[i];
@@ -389,29 +539,32 @@ __global__ __launch_bounds__(VB) void k_cg_upd(T* __restrict__ x, T* __restrict_
   }
   const double bs = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&ctl->cnt_upd, (int)gridDim.x, &flag)) {
-    const double tot = reduce_partials(part, (int)gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ctl->rho_prev = rho;
-      ctl->rho = tot;
-      const int it = ctl->iter + 1;
-      ctl->iter = it;
-      const int conv = sqrt(tot) < ctl->thr;
-      ctl->converged = conv;
-      ctl->done = conv || it >= ctl->maxiter;
-      ctl->cnt_upd = 0;
-    }
+}
+
+// one block: rho_prev = rho ; rho = sum(part) ; iter++ ; stopping test
+__global__ __launch_bounds__(VB) void k_cg_upd_fin(const double* __restrict__ part, int np,
+                                                   CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double sh[VB / 64];
+  const double tot = reduce_partials(part, np, sh);
+  if (threadIdx.x == 0) {
+    ctl->rho_prev = ctl->rho;
+    ctl->rho = tot;
+    const int it = ctl->iter + 1;
+    ctl->iter = it;
+    const int conv = sqrt(tot) < ctl->thr;
+    ctl->converged = conv;
+    ctl->done = conv || it >= ctl->maxiter;
   }
 }
 
-// r64 = rhs - q64 (q64 = N c) ; |r|_M^2 -> ctl->acc
+// r64 = rhs - q64 (q64 = N c) ; partials of |r|_M^2
 __global__ __launch_bounds__(VB) void k_residual(const double* __restrict__ rhs,
                                                  const double* __restrict__ q,
                                                  const double* __restrict__ minv,
                                                  double* __restrict__ r, int64_t n,
-                                                 double* part, CgCtl* ctl) {
+                                                 double* part) {
   __shared__ double sh[VB / 64];
-  __shared__ int flag;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += (int64_t)gridDim.x * VB) {
     const double rv = rhs[i] - q[i];
@@ -420,13 +573,55 @@ __global__ __launch_bounds__(VB) void k_residual(const double* __restrict__ rhs,
   }
   const double bs = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&ctl->cnt_misc, (int)gridDim.x, &flag)) {
-    const double tot = reduce_partials(part, (int)gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ctl->acc = tot;
-      ctl->cnt_misc = 0;
+}
+
+// one block: ctl->acc = sum(part)
+__global__ __launch_bounds__(VB) void k_sum_fin(const double* __restrict__ part, int np,
+                                                CgCtl* ctl) {
+  __shared__ double sh[VB / 64];
+  const double tot = reduce_partials(part, np, sh);
+  if (threadIdx.x == 0) ctl->acc = tot;
+}
+
+// Squared distance from every grid node (np.linspace coordinates) to the
+// nearest of m points; points staged through LDS in tiles.
+__global__ __launch_bounds__(256) void k_nearest_d2(GridDev g, const double* __restrict__ pts,
+                                                    int64_t m, double* __restrict__ d2) {
+  __shared__ double sp[3][256];
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int k = (int)(id / nxy);
+  const int64_t r = id - (int64_t)k * nxy;
+  const int j = (int)(r / g.nx);
+  const int i = (int)(r - (int64_t)j * g.nx);
+  // np.linspace(o, o + (n-1) h, n): x_i = i * ((max - o)/(n-1)) + o, last = max
+  auto lin = [](int i, int nn, double o, double h) {
+    const double mx = o + (double)(nn - 1) * h;
+    if (i == nn - 1) return mx;
+    const double st = (mx - o) / (double)(nn - 1);
+    return (double)i * st + o;
+  };
+  const double x = lin(i, g.nx, g.ox, g.hx), y = lin(j, g.ny, g.oy, g.hy),
+               z = lin(k, g.nz, g.oz, g.hz);
+  double best = __builtin_inf();
+  for (int64_t t0 = 0; t0 < m; t0 += 256) {
+    __syncthreads();
+    const int64_t q = t0 + threadIdx.x;
+    if (q < m) {
+      sp[0][threadIdx.x] = pts[3 * q];
+      sp[1][threadIdx.x] = pts[3 * q + 1];
+      sp[2][threadIdx.x] = pts[3 * q + 2];
+    }
+    __syncthreads();
+    const int cnt = (int)min((int64_t)256, m - t0);
+    for (int u = 0; u < cnt; ++u) {
+      const double dx = x - sp[0][u], dy = y - sp[1][u], dz = z - sp[2][u];
+      const double dd = (dx * dx + dy * dy) + dz * dz;
+      best = dd < best ? dd : best;
     }
   }
+  if (id < n) d2[id] = best;
 }
 
 __global__ void k_axpy_f32_to_f64(const float* __restrict__ d, double* __restrict__ c, int64_t n) {
@@ -449,9 +644,18 @@ struct lsamd_fd_system {
   lsamd_fd_reg reg;
   int64_t n = 0;
   double* scale2 = nullptr;
-  Shell shell;
-  int zc = 8, nbx = 0, nby = 0;
-  int64_t nbulk = 0, nst_blocks = 0;
+  // stencil plan
+  bool regular = false;   // class-table kernels usable
+  int nbx = 0, nby = 0;   // 64x8 xy tiles
+  int2* tiles_int = nullptr;
+  int2* tiles_edge = nullptr;
+  int n_int = 0, n_edge = 0;
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  Coef25<double> ku64;
+  Coef25<float> ku32;
+  int zc[2] = {0, 0};          // tuned z-chunk per precision (0: not tuned yet)
+  float tune_us[2][3] = {{0}};  // timings of the candidates 8, 16, 32
   // data rows
   int64_t nrows = 0, nnz = 0, ndn = 0;
   int64_t* row_ptr = nullptr;
@@ -481,6 +685,9 @@ struct lsamd_fd_system {
   CgCtl* ctl_apply = nullptr; // standalone apply (never done)
   CgCtl* ctl_host = nullptr;  // pinned mirror
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  const void* minv_graph[2] = {nullptr, nullptr};
+  double* ones64 = nullptr;
+  float* ones32 = nullptr;
   int graph_chunk[2] = {0, 0};
   std::vector<void*> owned;
 };
@@ -536,36 +743,118 @@ int vec_blocks(int64_t n) {
   return (int)b;
 }
 
-template <typename T>
-StencilArgs<T> stencil_args(const lsamd_fd_system* s) {
-  StencilArgs<T> a;
-  a.g = s->gd;
-  a.k = make_composite<T>(s->reg);
-  a.w = make_regw<T>(s->reg);
-  a.scale2 = s->scale2;
-  a.sh = s->shell;
-  a.nbx = s->nbx;
-  a.nby = s->nby;
-  a.zc = s->zc;
-  a.nbulk = s->nbulk;
-  return a;
+constexpr int kZcCand[3] = {8, 16, 32};
+
+int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
+  if (!s->regular) return ceil_div(s->n, 256);
+  const int64_t nbz = ceil_div(s->gd.nz - 6, zc);
+  return (int64_t)(s->n_int + s->n_edge) * nbz + 2 * (int64_t)s->nbx * s->nby;
 }
 
-// q = N x (stencil + rows), partial dots into ctl->pq_st / pq_rows
+template <typename T, int ZC>
+void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
+  const bool f32 = sizeof(T) == 4;
+  const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
+  Coef25<T> ku;
+  for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
+  const int64_t nbz = ceil_div(s->gd.nz - 6, ZC);
+  int64_t off = 0;
+  if (s->n_int > 0) {
+    hipLaunchKernelGGL((k_st_bulk<T, ZC, 4, true>), dim3((unsigned)(s->n_int * nbz)), dim3(NT), 0,
+                       st, x, y, s->gd, tab, ku, (const int2*)s->tiles_int, s->n_int, s->part, ctl);
+    off += s->n_int * nbz;
+  }
+  if (s->n_edge > 0) {
+    hipLaunchKernelGGL((k_st_bulk<T, ZC, 4, false>), dim3((unsigned)(s->n_edge * nbz)), dim3(NT),
+                       0, st, x, y, s->gd, tab, ku, (const int2*)s->tiles_edge, s->n_edge,
+                       s->part + off, ctl);
+    off += s->n_edge * nbz;
+  }
+  hipLaunchKernelGGL(k_st_zb<T>, dim3((unsigned)(2 * s->nbx * s->nby)), dim3(NT), 0, st, x, y,
+                     s->gd, tab, s->nbx, s->nby, s->part + off, ctl);
+}
+
+// Launch the regulariser stencil; returns the number of block partials.
+template <typename T>
+int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st,
+                       int zc) {
+  if (!s->regular) {
+    hipLaunchKernelGGL(k_st_general<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
+                       y, s->gd, make_regw<T>(s->reg), (const double*)s->scale2, s->part, ctl);
+    return ceil_div(s->n, 256);
+  }
+  switch (zc) {
+    case 8: launch_bulk<T, 8>(s, x, y, ctl, st); break;
+    case 16: launch_bulk<T, 16>(s, x, y, ctl, st); break;
+    default: launch_bulk<T, 32>(s, x, y, ctl, st); break;
+  }
+  return stencil_partials(s, zc);
+}
+
+// Autotuner: time each z-chunk candidate (HIP events) and keep the fastest.
+// LSAMD_STENCIL_ZC in the environment pins the choice.
+template <typename T>
+int ensure_tuned(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (s->zc[pi]) return LSAMD_OK;
+  if (!s->regular) {
+    s->zc[pi] = 1;
+    return LSAMD_OK;
+  }
+  if (const char* e = getenv("LSAMD_STENCIL_ZC")) {
+    const int v = atoi(e);
+    if (v == 8 || v == 16 || v == 32) {
+      s->zc[pi] = v;
+      return LSAMD_OK;
+    }
+  }
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestzc = 16;
+  for (int c = 0; c < 3; ++c) {
+    const int zc = kZcCand[c];
+    if (zc > 8 && s->gd.nz - 6 < zc) {  // a chunk longer than the bulk range gains nothing
+      s->tune_us[pi][c] = -1.f;
+      continue;
+    }
+    launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
+    LSAMD_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < 5; ++r) launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
+    LSAMD_HIP(hipEventRecord(e1, st));
+    LSAMD_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    s->tune_us[pi][c] = ms * 1e3f / 5;
+    if (ms < best) {
+      best = ms;
+      bestzc = zc;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("stencil autotune");
+  s->zc[pi] = bestzc;
+  return LSAMD_OK;
+}
+
+// q = N x (stencil + rows); the scatter kernel's block 0 reduces the two
+// partial-dot arrays into pq and alpha = rho / pq.
 template <typename T>
 int launch_apply(lsamd_fd_system* s, const T* x, T* y, T* t, CgCtl* ctl, hipStream_t st) {
   const bool f32 = sizeof(T) == 4;
   RowsDev R = rows_dev(s, f32);
-  if (s->nrows > 0) {
-    hipLaunchKernelGGL(k_rows_t<T>, dim3((unsigned)ceil_div(s->nrows, 256)), dim3(256), 0, st, x,
-                       R, t, s->part_rows, ctl);
+  const int nrb = s->nrows > 0 ? (int)ceil_div(s->nrows, 256) : 0;
+  if (nrb > 0) {
+    hipLaunchKernelGGL(k_rows_t<T>, dim3((unsigned)nrb), dim3(256), 0, st, x, R, t, s->part_rows,
+                       ctl);
   }
-  hipLaunchKernelGGL(k_stencil<T>, dim3((unsigned)s->nst_blocks), dim3(NT), 0, st, x, y,
-                     stencil_args<T>(s), s->part, ctl);
-  if (s->ndn > 0) {
-    hipLaunchKernelGGL((k_rows_scatter<T, T>), dim3((unsigned)ceil_div(s->ndn, 256)), dim3(256), 0,
-                       st, R, t, y, 1, ctl);
-  }
+  const int64_t nst = launch_stencil<T>(s, x, y, ctl, st, s->zc[f32 ? 1 : 0]);
+  const int nsb = (int)std::max<int64_t>(1, ceil_div(s->ndn, 256));
+  hipLaunchKernelGGL((k_rows_scatter<T, T>), dim3((unsigned)nsb), dim3(256), 0, st, R,
+                     (const T*)t, y, 1, ctl, 1, (const double*)s->part, (int)nst,
+                     (const double*)s->part_rows, nrb);
   LSAMD_CHECK_LAUNCH("apply");
   return LSAMD_OK;
 }
@@ -577,7 +866,19 @@ int launch_iteration(lsamd_fd_system* s, T* x, T* r, T* p, T* q, T* t, const T* 
   hipLaunchKernelGGL(k_cg_dir<T>, dim3(vb), dim3(VB), 0, st, r, minv, p, s->n, ctl);
   if (int rc = launch_apply<T>(s, p, q, t, ctl, st)) return rc;
   hipLaunchKernelGGL(k_cg_upd<T>, dim3(vb), dim3(VB), 0, st, x, r, p, q, minv, s->n, s->part, ctl);
+  hipLaunchKernelGGL(k_cg_upd_fin, dim3(1), dim3(VB), 0, st, (const double*)s->part, vb, ctl);
   LSAMD_CHECK_LAUNCH("cg iteration");
+  return LSAMD_OK;
+}
+
+template <typename T>
+int launch_init(lsamd_fd_system* s, const double* b, const T* minv, T* x, T* r, double rtol,
+                double atol, int maxiter, hipStream_t st) {
+  const int vb = vec_blocks(s->n);
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(vb), dim3(VB), 0, st, b, minv, x, r, s->n, s->part);
+  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(VB), 0, st, (const double*)s->part, vb, rtol,
+                     atol, maxiter, s->ctl);
+  LSAMD_CHECK_LAUNCH("cg init");
   return LSAMD_OK;
 }
 
@@ -588,12 +889,14 @@ int ensure_work(lsamd_fd_system* s, bool f32) {
     if (int rc = dalloc(s, &s->r64, n)) return rc;
     if (int rc = dalloc(s, &s->p64, n)) return rc;
     if (int rc = dalloc(s, &s->q64, n)) return rc;
+    for (double* v : {s->x64, s->r64, s->p64, s->q64}) LSAMD_HIP(hipMemset(v, 0, n * 8));
   }
   if (f32 && !s->x32) {
     if (int rc = dalloc(s, &s->x32, n)) return rc;
     if (int rc = dalloc(s, &s->r32, n)) return rc;
     if (int rc = dalloc(s, &s->p32, n)) return rc;
     if (int rc = dalloc(s, &s->q32, n)) return rc;
+    for (float* v : {s->x32, s->r32, s->p32, s->q32}) LSAMD_HIP(hipMemset(v, 0, n * 4));
   }
   return LSAMD_OK;
 }
@@ -665,14 +968,40 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
     if (int rc = dalloc(s, &s->scale2, s->n)) return bail(rc);
     LSAMD_HIP(hipMemcpy(s->scale2, scale2, s->n * sizeof(double), hipMemcpyDefault));
   }
-  s->shell = make_shell(s->gd, scale2 != nullptr);
-  if (!s->shell.all) {
+  s->regular = scale2 == nullptr && s->gd.nx >= 7 && s->gd.ny >= 7 && s->gd.nz >= 7;
+  if (s->regular) {
     s->nbx = (int)ceil_div(s->gd.nx, TX);
     s->nby = (int)ceil_div(s->gd.ny, TY);
-    s->nbulk = (int64_t)s->nbx * s->nby * ceil_div(s->gd.nz, s->zc);
+    std::vector<int2> ti, te;
+    for (int j = 0; j < s->nby; ++j)
+      for (int i = 0; i < s->nbx; ++i) {
+        const bool inter = i * TX >= 3 && i * TX + TX - 1 <= s->gd.nx - 4 && j * TY >= 3 &&
+                           j * TY + TY - 1 <= s->gd.ny - 4;
+        (inter ? ti : te).push_back(make_int2(i, j));
+      }
+    s->n_int = (int)ti.size();
+    s->n_edge = (int)te.size();
+    if (int rc = dalloc(s, &s->tiles_int, std::max<int64_t>(1, s->n_int))) return bail(rc);
+    if (int rc = dalloc(s, &s->tiles_edge, std::max<int64_t>(1, s->n_edge))) return bail(rc);
+    if (s->n_int)
+      LSAMD_HIP(hipMemcpy(s->tiles_int, ti.data(), ti.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (s->n_edge)
+      LSAMD_HIP(hipMemcpy(s->tiles_edge, te.data(), te.size() * sizeof(int2),
+                          hipMemcpyHostToDevice));
+    std::vector<double> tab(kClasses * kTaps);
+    build_class_table(s->reg, tab.data());
+    std::vector<float> tabf(tab.begin(), tab.end());
+    if (int rc = dalloc(s, &s->tab64, kClasses * kTaps)) return bail(rc);
+    if (int rc = dalloc(s, &s->tab32, kClasses * kTaps)) return bail(rc);
+    LSAMD_HIP(hipMemcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    LSAMD_HIP(hipMemcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
+    const int bulk = ((3 * 7 + 3) * 7 + 3) * kTaps;
+    for (int q = 0; q < kTaps; ++q) {
+      s->ku64.k[q] = tab[bulk + q];
+      s->ku32.k[q] = (float)tab[bulk + q];
+    }
   }
-  s->nst_blocks = s->nbulk + ceil_div(s->shell.count, NT);
-  s->part_n = std::max<int64_t>(s->nst_blocks, 8192);
+  s->part_n = std::max<int64_t>(stencil_partials(s, 8), 8192);
   if (int rc = dalloc(s, &s->part, s->part_n)) return bail(rc);
   if (int rc = dalloc(s, &s->ctl, 1)) return bail(rc);
   if (int rc = dalloc(s, &s->ctl_apply, 1)) return bail(rc);
@@ -832,7 +1161,8 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     hipLaunchKernelGGL(k_w2b, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, st, s->w2_64,
                        s->b, s->t64, nrows);
     hipLaunchKernelGGL((k_rows_scatter<double, double>), dim3((unsigned)ceil_div(s->ndn, 256)),
-                       dim3(256), 0, st, R, s->t64, s->rhs, 1, (CgCtl*)nullptr);
+                       dim3(256), 0, st, R, (const double*)s->t64, s->rhs, 1, (CgCtl*)nullptr, 0,
+                       (const double*)nullptr, 0, (const double*)nullptr, 0);
     hipLaunchKernelGGL(k_rows_diag, dim3((unsigned)ceil_div(s->ndn, 256)), dim3(256), 0, st, R,
                        s->coef64, s->w2_64, s->diag);
   }
@@ -848,6 +1178,14 @@ int lsamd_fd_system_apply(lsamd_fd_system* s, int32_t precision, const void* x, 
   if (int rc = injected("apply")) return rc;
   if (!s || !x || !y) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_apply: NULL argument");
   hipStream_t st = (hipStream_t)stream;
+  if (precision != LSAMD_FP64 && precision != LSAMD_FP32)
+    return fail(LSAMD_ERR_ARG, "unknown precision");
+  if (int rc = ensure_work(s, precision == LSAMD_FP32)) return rc;
+  if (precision == LSAMD_FP64) {
+    if (int rc = ensure_tuned<double>(s, s->p64, s->q64, st)) return rc;
+  } else {
+    if (int rc = ensure_tuned<float>(s, s->p32, s->q32, st)) return rc;
+  }
   if (precision == LSAMD_FP64) {
     if (int rc = launch_apply<double>(s, (const double*)x, (double*)y, s->t64, s->ctl_apply, st))
       return rc;
@@ -881,6 +1219,29 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = s->n;
   const double* b = rhs ? rhs : s->rhs;
+  // preconditioner: Jacobi (= the reference's column scaling) or identity
+  const double* minv64 = s->minv64;
+  const float* minv32 = s->minv32;
+  if (!o->jacobi) {
+    if (!s->ones64) {
+      if (int rc = dalloc(s, &s->ones64, n)) return rc;
+      if (int rc = dalloc(s, &s->ones32, n)) return rc;
+      hipLaunchKernelGGL(k_fill_one, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                         s->ones64, s->ones32, n);
+      LSAMD_CHECK_LAUNCH("k_fill_one");
+    }
+    minv64 = s->ones64;
+    minv32 = s->ones32;
+  }
+  if (s->minv_graph[0] != minv64 || s->minv_graph[1] != (const void*)minv32) {
+    for (int i = 0; i < 2; ++i)
+      if (s->graph[i]) {
+        hipGraphExecDestroy(s->graph[i]);
+        s->graph[i] = nullptr;
+      }
+    s->minv_graph[0] = minv64;
+    s->minv_graph[1] = minv32;
+  }
   const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * n;
   const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
   const int chunk = o->check_every > 0 ? o->check_every : 64;
@@ -889,10 +1250,10 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
   memset(&inf, 0, sizeof(inf));
   if (o->precision == LSAMD_FP64) {
     if (int rc = ensure_work(s, false)) return rc;
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(vb), dim3(VB), 0, st, b, s->minv64, s->x64,
-                       s->r64, n, o->rtol, o->atol, maxiter, s->part, s->ctl);
-    LSAMD_CHECK_LAUNCH("cg init");
-    if (int rc = run_pcg<double>(s, chunk, s->x64, s->r64, s->p64, s->q64, s->t64, s->minv64, st))
+    if (int rc = ensure_tuned<double>(s, s->p64, s->q64, st)) return rc;
+    if (int rc = launch_init<double>(s, b, minv64, s->x64, s->r64, o->rtol, o->atol, maxiter, st))
+      return rc;
+    if (int rc = run_pcg<double>(s, chunk, s->x64, s->r64, s->p64, s->q64, s->t64, minv64, st))
       return rc;
     hipLaunchKernelGGL(k_copy_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->x64, c, n);
     LSAMD_HIP(hipStreamSynchronize(st));
@@ -904,12 +1265,14 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
     // FP64 outer refinement around FP32 inner PCG solves of N d = r.
     if (int rc = ensure_work(s, false)) return rc;
     if (int rc = ensure_work(s, true)) return rc;
+    if (int rc = ensure_tuned<double>(s, s->p64, s->q64, st)) return rc;
+    if (int rc = ensure_tuned<float>(s, s->p32, s->q32, st)) return rc;
     const double inner = o->inner_rtol > 0 ? o->inner_rtol : 1e-4;
     const int64_t max_outer = o->max_outer > 0 ? o->max_outer : 50;
     hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, c, n);
     // threshold from |S rhs| (same test as the FP64 form)
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(vb), dim3(VB), 0, st, b, s->minv64, s->x64, s->r64,
-                       n, o->rtol, o->atol, maxiter, s->part, s->ctl);
+    if (int rc = launch_init<double>(s, b, minv64, s->x64, s->r64, o->rtol, o->atol, maxiter, st))
+      return rc;
     LSAMD_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, st));
     LSAMD_HIP(hipStreamSynchronize(st));
     const double thr = s->ctl_host->thr;
@@ -920,10 +1283,9 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
       // inner: r64 already holds the residual; run FP32 PCG on it
       const int left = (int)std::min<int64_t>(maxiter64 - total, 2000000000LL);
       const double irtol = inner;
-      hipLaunchKernelGGL(k_cg_init<float>, dim3(vb), dim3(VB), 0, st, s->r64, s->minv32, s->x32,
-                         s->r32, n, irtol, 0.0, left, s->part, s->ctl);
-      LSAMD_CHECK_LAUNCH("inner init");
-      if (int rc = run_pcg<float>(s, chunk, s->x32, s->r32, s->p32, s->q32, s->t32, s->minv32, st))
+      if (int rc = launch_init<float>(s, s->r64, minv32, s->x32, s->r32, irtol, 0.0, left, st))
+        return rc;
+      if (int rc = run_pcg<float>(s, chunk, s->x32, s->r32, s->p32, s->q32, s->t32, minv32, st))
         return rc;
       total += s->ctl_host->iter;
       ++outer;
@@ -931,8 +1293,10 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
                          s->x32, c, n);
       // r64 = rhs - N c in FP64
       if (int rc = launch_apply<double>(s, c, s->q64, s->t64, s->ctl_apply, st)) return rc;
-      hipLaunchKernelGGL(k_residual, dim3(vb), dim3(VB), 0, st, b, s->q64, s->minv64, s->r64, n,
-                         s->part, s->ctl_apply);
+      hipLaunchKernelGGL(k_residual, dim3(vb), dim3(VB), 0, st, b, s->q64, minv64, s->r64, n,
+                         s->part);
+      hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(VB), 0, st, (const double*)s->part, vb,
+                         s->ctl_apply);
       LSAMD_HIP(hipMemcpyAsync(s->ctl_host, s->ctl_apply, sizeof(CgCtl), hipMemcpyDeviceToHost,
                                st));
       LSAMD_HIP(hipStreamSynchronize(st));
@@ -948,6 +1312,62 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
     return fail(LSAMD_ERR_ARG, "unknown precision");
   }
   if (info) *info = inf;
+  return LSAMD_OK;
+}
+
+int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
+                                 double* avg_ms, void* stream) {
+  if (!s || !avg_ms || iters < 1) return fail(LSAMD_ERR_ARG, "time_stencil: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const bool f32 = precision == LSAMD_FP32;
+  if (int rc = ensure_work(s, f32)) return rc;
+  if (f32) {
+    if (int rc = ensure_tuned<float>(s, s->p32, s->q32, st)) return rc;
+  } else {
+    if (int rc = ensure_tuned<double>(s, s->p64, s->q64, st)) return rc;
+  }
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  double total = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    LSAMD_HIP(hipEventRecord(e0, st));
+    if (f32)
+      launch_stencil<float>(s, s->p32, s->q32, s->ctl_apply, st, s->zc[1]);
+    else
+      launch_stencil<double>(s, s->p64, s->q64, s->ctl_apply, st, s->zc[0]);
+    LSAMD_HIP(hipEventRecord(e1, st));
+    LSAMD_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    total += ms;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("time_stencil");
+  *avg_ms = total / iters;
+  return LSAMD_OK;
+}
+
+int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,
+                                 float* tune_us) {
+  if (!s || !zc) return fail(LSAMD_ERR_ARG, "stencil_info: NULL argument");
+  const int pi = precision == LSAMD_FP32 ? 1 : 0;
+  *zc = s->regular ? s->zc[pi] : 0;
+  if (tune_us)
+    for (int c = 0; c < 3; ++c) tune_us[c] = s->tune_us[pi][c];
+  return LSAMD_OK;
+}
+
+int lsamd_fd_nearest_d2(const lsamd_grid* g, const double* pts, int64_t m, double* d2,
+                        void* stream) {
+  if (int rc = injected("nearest_d2")) return rc;
+  if (int rc = check_grid(g)) return rc;
+  if (m < 0 || !d2 || (m > 0 && !pts)) return fail(LSAMD_ERR_ARG, "lsamd_fd_nearest_d2: bad args");
+  const int64_t n = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  hipLaunchKernelGGL(k_nearest_d2, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, to_dev(*g), pts, m, d2);
+  LSAMD_CHECK_LAUNCH("k_nearest_d2");
   return LSAMD_OK;
 }
 

@@ -345,10 +345,59 @@ __global__ void k_evaluate(GridDev g, const double* __restrict__ c, const double
   }
 }
 
+// numpy's pairwise summation of a contiguous run of n <= 128 doubles
+// (n < 8: sequential from 0; else eight interleaved accumulators).
+__device__ inline double np_sum(const double* a, int n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  double r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += a[i];
+  return res;
+}
+
+__global__ void k_rows_normalise(int64_t nrows, int k, double* __restrict__ coef,
+                                 double* __restrict__ b, double* __restrict__ norm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  double sq[128];
+  double* a = coef + r * k;
+  for (int j = 0; j < k; ++j) sq[j] = a[j] * a[j];
+  const double len = sqrt(np_sum(sq, k));
+  if (len > 0) {
+    b[r] /= len;
+    for (int j = 0; j < k; ++j) a[j] /= len;
+  }
+  if (norm) norm[r] = len;
+}
+
 }  // namespace
 }  // namespace lsamd
 
 using namespace lsamd;
+
+extern "C" int lsamd_rows_normalise(int64_t nrows, int32_t k, double* coef, double* b,
+                                    double* norm, void* stream) {
+  if (int rc = injected("rows_normalise")) return rc;
+  if (nrows < 0 || k < 1 || k > 128)
+    return fail(LSAMD_ERR_ARG, "lsamd_rows_normalise: need nrows >= 0 and 1 <= k <= 128");
+  if (nrows == 0) return LSAMD_OK;
+  if (!coef || !b) return fail(LSAMD_ERR_ARG, "lsamd_rows_normalise: NULL input");
+  const int bs = 64;
+  hipLaunchKernelGGL(k_rows_normalise, dim3((unsigned)ceil_div(nrows, bs)), dim3(bs), 0,
+                     (hipStream_t)stream, nrows, (int)k, coef, b, norm);
+  LSAMD_CHECK_LAUNCH("k_rows_normalise");
+  return LSAMD_OK;
+}
 
 extern "C" int lsamd_fd_locate(const lsamd_grid* g, const double* pts, int64_t n, int64_t* idc,
                                double* w, double* T, uint8_t* inside, void* stream) {

@@ -25,31 +25,6 @@ struct RegW {
   T xx, yy, zz, xy, xz, yz, bx, by, bz, ridge;
 };
 
-// 25-tap composite kernel K = sum_f W_f autocorr(d_f) + ridge2 delta, valid
-// at nodes whose distance to every face is >= 3 (all rows around them are
-// interior rows of uniform weight).  Taps: centre, +-1 and +-2 on each axis,
-// (+-2,+-2) in each coordinate plane.
-template <typename T>
-struct Composite {
-  T c0, cx1, cy1, cz1, cx2, cy2, cz2, cxy, cxz, cyz;
-};
-
-template <typename T>
-inline Composite<T> make_composite(const lsamd_fd_reg& r) {
-  Composite<T> k;
-  k.c0 = (T)(r.w2_xx + r.w2_yy + r.w2_zz + r.w2_xy + r.w2_xz + r.w2_yz + r.ridge2);
-  k.cx1 = (T)(-2.0 / 3.0 * r.w2_xx);
-  k.cy1 = (T)(-2.0 / 3.0 * r.w2_yy);
-  k.cz1 = (T)(-2.0 / 3.0 * r.w2_zz);
-  k.cx2 = (T)(r.w2_xx / 6.0 - 0.5 * r.w2_xy - 0.5 * r.w2_xz);
-  k.cy2 = (T)(r.w2_yy / 6.0 - 0.5 * r.w2_xy - 0.5 * r.w2_yz);
-  k.cz2 = (T)(r.w2_zz / 6.0 - 0.5 * r.w2_xz - 0.5 * r.w2_yz);
-  k.cxy = (T)(0.25 * r.w2_xy);
-  k.cxz = (T)(0.25 * r.w2_xz);
-  k.cyz = (T)(0.25 * r.w2_yz);
-  return k;
-}
-
 template <typename T>
 inline RegW<T> make_regw(const lsamd_fd_reg& r) {
   RegW<T> w;
@@ -194,64 +169,90 @@ __device__ inline double general_diag(const GridDev& g, const RegW<double>& w,
   return y;
 }
 
-// Boundary shell = nodes at distance <= 2 from a face; the bulk (composite
-// kernel) covers i in [3, n-4] on every axis.
-struct Shell {
-  int64_t count;
-  int mx, my, mz;  // bulk extent per axis (n - 6), <= 0 means no bulk
-  bool all;        // every node handled by the general formula
-};
+// ------------------------------------------------------- class table ----
+// On a grid with >= 7 nodes per axis, the operator row of node j depends on
+// j only through the "class" of each coordinate: 0,1,2 (distance to the low
+// face), 3 (distance >= 3 from both faces), 4,5,6 (n-3, n-2, n-1).  A node's
+// 25 coefficients are therefore one of 7^3 sets, computed exactly on a
+// virtual 7x7x7 grid whose node (cx,cy,cz) has the same row layout around it.
+// Tap order: in-plane 0:(0,0) 1:(+1,0) 2:(-1,0) 3:(0,+1) 4:(0,-1) 5:(+2,0)
+// 6:(-2,0) 7:(0,+2) 8:(0,-2) 9:(+2,+2) 10:(-2,+2) 11:(+2,-2) 12:(-2,-2);
+// 13:(0,0,-1) 14:(0,0,+1); 15..19 plane z-2: (0,0),(+2,0),(-2,0),(0,+2),
+// (0,-2); 20..24 plane z+2: same.  Taps reaching outside the grid have
+// coefficient 0 (every row touching them is absent).
+constexpr int kTaps = 25;
+constexpr int kClasses = 343;
 
-inline Shell make_shell(const GridDev& g, bool force_all) {
-  Shell s;
-  s.mx = g.nx - 6;
-  s.my = g.ny - 6;
-  s.mz = g.nz - 6;
-  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-  s.all = force_all || s.mx <= 0 || s.my <= 0 || s.mz <= 0;
-  s.count = s.all ? n : n - (int64_t)s.mx * s.my * s.mz;
-  return s;
-}
+__host__ __device__ inline int cls7(int i, int n) { return i < 3 ? i : (i > n - 4 ? i - (n - 7) : 3); }
 
-// shell index -> node (i,j,k)
-__device__ inline void shell_decode(const GridDev& g, const Shell& s, int64_t t, int& i, int& j,
-                                    int& k) {
-  const int64_t nxy = (int64_t)g.nx * g.ny;
-  if (s.all) {
-    k = (int)(t / nxy);
-    const int64_t r = t - (int64_t)k * nxy;
-    j = (int)(r / g.nx);
-    i = (int)(r - (int64_t)j * g.nx);
-    return;
-  }
-  if (t < 3 * nxy) {
-    k = (int)(t / nxy);
-  } else if (t < 6 * nxy) {
-    t -= 3 * nxy;
-    k = g.nz - 3 + (int)(t / nxy);
-  } else {
-    t -= 6 * nxy;
-    const int64_t ring = nxy - (int64_t)s.mx * s.my;
-    k = 3 + (int)(t / ring);
-    int64_t r = t - (int64_t)(k - 3) * ring;
-    if (r < 3 * (int64_t)g.nx) {
-      j = (int)(r / g.nx);
-      i = (int)(r - (int64_t)j * g.nx);
-    } else if (r < 6 * (int64_t)g.nx) {
-      r -= 3 * (int64_t)g.nx;
-      j = g.ny - 3 + (int)(r / g.nx);
-      i = (int)(r - (int64_t)(j - (g.ny - 3)) * g.nx);
-    } else {
-      r -= 6 * (int64_t)g.nx;
-      j = 3 + (int)(r / 6);
-      const int c = (int)(r - (int64_t)(j - 3) * 6);
-      i = c < 3 ? c : g.nx - 6 + c;
+inline void build_class_table(const lsamd_fd_reg& r, double* tab) {
+  static const int taps[kTaps][3] = {
+      {0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},  {2, 0, 0},  {-2, 0, 0},
+      {0, 2, 0},  {0, -2, 0}, {2, 2, 0},  {-2, 2, 0}, {2, -2, 0},  {-2, -2, 0}, {0, 0, -1},
+      {0, 0, 1},  {0, 0, -2}, {2, 0, -2}, {-2, 0, -2}, {0, 2, -2}, {0, -2, -2}, {0, 0, 2},
+      {2, 0, 2},  {-2, 0, 2}, {0, 2, 2},  {0, -2, 2}};
+  struct Off {
+    int d[3];
+    double c;
+  };
+  Off fam[6][4];
+  int nfam[6] = {3, 3, 3, 4, 4, 4};
+  const double s6 = 0.40824829046386301637;  // 1/sqrt(6)
+  for (int a = 0; a < 3; ++a)
+    for (int t = -1; t <= 1; ++t) {
+      Off& o = fam[a][t + 1];
+      o.d[0] = o.d[1] = o.d[2] = 0;
+      o.d[a] = t;
+      o.c = (t == 0 ? -2.0 : 1.0) * s6;
     }
-    return;
+  const int pl[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int f = 0; f < 3; ++f) {
+    int q = 0;
+    for (int u = -1; u <= 1; u += 2)
+      for (int v = -1; v <= 1; v += 2) {
+        Off& o = fam[3 + f][q++];
+        o.d[0] = o.d[1] = o.d[2] = 0;
+        o.d[pl[f][0]] = u;
+        o.d[pl[f][1]] = v;
+        o.c = (u * v > 0) ? -0.5 : 0.5;
+      }
   }
-  const int64_t r = t - (t / nxy) * nxy;
-  j = (int)(r / g.nx);
-  i = (int)(r - (int64_t)j * g.nx);
+  // squared row weight of family f centred at (i,j,k) of the 7-grid
+  auto W = [&](int f, int i, int j, int k) {
+    if (i < 0 || i > 6 || j < 0 || j > 6 || k < 0 || k > 6) return 0.0;
+    const bool I = i >= 1 && i <= 5 && j >= 1 && j <= 5 && k >= 1 && k <= 5;
+    switch (f) {
+      case 0: return (I ? r.w2_xx : 0.0) + (i == 1 ? r.w2_bx : 0.0) + (i == 5 ? r.w2_bx : 0.0);
+      case 1: return (I ? r.w2_yy : 0.0) + (j == 1 ? r.w2_by : 0.0) + (j == 5 ? r.w2_by : 0.0);
+      case 2: return (I ? r.w2_zz : 0.0) + (k == 1 ? r.w2_bz : 0.0) + (k == 5 ? r.w2_bz : 0.0);
+      case 3: return I ? r.w2_xy : 0.0;
+      case 4: return I ? r.w2_xz : 0.0;
+      default: return I ? r.w2_yz : 0.0;
+    }
+  };
+  for (int cz = 0; cz < 7; ++cz)
+    for (int cy = 0; cy < 7; ++cy)
+      for (int cx = 0; cx < 7; ++cx) {
+        double* out = tab + ((cz * 7 + cy) * 7 + cx) * kTaps;
+        for (int t = 0; t < kTaps; ++t) {
+          const int* dd = taps[t];
+          double acc = (t == 0) ? r.ridge2 : 0.0;
+          for (int f = 0; f < 6; ++f)
+            for (int a = 0; a < nfam[f]; ++a) {
+              // row centred at j - o1 holds column j at offset o1 and column
+              // j + dd at offset o1 + dd
+              const Off& o1 = fam[f][a];
+              double c2 = 0.0;
+              for (int b = 0; b < nfam[f]; ++b)
+                if (fam[f][b].d[0] == o1.d[0] + dd[0] && fam[f][b].d[1] == o1.d[1] + dd[1] &&
+                    fam[f][b].d[2] == o1.d[2] + dd[2])
+                  c2 = fam[f][b].c;
+              if (c2 == 0.0) continue;
+              acc += o1.c * W(f, cx - o1.d[0], cy - o1.d[1], cz - o1.d[2]) * c2;
+            }
+          out[t] = acc;
+        }
+      }
 }
 
 }  // namespace lsamd

@@ -94,32 +94,18 @@ __device__ inline int coherent_load(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// "Last block done": every block publishes its partial, then one ticket is
-// drawn; the block holding the final ticket sees every partial and reduces
-// them in index order (deterministic).  Returns true in all threads of the
-// last block.  `counter` is reset to 0 by the last block.
-__device__ inline bool last_block(int* counter, int nblocks, int* sh_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = atomicAdd(counter, 1);
-    *sh_flag = (t == nblocks - 1);
-  }
-  __syncthreads();
-  bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
-}
-
-// Sum `n` partials in fixed order with the whole block; result in thread 0.
+// Sum `n` partials (written by an EARLIER kernel of the stream, so plain
+// loads see them) in a fixed order with the whole block; result in thread 0.
+// Cross-workgroup reductions always go through a kernel boundary: an
+// agent-scope fence inside a kernel writes back and invalidates the L2 of
+// the XCD on gfx950, which costs far more than one extra launch.
 __device__ inline double reduce_partials(const double* part, int n, double* sh) {
   double v = 0.0;
-  // each thread sums a contiguous, fixed chunk
   const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
   const int b = threadIdx.x * per;
   for (int i = 0; i < per; ++i) {
     int k = b + i;
-    if (k < n) v += coherent_load(part + k);
+    if (k < n) v += part[k];
   }
   return block_sum(v, sh);
 }

@@ -153,7 +153,7 @@ class FdSystem:
         return out
 
     def solve(self, rtol=1e-5, atol=0.0, maxiter=None, precision="fp64", rhs=None,
-              check_every=64, inner_rtol=1e-4, max_outer=50):
+              check_every=64, inner_rtol=1e-4, max_outer=50, jacobi=True):
         o = _lib.SolveOpts()
         o.rtol = float(rtol)
         o.atol = float(atol)
@@ -162,6 +162,7 @@ class FdSystem:
         o.check_every = int(check_every)
         o.inner_rtol = float(inner_rtol)
         o.max_outer = int(max_outer)
+        o.jacobi = 1 if jacobi else 0
         info = _lib.SolveInfo()
         c = torch.empty(self.n, dtype=F64, device=self.dev)
         if rhs is not None:
@@ -171,6 +172,23 @@ class FdSystem:
         return c, {"iterations": int(info.iterations), "outer": int(info.outer),
                    "converged": bool(info.converged), "residual": float(info.residual),
                    "threshold": float(info.threshold)}
+
+    def time_stencil(self, precision="fp64", iters=20) -> float:
+        """Mean duration (ms) of the stencil kernel alone, HIP events on the
+        current stream."""
+        ms = C.c_double(0.0)
+        _lib.call("lsamd_fd_system_time_stencil", self._h,
+                  _lib.FP32 if precision == "fp32" else _lib.FP64, int(iters), C.byref(ms),
+                  _lib.stream())
+        return float(ms.value)
+
+    def stencil_info(self, precision="fp64"):
+        """Autotuned z-chunk (0: per-node kernel) and candidate timings (us)."""
+        zc = C.c_int32(0)
+        us = (C.c_float * 3)()
+        _lib.call("lsamd_fd_system_stencil_info", self._h,
+                  _lib.FP32 if precision == "fp32" else _lib.FP64, C.byref(zc), us)
+        return {"zc": int(zc.value), "candidates_us": {8: us[0], 16: us[1], 32: us[2]}}
 
     def adjoint_rows(self, c: torch.Tensor, lam: torch.Tensor):
         R = self.rows.nrows if self.rows is not None else 0

@@ -1,0 +1,991 @@
+"""Drop-in interpolators: ``FiniteDifferenceInterpolator`` on the MI355X.
+
+Same class names, method names, keyword arguments, attributes and error
+behaviour as the reference (LI/_geological_interpolator.py,
+LI/_discrete_interpolator.py, LI/_finite_difference_interpolator.py); the
+numerical work (point location, row assembly and normalisation, the
+matrix-free regularised normal operator, the PCG solve and evaluation) runs in
+the HIP kernels of libloopstructural_amd.  Constraint rows live in HBM; the
+regulariser is never materialised.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import logging
+from collections import OrderedDict, defaultdict
+from enum import Enum
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import constraints as cons
+from . import fd_system as fs
+from .diagnostics import (ConstraintDiagnosticsReport, ConstraintFamilyDiagnostics,
+                          RegionCoverageDiagnostics)
+from .supports import Operator
+
+logger = logging.getLogger(__name__)
+
+ValidationError = cons.ValidationError
+
+
+class InterpolatorType(Enum):
+    """LI/_interpolatortype.py."""
+
+    BASE = "BASE"
+    BASE_DISCRETE = "BASE_DISCRETE"
+    FINITE_DIFFERENCE = "FINITE_DIFFERENCE"
+    DISCRETE_FOLD = "DISCRETE_FOLD"
+    PIECEWISE_LINEAR = "PIECEWISE_LINEAR"
+    PIECEWISE_QUADRATIC = "PIECEWISE_QUADRATIC"
+    BASE_DATA_SUPPORTED = "BASE_DATA_SUPPORTED"
+    SURFE = "SURFE"
+    PIECEWISE_LINEAR_CONSTANT_NORM = "PIECEWISE_LINEAR_CONSTANT_NORM"
+    FINITE_DIFFERENCE_CONSTANT_NORM = "FINITE_DIFFERENCE_CONSTANT_NORM"
+
+
+class _Constraint(dict):
+    """``constraints[name]`` entry: {"matrix", "b", "w"} with device tensors
+    converted to numpy on access (LI/_discrete_interpolator.py:383-389)."""
+
+    def __getitem__(self, key):
+        v = dict.__getitem__(self, key)
+        if isinstance(v, torch.Tensor):
+            return v.detach().cpu().numpy()
+        return v
+
+    def get(self, key, default=None):
+        return self[key] if key in self else default
+
+
+class StencilFamily:
+    """A regulariser family kept matrix-free: rows of one normalised mask at a
+    set of centre nodes (interior nodes, or one border layer)."""
+
+    def __init__(self, name, kind, weight, nrows, dof):
+        self.name, self.kind, self.weight, self.nrows, self.dof = name, kind, weight, nrows, dof
+
+    @property
+    def shape(self):
+        return (self.nrows, self.dof)
+
+
+def _mask_kind(mask):
+    """Map a (3,3,3) mask to the canonical family it equals after row
+    normalisation (LI/_discrete_interpolator.py:361-366), else None."""
+    m = np.asarray(mask, dtype=float)
+    if m.shape != (3, 3, 3):
+        return None
+    nm = np.linalg.norm(m[m != 0]) if np.any(m != 0) else 0.0
+    if nm == 0:
+        return None
+    for kind, ref in (("xx", Operator.Dxx_mask), ("yy", Operator.Dyy_mask),
+                      ("zz", Operator.Dzz_mask), ("xy", Operator.Dxy_mask),
+                      ("xz", Operator.Dxz_mask), ("yz", Operator.Dyz_mask)):
+        r = np.asarray(ref, dtype=float)
+        r = r / np.linalg.norm(r[r != 0])
+        if np.allclose(m / nm, r, atol=1e-12) or np.allclose(m / nm, -r, atol=1e-12):
+            return kind
+    return None
+
+
+class GeologicalInterpolator:
+    """LI/_geological_interpolator.py:40-897 (data handling, projection,
+    evaluation entry points, serialisation)."""
+
+    def __init__(self, data=None, up_to_date=False):
+        self._data = {}
+        self.data = data
+        self.clean()
+        self.n_g = self.n_i = self.n_n = self.n_t = 0
+        self.type = InterpolatorType.BASE
+        self.up_to_date = up_to_date
+        self.constraints = []
+        self.valid = True
+        self.dimensions = 3
+        self.support = None
+        self.bounding_box = None
+        self.latest_diagnostics_report = None
+
+    @property
+    def data(self):
+        return self._data
+
+    @data.setter
+    def data(self, data):
+        if data is None:
+            data = {}
+        for k, v in data.items():
+            self._data[k] = np.array(v)
+
+    def clean(self):
+        """LI/_geological_interpolator.py:864-885."""
+        self.data = {"gradient": np.zeros((0, 7)), "value": np.zeros((0, 5)),
+                     "normal": np.zeros((0, 7)), "tangent": np.zeros((0, 7)),
+                     "interface": np.zeros((0, 5)), "inequality": np.zeros((0, 6)),
+                     "inequality_pairs": np.zeros((0, 4))}
+        self.up_to_date = False
+        self.n_g = self.n_i = self.n_n = self.n_t = 0
+
+    def __str__(self):
+        return (f"{self.type} \n{self.n_g} gradient points\n{self.n_i} interface points\n"
+                f"{self.n_n} normal points\n{self.n_t} tangent points\n"
+                f"{self.n_g + self.n_i + self.n_n + self.n_t} total points\n")
+
+    # -------------------------------------------------------- projection --
+    def _project_points(self, points):
+        return points if self.bounding_box is None else self.bounding_box.project(points)
+
+    def _reproject_vectors(self, vectors):
+        if self.bounding_box is None:
+            return vectors
+        return self.bounding_box.reproject_vectors(vectors)
+
+    def _project_constraint_array(self, array, has_vector=False):
+        """LI/_geological_interpolator.py:273-286."""
+        if self.bounding_box is None or array.shape[0] == 0:
+            return array
+        d = self.dimensions
+        array = array.copy()
+        array[:, :d] = self.bounding_box.project(array[:, :d])
+        if has_vector:
+            array[:, d:2 * d] = self.bounding_box.project_vectors(array[:, d:2 * d])
+        return array
+
+    # ------------------------------------------------------------ set_* --
+    def set_value_constraints(self, points):
+        """(N,4|5) [x,y,z,value(,w)] (LI/_geological_interpolator.py:303-333)."""
+        try:
+            pts = self._project_constraint_array(cons.value_array(points, self.dimensions))
+            self.data["value"] = pts.copy()
+            self.n_i = pts.shape[0]
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set value constraints: {e}") from e
+
+    def set_gradient_constraints(self, points):
+        """(N,6|7) [x,y,z,gx,gy,gz(,w)] (:335-367)."""
+        try:
+            pts = self._project_constraint_array(cons.gradient_array(points, self.dimensions),
+                                                 has_vector=True)
+            self.n_g = pts.shape[0]
+            self.data["gradient"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set gradient constraints: {e}") from e
+
+    def set_normal_constraints(self, points):
+        """(N,6|7) [x,y,z,nx,ny,nz(,w)] (:369-400)."""
+        try:
+            pts = self._project_constraint_array(
+                cons.gradient_array(points, self.dimensions, is_normal=True), has_vector=True)
+            self.n_n = pts.shape[0]
+            self.data["normal"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set normal constraints: {e}") from e
+
+    def set_tangent_constraints(self, points):
+        """(N,6|7) [x,y,z,tx,ty,tz(,w)] (:402-433)."""
+        try:
+            pts = self._project_constraint_array(cons.gradient_array(points, self.dimensions),
+                                                 has_vector=True)
+            self.n_t = pts.shape[0]
+            self.data["tangent"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set tangent constraints: {e}") from e
+
+    def set_interface_constraints(self, points):
+        """(N,4|5) [x,y,z,id(,w)] (:435-462)."""
+        try:
+            pts = self._project_constraint_array(cons.interface_array(points, self.dimensions))
+            self.data["interface"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set interface constraints: {e}") from e
+
+    def set_value_inequality_constraints(self, points):
+        """Stored for API compatibility; the inequality (ADMM) solver is not
+        part of this package and such constraints are ignored with a warning."""
+        pts = np.asarray(points, dtype=float)
+        if pts.ndim != 2 or pts.shape[1] not in (5, 6):
+            raise cons.ShapeError("Inequality constraint array must have 5 or 6 columns")
+        if pts.shape[1] == 5:
+            pts = np.hstack([pts, np.ones((pts.shape[0], 1))])
+        self.data["inequality"] = pts.copy()
+        self.up_to_date = False
+
+    def set_inequality_pairs_constraints(self, points):
+        pts = np.asarray(points, dtype=float)
+        if pts.ndim != 2 or pts.shape[1] not in (4, 5):
+            raise cons.ShapeError("Inequality pair array must have 4 or 5 columns")
+        self.data["inequality_pairs"] = pts[:, :4].copy()
+        self.up_to_date = False
+
+    def get_value_constraints(self):
+        return self.data["value"]
+
+    def get_gradient_constraints(self):
+        return self.data["gradient"]
+
+    def get_tangent_constraints(self):
+        return self.data["tangent"]
+
+    def get_norm_constraints(self):
+        return self.data["normal"]
+
+    def get_interface_constraints(self):
+        return self.data["interface"]
+
+    def get_inequality_value_constraints(self):
+        return self.data["inequality"]
+
+    def get_inequality_pairs_constraints(self):
+        return self.data["inequality_pairs"]
+
+    def get_data_locations(self):
+        return np.vstack([d[:, :3] for d in self.data.values()])
+
+    # ------------------------------------------------------ diagnostics --
+    def _outside_model_points_from_data(self):
+        if self.support is None:
+            return {}
+        mapping = {"value": "value", "gradient": "gradient", "normal": "normal",
+                   "tangent": "tangent", "interface": "interface",
+                   "inequality": "inequality_value", "inequality_pairs": "inequality_pairs"}
+        out = {}
+        for key, fam in mapping.items():
+            pts = self.data.get(key)
+            if pts is None or pts.shape[0] == 0:
+                out[fam] = 0
+                continue
+            out[fam] = int((~self.support.inside(pts[:, :3])).sum())
+        return out
+
+    def get_constraint_diagnostics_report(self, refresh=False):
+        if self.latest_diagnostics_report is None or refresh:
+            self.latest_diagnostics_report = self._build_constraint_diagnostics_report()
+        return self.latest_diagnostics_report
+
+    def setup(self, **kwargs):
+        """Run setup and return a diagnostics report (:698-704)."""
+        report = self.setup_interpolator(**kwargs)
+        if isinstance(report, ConstraintDiagnosticsReport):
+            self.latest_diagnostics_report = report
+            return report
+        return self.get_constraint_diagnostics_report(refresh=True)
+
+    # --------------------------------------------------------- evaluate --
+    def evaluate_value(self, locations):
+        """World-coordinate evaluation (:723-730)."""
+        return self._evaluate_value_local(self._project_points(locations))
+
+    def evaluate_gradient(self, locations):
+        """World-coordinate gradient (:732-741)."""
+        return self._reproject_vectors(
+            self._evaluate_gradient_local(self._project_points(locations)))
+
+    # -------------------------------------------------------- serialise --
+    def to_dict(self):
+        """LI/_geological_interpolator.py:799-835."""
+        def safe(v):
+            if isinstance(v, np.ndarray):
+                return v.tolist()
+            if isinstance(v, (np.floating, np.integer)):
+                return v.item()
+            if isinstance(v, dict):
+                return {k: safe(x) for k, x in v.items()}
+            if isinstance(v, (list, tuple)):
+                return [safe(x) for x in v]
+            return v
+
+        payload = {"type": self.type.value,
+                   "data": safe({k: np.asarray(v) for k, v in self.data.items()}),
+                   "up_to_date": self.up_to_date, "valid": self.valid}
+        if self.support is not None and hasattr(self.support, "to_dict"):
+            sd = safe(self.support.to_dict())
+            if isinstance(sd, dict) and "nsteps" in sd and hasattr(self.support, "nsteps_cells"):
+                sd["nsteps"] = safe(np.asarray(self.support.nsteps_cells))
+            payload["support"] = sd
+        return payload
+
+    def to_json(self, indent: int = 2) -> str:
+        return json.dumps(self.to_dict(), indent=indent)
+
+    def to_yaml(self, file_path=None):
+        import yaml
+
+        if file_path is None:
+            return yaml.safe_dump(self.to_dict(), sort_keys=False, allow_unicode=True)
+        with open(file_path, "w") as f:
+            yaml.safe_dump(self.to_dict(), f, sort_keys=False, allow_unicode=True)
+
+
+class DiscreteInterpolator(GeologicalInterpolator):
+    """Least-squares interpolator on a discrete support
+    (LI/_discrete_interpolator.py:29-1405)."""
+
+    def __init__(self, support, data=None, c=None, up_to_date=False):
+        GeologicalInterpolator.__init__(self, data=data, up_to_date=up_to_date)
+        self.support = support
+        self.dimensions = support.dimension
+        self.c = (np.array(c) if c is not None and np.array(c).shape[0] == support.n_nodes
+                  else np.zeros(support.n_nodes))
+        self._c_dev = None
+        self.shape = "rectangular"
+        self.c_ = 0
+        self.solver = "cg"
+        self.constraints = OrderedDict()
+        self.ineq_constraints = {}
+        self.interpolation_weights = {}
+        self.type = InterpolatorType.BASE_DISCRETE
+        self.apply_scaling_matrix = True
+        self.add_ridge_regulatisation = True
+        self.ridge_factor = 1e-8
+        self.solver_history = None
+        self.latest_solve_timing = {}
+        self._rows = OrderedDict()       # name -> fs.FdRows (device)
+        self._stencils = OrderedDict()   # name -> StencilFamily
+        self._system = None
+        self._system_key = None
+        self._scale_sources = []         # pending regularisation_scale terms
+        self._regularisation_scale = None
+        if hasattr(support, "associateInterpolator"):
+            support.associateInterpolator(self)
+
+    # ----------------------------------------------------------- basics --
+    def get_last_solve_timing(self) -> dict:
+        return dict(self.latest_solve_timing)
+
+    def set_nelements(self, nelements: int) -> int:
+        return self.support.set_nelements(nelements)
+
+    @property
+    def n_elements(self) -> int:
+        return self.support.n_elements
+
+    @property
+    def dof(self) -> int:
+        return int(self.support.n_nodes)
+
+    @property
+    def region(self):
+        return np.ones(self.support.n_nodes, dtype=bool)
+
+    @property
+    def region_map(self):
+        return np.arange(self.support.n_nodes, dtype=int)
+
+    def set_region(self, region=None):
+        logger.info("Region masking has been removed; the interpolator always uses all "
+                    "support nodes (%s DOF).", self.dof)
+
+    def set_interpolation_weights(self, weights):
+        for key in weights:
+            self.up_to_date = False
+            self.interpolation_weights[key] = weights[key]
+
+    def _apply_isotropic_regularisation_weight(self, value, keys):
+        if value is None:
+            return
+        for key in keys:
+            self.interpolation_weights[key] = value
+
+    def _apply_interpolation_weight_kwargs(self, kwargs, skip_keys=()):
+        for key, value in kwargs.items():
+            self.up_to_date = False
+            if key in skip_keys:
+                continue
+            self.interpolation_weights[key] = value
+
+    def reset(self):
+        """LI/_discrete_interpolator.py:306-314."""
+        self.constraints = OrderedDict()
+        self._rows = OrderedDict()
+        self._stencils = OrderedDict()
+        self.c_ = 0
+        self._scale_sources = []
+        self._regularisation_scale = None
+
+    @property
+    def regularisation_scale(self):
+        """ones(dof) + one ``1 + 10 exp(-d^2 / 2 sigma^2)`` term per normal
+        family added (LI/_finite_difference_interpolator.py:436-444), d the
+        distance to the nearest normal constraint; computed on the GPU on
+        first use."""
+        if self._regularisation_scale is None:
+            s = torch.ones(self.dof, dtype=torch.float64, device=_lib.device())
+            for pts, sigma in self._scale_sources:
+                s += _regularisation_weighting(self.support, pts, sigma)
+            self._regularisation_scale = s
+        return self._regularisation_scale.cpu().numpy()
+
+    @regularisation_scale.setter
+    def regularisation_scale(self, value):
+        self._regularisation_scale = torch.as_tensor(np.asarray(value, dtype=float),
+                                                     device=_lib.device())
+        self._scale_sources = []
+
+    def _pre_solve(self):
+        self.c = np.full(self.support.n_nodes, np.nan)
+        return True
+
+    def clear_constraints(self):
+        self.constraints = OrderedDict()
+        self._rows = OrderedDict()
+        self._stencils = OrderedDict()
+        self.ineq_constraints = {}
+
+    # ------------------------------------------------------------- rows --
+    def _unique_name(self, name):
+        """Collision renaming of LI/_discrete_interpolator.py:375-380."""
+        base = name
+        while name in self.constraints:
+            count = 0
+            if "_" in name:
+                count = int(name.split("_")[1]) + 1
+            name = base + f"_{count}"
+        return name
+
+    def _register_rows(self, blk: fs.FdRows, name: str):
+        if blk.nrows == 0:
+            name = self._unique_name(name)
+            self._rows[name] = blk
+            self.constraints[name] = _Constraint(matrix=blk, b=blk.b, w=blk.w)
+            return name
+        if bool(torch.isnan(blk.coef).any()) or bool(torch.isnan(blk.b).any()):
+            logger.warning(f"Constraints contain nan not adding constraints: {name}")
+            return None
+        name = self._unique_name(name)
+        self._rows[name] = blk
+        self.constraints[name] = _Constraint(matrix=blk, b=blk.b, w=blk.w)
+        return name
+
+    def add_constraints_to_least_squares(self, A, B, idc, w=1.0, name="undefined"):
+        """Generic rows (LI/_discrete_interpolator.py:316-389), normalised on the GPU."""
+        A = np.array(A, dtype=float)
+        B = np.array(B, dtype=float)
+        idc = np.array(idc)
+        if A.shape != idc.shape:
+            logger.error(f"Cannot add constraints: A and indexes have different shape : {name}")
+            return
+        if A.ndim > 2:
+            if isinstance(w, np.ndarray):
+                w = np.tile(w, (A.shape[1]))
+            A = A.reshape((A.shape[0] * A.shape[1], A.shape[2]))
+            idc = idc.reshape((idc.shape[0] * idc.shape[1], idc.shape[2]))
+            B = B.reshape(A.shape[0])
+        if np.any(np.isnan(idc)) or np.any(np.isnan(A)) or np.any(np.isnan(B)):
+            logger.warning(f"Constraints contain nan not adding constraints: {name}")
+            return
+        if isinstance(w, (float, int)):
+            w = np.ones(A.shape[0]) * w
+        if not isinstance(w, np.ndarray):
+            raise TypeError("w must be a numpy array")
+        if w.shape[0] != A.shape[0]:
+            raise ValueError("Weight array does not match number of constraints")
+        dev = _lib.device()
+        R, k = A.shape
+        coef = torch.from_numpy(np.ascontiguousarray(A)).to(dev)
+        b = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
+        norm = torch.empty(R, dtype=torch.float64, device=dev)
+        if R:
+            _lib.call("lsamd_rows_normalise", R, k, _lib.ptr(coef), _lib.ptr(b), _lib.ptr(norm),
+                      _lib.stream())
+        cols = torch.from_numpy(np.ascontiguousarray(idc.astype(np.int32))).to(dev)
+        blk = fs.fixed_width_rows(cols, coef, b,
+                                  torch.from_numpy(np.ascontiguousarray(w, dtype=float)).to(dev),
+                                  norm, self.dof)
+        return self._register_rows(blk, name)
+
+    # ------------------------------------------------------------ solve --
+    def _normalise_solver_choice(self, solver):
+        """LI/_solver_strategy.py:15-29."""
+        if callable(solver):
+            return solver
+        if isinstance(solver, str) or solver is None:
+            if solver not in ["cg", "lsmr", "admm", None]:
+                logger.warning("Unknown solver %s using cg. Available solvers are cg, lsmr, "
+                               "admm or a custom callable", solver)
+                return "cg"
+            return "cg" if solver is None else solver
+        logger.warning("Unsupported solver type %s, using cg", type(solver).__name__)
+        return "cg"
+
+    def _system_rows(self):
+        return fs.concat_rows(list(self._rows.values()), self.dof)
+
+    def _build_system(self):
+        raise NotImplementedError
+
+    def solve_system(self, solver=None, tol=None, solver_kwargs=None) -> bool:
+        """LI/_discrete_interpolator.py:1221-1319 on the GPU.
+
+        ``solver_kwargs`` follow scipy's cg: rtol (default 1e-5), atol
+        (default 0; ``tol`` sets atol unless both rtol and atol are given),
+        maxiter (default 10*dof).  Extra keys: ``precision`` ('fp64' |
+        'fp32': FP32 operator + FP64 refinement), ``inner_rtol``.
+        """
+        if not self._pre_solve():
+            raise ValueError("Pre solve failed")
+        started = perf_counter()
+        choice = self._normalise_solver_choice(solver)
+        kw = dict(solver_kwargs or {})
+        for k in ("constant_norm_iterations", "constant_norm_weight", "constant_norm_target"):
+            if kw.pop(k, None):
+                logger.warning("%s is not supported by the GPU solver; ignored", k)
+        kw.pop("backend", None)
+        timing = {"solver": "callable" if callable(choice) else choice, "backend": "hip"}
+        if self.get_inequality_value_constraints().shape[0] or \
+                self.get_inequality_pairs_constraints().shape[0]:
+            logger.warning("Inequality constraints are not supported by the GPU path; ignored")
+        timing["inequality_rows"] = 0
+        if choice == "admm":
+            logger.warning("No inequality constraints, using lsmr")
+            choice = "lsmr"
+        t0 = perf_counter()
+        system, nrows_total, nnz = self._build_system()
+        torch.cuda.synchronize()
+        timing["assembly_seconds"] = perf_counter() - t0
+        timing["matrix_rows"] = int(nrows_total)
+        timing["matrix_cols"] = int(self.dof)
+        timing["matrix_nnz"] = int(nnz)
+        timing["preprocess_seconds"] = 0.0
+        if callable(choice):
+            raise NotImplementedError(
+                "a callable solver needs the explicit sparse system; use solver='cg'")
+        if choice == "lsmr":
+            logger.warning("lsmr runs as GPU CG on the normal equations (same minimiser)")
+            kw["rtol"] = kw.pop("btol", tol if tol is not None else 1e-6)
+            for k in ("atol", "conlim", "damp", "show"):
+                kw.pop(k, None)
+        elif ("atol" not in kw or "rtol" not in kw) and tol is not None:
+            kw["atol"] = tol
+        for k in ("x0", "M", "callback"):
+            if kw.pop(k, None) is not None:
+                logger.warning("solver_kwargs[%r] is not supported by the GPU solver; ignored", k)
+        t1 = perf_counter()
+        c, info = system.solve(rtol=kw.pop("rtol", 1e-5), atol=kw.pop("atol", 0.0),
+                               maxiter=kw.pop("maxiter", None),
+                               precision=kw.pop("precision", "fp64"),
+                               inner_rtol=kw.pop("inner_rtol", 1e-4),
+                               jacobi=self.apply_scaling_matrix)
+        torch.cuda.synchronize()
+        timing["solve_seconds"] = perf_counter() - t1
+        timing.update({"iterations": info["iterations"], "refinement_sweeps": info["outer"],
+                       "residual": info["residual"], "threshold": info["threshold"]})
+        if not info["converged"]:
+            logger.warning("CG reached iteration limit (%s) and did not converge; using last "
+                           "iteration", info["iterations"])
+        self._c_dev = c
+        self.c = c.cpu().numpy()
+        self.up_to_date = True
+        timing["total_seconds"] = perf_counter() - started
+        timing["up_to_date"] = True
+        self.latest_solve_timing = timing
+        return self.up_to_date
+
+    def update(self) -> bool:
+        """LI/_discrete_interpolator.py:1321-1340."""
+        if self.solver is None:
+            return False
+        if not self.up_to_date:
+            self.setup_interpolator()
+            self.up_to_date = self.solve_system(self.solver)
+            return self.up_to_date
+        return bool(self.up_to_date)
+
+    def _c_device(self):
+        if self._c_dev is None or self._c_dev.shape[0] != self.dof:
+            self._c_dev = torch.from_numpy(np.ascontiguousarray(self.c, dtype=float)).to(
+                _lib.device())
+        return self._c_dev
+
+    def to_dict(self):
+        # LI/_discrete_interpolator.py:1381-1388: base-class keys win
+        return {"type": self.type.name, "support": self.support.to_dict(),
+                "c": np.asarray(self.c).tolist(), **super().to_dict()}
+
+    # ------------------------------------------------------ diagnostics --
+    @staticmethod
+    def _family_of(name):
+        """LI/_discrete_interpolator.py:179-201."""
+        n = name.lower()
+        for pre, fam in (("value", "value"), ("gradient", "gradient"), ("norm", "normal"),
+                         ("tangent", "tangent"), ("interface", "interface"),
+                         ("inequality_value", "inequality_value"),
+                         ("inequality_pairs", "inequality_pairs"), ("fold", "fold")):
+            if n.startswith(pre):
+                return fam
+        if n.startswith("d") and len(n) in (3, 4, 5, 6, 7, 8):
+            return "regularisation"
+        return "other"
+
+    def _build_constraint_diagnostics_report(self):
+        """LI/_discrete_interpolator.py:203-278."""
+        outside = self._outside_model_points_from_data()
+        raw = {"value": self.get_value_constraints().shape[0],
+               "gradient": self.get_gradient_constraints().shape[0],
+               "normal": self.get_norm_constraints().shape[0],
+               "tangent": self.get_tangent_constraints().shape[0],
+               "interface": self.get_interface_constraints().shape[0],
+               "inequality_value": self.get_inequality_value_constraints().shape[0],
+               "inequality_pairs": self.get_inequality_pairs_constraints().shape[0]}
+        rows = defaultdict(int)
+        wts = defaultdict(list)
+        for name, con in self.constraints.items():
+            fam = self._family_of(name)
+            m = dict.__getitem__(con, "matrix")
+            rows[fam] += int(m.shape[0])
+            w = con["w"]
+            if w is not None:
+                w = np.atleast_1d(np.asarray(w, dtype=float))
+                if w.size:
+                    wts[fam].append(w)
+        fams = {}
+        for fam in sorted(set(raw) | set(rows)):
+            rc = int(rows.get(fam, 0))
+            src = int(raw.get(fam, 0))
+            dropped = None
+            if fam in ("value", "normal", "tangent", "inequality_value"):
+                dropped = max(src - rc, 0)
+            elif fam == "gradient":
+                dropped = max(src * 2 - rc, 0)
+            if wts.get(fam):
+                allw = np.concatenate(wts[fam])
+                mw, lo, hi = float(np.mean(allw)), float(np.min(allw)), float(np.max(allw))
+            else:
+                mw = lo = hi = None
+            fams[fam] = ConstraintFamilyDiagnostics(fam, rc > 0, rc, dropped, mw, lo, hi, src,
+                                                    int(outside.get(fam, 0)))
+        n = int(self.support.n_nodes)
+        cov = RegionCoverageDiagnostics(n, n, 0, 1.0 if n else 0.0)
+        return ConstraintDiagnosticsReport(self.type.name, fams, cov, outside)
+
+    def finalize_setup_diagnostics_report(self):
+        self.latest_diagnostics_report = self._build_constraint_diagnostics_report()
+        return self.latest_diagnostics_report
+
+    # ------------------------------------------------------ evaluation --
+    def _evaluate_value_local(self, locations):
+        """LI/_discrete_interpolator.py:1342-1362: NaN coordinates -> 0."""
+        self.update()
+        pts = np.array(locations, dtype=float)
+        if pts.shape[0] == 0:
+            return np.zeros(0)
+        v, _ = self._evaluate(pts, True, False)
+        return v
+
+    def _evaluate_gradient_local(self, locations):
+        """LI/_discrete_interpolator.py:1364-1379."""
+        self.update()
+        pts = np.asarray(locations, dtype=float)
+        if pts.shape[0] > 0:
+            _, g = self._evaluate(pts, False, True)
+            return g
+        return np.zeros((0, 3))
+
+
+def _regularisation_weighting(support, pts, sigma):
+    """``1 + 10 exp(-d^2 / (2 sigma^2))`` at every node, d = distance to the
+    nearest of ``pts`` (compute_weighting, LI/_finite_difference_interpolator.py:66-88)."""
+    dev = _lib.device()
+    p = torch.from_numpy(np.ascontiguousarray(pts[:, :3], dtype=float)).to(dev)
+    d2 = torch.empty(support.n_nodes, dtype=torch.float64, device=dev)
+    g = support._lsgrid()
+    _lib.call("lsamd_fd_nearest_d2", ctypes.byref(g), _lib.ptr(p), int(p.shape[0]),
+              _lib.ptr(d2), _lib.stream())
+    return 1.0 + 10.0 * torch.exp(-d2 / (2.0 * float(sigma) ** 2))
+
+
+class FiniteDifferenceInterpolator(DiscreteInterpolator):
+    """LI/_finite_difference_interpolator.py:91-1205 on the MI355X."""
+
+    def __init__(self, grid, data=None):
+        self.shape = "rectangular"
+        DiscreteInterpolator.__init__(self, grid, data=data)
+        self.set_interpolation_weights({"dxy": 1.0, "dyz": 1.0, "dxz": 1.0, "dxx": 1.0,
+                                        "dyy": 1.0, "dzz": 1.0, "dx": 1.0, "dy": 1.0,
+                                        "dz": 1.0, "cpw": 1.0, "gpw": 1.0, "npw": 10.0,
+                                        "tpw": 1.0, "ipw": 1.0})
+        self.type = InterpolatorType.FINITE_DIFFERENCE
+        self.use_regularisation_weight_scale = False
+        self.regularisation_weight_sigma = None
+        self.interface_pair_mode = "auto"
+        self.interface_pair_threshold = 200
+        self.interface_pair_fallback = "star"
+        self.regularisation_matrix_free = False
+        self.matrix_free_regularisation_blocks = {}
+
+    def copy(self):
+        return FiniteDifferenceInterpolator(self.support)
+
+    # ------------------------------------------------------------ setup --
+    def setup_interpolator(self, **kwargs):
+        """LI/_finite_difference_interpolator.py:132-224."""
+        self.reset()
+        reg = kwargs.get("regularisation", None)
+        directional = kwargs.get("directional_regularisation", None)
+        isotropic = None
+        if reg is not None:
+            if np.isscalar(reg):
+                isotropic = float(reg)
+            elif isinstance(reg, dict) and ("isotropic" in reg or "weight" in reg) and \
+                    "direction" not in reg and "vector" not in reg:
+                isotropic = float(reg.get("isotropic", reg.get("weight")))
+                directional = directional or reg.get("directional") or None
+            else:
+                directional = reg
+        if directional:
+            raise NotImplementedError("directional regularisation is not part of this package")
+        self._apply_isotropic_regularisation_weight(
+            isotropic, ("dxy", "dyz", "dxz", "dxx", "dyy", "dzz"))
+        self._apply_interpolation_weight_kwargs(
+            kwargs, skip_keys=("regularisation", "directional_regularisation"))
+        operators = kwargs.get("operators",
+                               self.support.get_operators(weights=self.interpolation_weights))
+        self.use_regularisation_weight_scale = kwargs.get("use_regularisation_weight_scale", False)
+        self.regularisation_weight_sigma = kwargs.get("regularisation_weight_sigma", None)
+        self.regularisation_matrix_free = bool(kwargs.get("regularisation_matrix_free", False))
+        if self.regularisation_matrix_free and self.apply_scaling_matrix:
+            logger.warning("regularisation_matrix_free=True with apply_scaling_matrix=True: the "
+                           "reference falls back to explicit assembly; the GPU operator is "
+                           "matrix-free either way")
+            self.regularisation_matrix_free = False
+        self.interface_pair_mode = kwargs.get("interface_pair_mode", self.interface_pair_mode)
+        self.interface_pair_threshold = int(kwargs.get("interface_pair_threshold",
+                                                       self.interface_pair_threshold))
+        self.interface_pair_fallback = kwargs.get("interface_pair_fallback",
+                                                  self.interface_pair_fallback)
+        self.add_norm_constraints(self.interpolation_weights["npw"])
+        self.add_gradient_constraints(self.interpolation_weights["gpw"])
+        self.add_value_constraints(self.interpolation_weights["cpw"])
+        self.add_tangent_constraints(self.interpolation_weights["tpw"])
+        self.add_interface_constraints(self.interpolation_weights["ipw"])
+        for k, o in operators.items():
+            self.assemble_inner(o[0], o[1], name=k)
+        self.assemble_borders()
+        return self.finalize_setup_diagnostics_report()
+
+    # ------------------------------------------------------- data rows ----
+    def _inside_points(self, points, ncols):
+        """Device copy of the rows of ``points`` inside the grid, and the mask."""
+        dev = _lib.device()
+        pts = torch.from_numpy(np.ascontiguousarray(points[:, :ncols], dtype=float)).to(dev)
+        _, ins, _, _ = fs.locate(self.support._lsgrid(), pts[:, :3].contiguous(), want_w=False)
+        return pts[ins].contiguous(), ins
+
+    def add_value_constraints(self, w=1.0):
+        """Trilinear rows, b = value, weight cpw * w_pt (:236-269)."""
+        points = self.get_value_constraints()
+        if points.shape[0] > 0:
+            pin, ins = self._inside_points(points, points.shape[1])
+            wcol = self.support.dimension + 1 if points.shape[1] > 4 else -1
+            blk = fs.build_rows(self.support._lsgrid(), _lib.ROWS_VALUE, pin, wcol, float(w),
+                                self.dof)
+            self._register_rows(blk, "value")
+            if pin.shape[0] <= 0:
+                logger.warning(f"{points.shape[0]} value constraints not added: outside of "
+                               f"model bounding box")
+
+    def add_norm_constraints(self, w=1.0):
+        """Three rows per point T_d, b = n_d, weight npw/3 * w_pt (:403-476)."""
+        points = self.get_norm_constraints()
+        if points.shape[0] > 0:
+            pin, ins = self._inside_points(points, points.shape[1])
+            sigma = self.regularisation_weight_sigma
+            if sigma is None:
+                sigma = self.support.nsteps[0] * 10
+            self._scale_sources.append((pin[:, :3].cpu().numpy(), sigma))
+            self._regularisation_scale = None
+            wcol = 2 * self.support.dimension if points.shape[1] > 6 else -1
+            w = w / self.support.dimension
+            blk = fs.build_rows(self.support._lsgrid(), _lib.ROWS_NORMAL, pin, wcol, float(w),
+                                self.dof)
+            n = pin.shape[0]
+            for d in range(3):
+                self._register_rows(_row_slice(blk, d * n, (d + 1) * n), f"norm_{d}")
+            if n <= 0:
+                logger.warning(f"{points.shape[0]} norm constraints not added: outside of "
+                               f"model bounding box")
+            self.up_to_date = False
+
+    def add_gradient_constraints(self, w=1.0):
+        """Strike and dip rows (:342-401); normalises the stored vectors in place."""
+        points = self.get_gradient_constraints()
+        if points.shape[0] > 0:
+            pin, ins = self._inside_points(points, points.shape[1])
+            blk = fs.build_rows(self.support._lsgrid(), _lib.ROWS_GRADIENT, pin, -1, float(w),
+                                self.dof)
+            insn = ins.cpu().numpy()
+            norm = np.linalg.norm(points[~insn, 3:6], axis=1)
+            points[~insn, 3:6] /= norm[:, None]
+            points[insn, 3:6] = pin[:, 3:6].cpu().numpy()
+            n = pin.shape[0]
+            self._register_rows(_row_slice(blk, 0, n), "gradient")
+            self._register_rows(_row_slice(blk, n, 2 * n), "gradient")
+            if n <= 0:
+                logger.warning(f" {points.shape[0]} norm constraints not added: outside of "
+                               f"model bounding box")
+
+    def add_tangent_constraints(self, w=1.0):
+        """LI/_discrete_interpolator.py:668-682."""
+        points = self.get_tangent_constraints()
+        if points.shape[0] > 0:
+            self.add_gradient_orthogonal_constraints(points[:, :3], points[:, 3:6], w)
+
+    def add_gradient_orthogonal_constraints(self, points, vectors, w=1.0, b=0,
+                                            name="gradient orthogonal"):
+        """t_hat . grad = b rows (:478-539); ``vectors`` normalised in place."""
+        points = np.asarray(points)
+        if points.shape[0] > 0:
+            arr = np.hstack([np.asarray(points[:, :3], dtype=float),
+                             np.asarray(vectors[:, :3], dtype=float)])
+            pin, ins = self._inside_points(arr, 6)
+            blk = fs.build_rows(self.support._lsgrid(), _lib.ROWS_TANGENT, pin, -1, float(w),
+                                self.dof)
+            insn = ins.cpu().numpy()
+            vout = np.asarray(vectors)
+            norm = np.linalg.norm(vout[~insn, :3], axis=1)
+            sel = np.nonzero(~insn)[0][norm > 0]
+            vout[sel, :3] /= norm[norm > 0, None]
+            vout[insn, :3] = pin[:, 3:6].cpu().numpy()
+            if b != 0:
+                bb = torch.full_like(blk.b, float(b))
+                blk.b = torch.where(blk.norm > 0, bb / blk.norm, bb)
+            self._register_rows(blk, name)
+            if pin.shape[0] <= 0:
+                logger.warning(f"{points.shape[0]} gradient constraints not added: outside of "
+                               f"model bounding box")
+            self.up_to_date = False
+
+    def _interface_pair_indices(self, n: int) -> np.ndarray:
+        """LI/_finite_difference_interpolator.py:271-295."""
+        if n < 2:
+            return np.zeros((0, 2), dtype=int)
+        mode = str(self.interface_pair_mode).lower()
+        threshold = max(2, int(self.interface_pair_threshold))
+        fallback = str(self.interface_pair_fallback).lower()
+        if fallback not in ("star", "chain"):
+            fallback = "star"
+        if mode == "auto":
+            mode = fallback if n > threshold else "all"
+        if mode == "all":
+            ii, jj = np.triu_indices(n, k=1)
+            return np.column_stack([ii, jj]).astype(int)
+        if mode == "star":
+            return np.column_stack([np.zeros(n - 1, dtype=int), np.arange(1, n, dtype=int)])
+        if mode == "chain":
+            return np.column_stack([np.arange(0, n - 1, dtype=int), np.arange(1, n, dtype=int)])
+        raise ValueError(f"Unknown interface_pair_mode '{self.interface_pair_mode}'. Use one "
+                         f"of: auto, all, star, chain")
+
+    def add_interface_constraints(self, w=1.0):
+        """Pairwise equal-value rows per interface id (:297-340), 16 entries per
+        row, normalised on the GPU."""
+        points = self.get_interface_constraints()
+        if points.shape[0] > 1:
+            dev = _lib.device()
+            pts = torch.from_numpy(np.ascontiguousarray(points[:, :3], dtype=float)).to(dev)
+            idc, ins, wt, _ = fs.locate(self.support._lsgrid(), pts, want_w=True)
+            insn = ins.cpu().numpy()
+            idc_in, w_in = idc[ins], wt[ins]
+            ids = points[insn, 3]
+            for uid in np.unique(points[np.logical_and(~np.isnan(points[:, 3]), insn), 3]):
+                mask = np.nonzero(ids == uid)[0]
+                pair = self._interface_pair_indices(len(mask))
+                if pair.shape[0] == 0:
+                    continue
+                m = torch.from_numpy(mask).to(dev)
+                p0 = torch.from_numpy(pair[:, 0]).to(dev)
+                p1 = torch.from_numpy(pair[:, 1]).to(dev)
+                A = torch.cat([w_in[m][p0], -w_in[m][p1]], dim=1).contiguous()
+                I = torch.cat([idc_in[m][p0], idc_in[m][p1]], dim=1).to(torch.int32).contiguous()
+                R = A.shape[0]
+                b = torch.zeros(R, dtype=torch.float64, device=dev)
+                norm = torch.empty(R, dtype=torch.float64, device=dev)
+                _lib.call("lsamd_rows_normalise", R, 16, _lib.ptr(A), _lib.ptr(b),
+                          _lib.ptr(norm), _lib.stream())
+                wv = torch.full((R,), float(w), dtype=torch.float64, device=dev)
+                self._register_rows(fs.fixed_width_rows(I, A, b, wv, norm, self.dof),
+                                    f"interface_{uid}")
+
+    # ------------------------------------------------------ regulariser ----
+    def assemble_inner(self, operator, w, name="regularisation"):
+        """Record an interior stencil family (:1146-1164); matrix-free."""
+        if operator is None:
+            raise NotImplementedError("rectilinear grids are not part of this package")
+        kind = _mask_kind(operator)
+        if kind is None:
+            raise NotImplementedError(
+                f"custom regularisation mask for '{name}' is not supported by the GPU operator")
+        n = self.support.nsteps
+        nrows = int(np.prod(n - 2)) if np.all(n >= 3) else 0
+        name = self._unique_name(name)
+        fam = StencilFamily(name, kind, float(w), nrows, self.dof)
+        self._stencils[name] = fam
+        self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(nrows),
+                                             w=np.full(nrows, float(w)))
+
+    def assemble_borders(self):
+        """Border layers 1 and n-2 with the axial masks, weights dx/dy/dz
+        (:1052-1144); skipped when the weight is 0 or the axis has < 3 nodes."""
+        n = self.support.nsteps
+        for axis, key, kind in ((0, "dx", "bx"), (1, "dy", "by"), (2, "dz", "bz")):
+            weight = self.interpolation_weights[key]
+            if weight == 0 or n[axis] < 3:
+                continue
+            nrows = int(np.prod(n) // n[axis])
+            for side in ("lower", "upper"):
+                name = self._unique_name(f"{key}_{side}")
+                fam = StencilFamily(name, kind, float(weight), nrows, self.dof)
+                self._stencils[name] = fam
+                self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(nrows),
+                                                     w=np.full(nrows, float(weight)))
+
+    def _reg_struct(self):
+        r = _lib.FdReg()
+        acc = defaultdict(float)
+        for fam in self._stencils.values():
+            acc[fam.kind] += fam.weight ** 2
+        r.w2_xx, r.w2_yy, r.w2_zz = acc["xx"], acc["yy"], acc["zz"]
+        r.w2_xy, r.w2_xz, r.w2_yz = acc["xy"], acc["xz"], acc["yz"]
+        # lower and upper layers share one weight slot each
+        r.w2_bx, r.w2_by, r.w2_bz = acc["bx"] / 2, acc["by"] / 2, acc["bz"] / 2
+        r.ridge2 = float(self.ridge_factor) ** 2 if self.add_ridge_regulatisation else 0.0
+        return r
+
+    def _build_system(self):
+        rows = self._system_rows()
+        scale2 = None
+        if self.use_regularisation_weight_scale:
+            self.regularisation_scale  # materialise on the device
+            scale2 = self._regularisation_scale ** 2
+        system = fs.FdSystem(self.support._lsgrid(), self._reg_struct(), scale2)
+        system.set_rows(rows)
+        self._system = system
+        nreg = sum(f.nrows for f in self._stencils.values())
+        ndata = rows.nrows if rows is not None else 0
+        nnz = (rows.nnz if rows is not None else 0) + sum(
+            f.nrows * (3 if f.kind in ("xx", "yy", "zz", "bx", "by", "bz") else 4)
+            for f in self._stencils.values())
+        return system, ndata + nreg, nnz
+
+    def _evaluate(self, pts, values, gradients):
+        v, g = fs.evaluate(self.support._lsgrid(), self._c_device(), torch.from_numpy(pts),
+                           values, gradients)
+        return (v.cpu().numpy() if v is not None else None,
+                g.cpu().numpy() if g is not None else None)
+
+
+def _row_slice(blk: fs.FdRows, r0: int, r1: int) -> fs.FdRows:
+    """Rows [r0, r1) of a fixed-width block."""
+    k = int(blk.row_ptr[1] - blk.row_ptr[0]) if blk.nrows else 8
+    rp = blk.row_ptr[r0:r1 + 1] - blk.row_ptr[r0] if blk.nrows else blk.row_ptr[:1]
+    e0, e1 = r0 * k, r1 * k
+    return fs.FdRows(rp, blk.cols[e0:e1], blk.coef[e0:e1], blk.b[r0:r1], blk.w[r0:r1],
+                     blk.norm[r0:r1], blk.dof)

@@ -1,0 +1,443 @@
+"""Structured-grid support with the reference's ``StructuredGrid`` interface.
+
+Geometry bookkeeping (origin, steps, node counts, setters) follows
+LC/supports/_3d_base_structured.py:19-245; point location, shape functions,
+their gradients and evaluation run in the HIP kernels of
+libloopstructural_amd (LC/supports/_3d_structured_grid.py:90-454), so every
+numerical result is produced on the GPU and copied back as numpy where the
+reference returns numpy.
+"""
+from __future__ import annotations
+
+import logging
+from enum import IntEnum
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import fd_system as fs
+
+logger = logging.getLogger(__name__)
+
+
+class SupportType(IntEnum):
+    """LC/supports/__init__.py support type ids used in ``to_dict``."""
+
+    StructuredGrid2D = 0
+    StructuredGrid = 1
+    UnStructuredTetMesh = 2
+    P1Unstructured2d = 3
+    P2Unstructured2d = 4
+    BaseUnstructured2d = 5
+    BaseStructured = 6
+    TetMesh = 10
+    P2UnstructuredTetMesh = 11
+
+
+class LoopException(Exception):
+    """LC/supports/_3d_base_structured.py:14."""
+
+
+class Operator:
+    """Finite-difference masks, (3,3,3) indexed [dz+1][dy+1][dx+1]
+    (LC/math/finite_difference_stencil.py:12-38)."""
+
+    z = np.zeros((3, 3))
+    Dx_mask = np.array([z, [[0.0, 0.0, 0.0], [-0.5, 0.0, 0.5], [0.0, 0.0, 0.0]], z])
+    Dy_mask = Dx_mask.swapaxes(1, 2)
+    Dz_mask = Dx_mask.swapaxes(0, 2)
+    Dxx_mask = np.array([z, [[0, 0, 0], [1, -2, 1], [0, 0, 0]], z])
+    Dyy_mask = Dxx_mask.swapaxes(1, 2)
+    Dzz_mask = Dxx_mask.swapaxes(0, 2)
+    Dxy_mask = np.array([z, [[-0.25, 0, 0.25], [0, 0, 0], [0.25, 0, -0.25]], z]) / np.sqrt(2)
+    Dxz_mask = Dxy_mask.swapaxes(0, 1)
+    Dyz_mask = Dxy_mask.swapaxes(0, 2)
+    Lapacian = np.array([[[0, 0, 0], [0, 1, 0], [0, 0, 0]],
+                         [[0, 1, 0], [1, -6, 1], [0, 1, 0]],
+                         [[0, 0, 0], [0, 1, 0], [0, 0, 0]]])
+
+
+def _as_points(pos) -> np.ndarray:
+    """check_position, LC/supports/_3d_base_structured.py:295-326."""
+    if not isinstance(pos, (np.ndarray, torch.Tensor)):
+        try:
+            pos = np.array(pos, dtype=float)
+        except Exception as e:
+            raise ValueError(f"Position array should be a numpy array or list of points, "
+                             f"not {type(pos)}") from e
+    if isinstance(pos, torch.Tensor):
+        pos = pos.detach().cpu().numpy()
+    if pos.ndim == 1:
+        pos = np.array([pos])
+    if pos.ndim != 2:
+        raise ValueError("Position array needs to be a list of points or a point")
+    return np.ascontiguousarray(pos[:, :3], dtype=float)
+
+
+class BaseStructuredSupport:
+    """Geometry of a regular grid: ``nsteps`` cells per axis -> ``nsteps+1`` nodes."""
+
+    dimension = 3
+
+    def __init__(self, origin=None, nsteps=None, step_vector=None, rotation_xy=None):
+        origin = np.zeros(3) if origin is None else np.array(origin, dtype=float)
+        nsteps = np.array([10, 10, 10]) if nsteps is None else np.array(nsteps)
+        step_vector = np.ones(3) if step_vector is None else np.array(step_vector, dtype=float)
+        if np.any(step_vector == 0):
+            logger.warning(f"Step vector {step_vector} has zero values")
+        if np.any(nsteps == 0):
+            raise LoopException("nsteps cannot be zero")
+        if np.any(nsteps < 0):
+            raise LoopException("nsteps cannot be negative")
+        self._nsteps = np.array(nsteps, dtype=int) + 1
+        self._step_vector = np.array(step_vector, dtype=float)
+        self._origin = np.array(origin, dtype=float)
+        self.supporttype = "Base"
+        self.type = SupportType.BaseStructured
+        self._rotation_xy = np.eye(3)
+        self.rotation_xy = rotation_xy
+        self.interpolator = None
+
+    # ---------------------------------------------------------- geometry --
+    @property
+    def nsteps(self):
+        return self._nsteps
+
+    @nsteps.setter
+    def nsteps(self, nsteps):
+        change_factor = nsteps / self.nsteps
+        self._step_vector /= change_factor
+        self._nsteps = np.array(nsteps, dtype=int)
+        self.onGeometryChange()
+
+    @property
+    def nsteps_cells(self):
+        return self.nsteps - 1
+
+    @property
+    def step_vector(self):
+        return self._step_vector
+
+    @step_vector.setter
+    def step_vector(self, step_vector):
+        change_factor = step_vector / self._step_vector
+        self._nsteps = np.ceil(self._nsteps / change_factor).astype(int)
+        self._step_vector = np.array(step_vector, dtype=float)
+        self.onGeometryChange()
+
+    @property
+    def origin(self):
+        return self._origin
+
+    @origin.setter
+    def origin(self, origin):
+        origin = np.array(origin, dtype=float)
+        length = (self.maximum - origin) / self.step_vector
+        self._nsteps = np.ceil(length).astype(np.int64)
+        self._nsteps[self._nsteps == 0] = 3
+        if np.any(~(self._nsteps > 0)):
+            raise ValueError("Cannot resize the interpolation support.")
+        self._origin = origin
+        self.onGeometryChange()
+
+    @property
+    def maximum(self):
+        return self.origin + self.nsteps_cells * self.step_vector
+
+    @maximum.setter
+    def maximum(self, maximum):
+        length = (np.array(maximum, dtype=float) - self.origin) / self.step_vector
+        self._nsteps = np.ceil(length).astype(np.int64)
+        self._nsteps[self._nsteps == 0] = 3
+        if np.any(~(self._nsteps > 0)):
+            raise ValueError("Cannot resize the interpolation support.")
+        self.onGeometryChange()
+
+    @property
+    def rotation_xy(self):
+        return self._rotation_xy
+
+    @rotation_xy.setter
+    def rotation_xy(self, rotation_xy):
+        if rotation_xy is None:
+            return
+        if isinstance(rotation_xy, (float, int)):
+            a = np.deg2rad(rotation_xy)
+            rotation_xy = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0],
+                                    [0, 0, 1]])
+        rotation_xy = np.array(rotation_xy)
+        if rotation_xy.shape != (3, 3):
+            raise ValueError(f"Rotation matrix should be 3x3, not {rotation_xy.shape}")
+        self._rotation_xy = rotation_xy
+
+    @property
+    def volume(self):
+        return np.prod(self.maximum - self.origin)
+
+    def set_nelements(self, nelements) -> int:
+        """LC/supports/_3d_base_structured.py:81-92."""
+        ele_vol = self.volume / nelements
+        step_vector = np.zeros(3)
+        step_vector[:] = ele_vol ** (1.0 / 3.0)
+        self.nsteps = np.ceil((self.maximum - self.origin) / step_vector).astype(int)
+        return self.n_elements
+
+    @property
+    def n_nodes(self):
+        return int(np.prod(self.nsteps))
+
+    @property
+    def n_elements(self):
+        return int(np.prod(self.nsteps_cells))
+
+    @property
+    def element_size(self):
+        return np.prod(self.step_vector)
+
+    @property
+    def element_scale(self):
+        return 1.0
+
+    def onGeometryChange(self):
+        if self.interpolator is not None:
+            self.interpolator.reset()
+
+    def associateInterpolator(self, interpolator):
+        self.interpolator = interpolator
+
+    @property
+    def nodes(self):
+        mx = self.origin + self.nsteps_cells * self.step_vector
+        x = np.linspace(self.origin[0], mx[0], self.nsteps[0])
+        y = np.linspace(self.origin[1], mx[1], self.nsteps[1])
+        z = np.linspace(self.origin[2], mx[2], self.nsteps[2])
+        xx, yy, zz = np.meshgrid(x, y, z, indexing="ij")
+        return np.array([xx.flatten(order="F"), yy.flatten(order="F"),
+                         zz.flatten(order="F")]).T
+
+    def to_dict(self):
+        return {"origin": self.origin, "nsteps": self.nsteps, "step_vector": self.step_vector,
+                "rotation_xy": self.rotation_xy}
+
+    def __str__(self):
+        return (f"LoopStructural interpolation support:  {self.supporttype} \n"
+                f"Origin: {self.origin[0]} {self.origin[1]} {self.origin[2]} \n"
+                f"Maximum: {self.maximum[0]} {self.maximum[1]} {self.maximum[2]} \n"
+                f"Step Vector: {self.step_vector[0]} {self.step_vector[1]} "
+                f"{self.step_vector[2]} \n"
+                f"Number of Steps: {self.nsteps[0]} {self.nsteps[1]} {self.nsteps[2]} \n"
+                f"Degrees of freedon {self.n_nodes}")
+
+    # ------------------------------------------------- index arithmetic --
+    def global_index_to_node_index(self, global_index):
+        gi = np.asarray(global_index)
+        shape = gi.shape
+        gi = gi.reshape(-1)
+        out = np.zeros((gi.shape[0], 3), dtype=int)
+        out[:, 0] = gi % self.nsteps[0]
+        out[:, 1] = gi // self.nsteps[0] % self.nsteps[1]
+        out[:, 2] = gi // self.nsteps[0] // self.nsteps[1]
+        return out.reshape(*shape, 3)
+
+    def global_index_to_cell_index(self, global_index):
+        gi = np.asarray(global_index)
+        out = np.zeros((gi.shape[0], 3), dtype=int)
+        nc = self.nsteps_cells
+        out[:, 0] = gi % nc[0]
+        out[:, 1] = gi // nc[0] % nc[1]
+        out[:, 2] = gi // nc[0] // nc[1]
+        return out
+
+    def _global_indicies(self, indexes, nsteps):
+        indexes = np.asarray(indexes)
+        if indexes.ndim == 1:
+            raise ValueError("Cell indexes needs to be Nx3")
+        if indexes.shape[-1] != 3:
+            raise ValueError("Last dimensions should be ijk indexing")
+        shape = indexes.shape
+        ix = indexes.reshape(-1, 3)
+        gi = ix[:, 0] + nsteps[0] * ix[:, 1] + nsteps[0] * nsteps[1] * ix[:, 2]
+        return gi.reshape(shape[:-1])
+
+    def global_node_indices(self, indexes):
+        return self._global_indicies(indexes, self.nsteps)
+
+    def global_cell_indices(self, indexes):
+        return self._global_indicies(indexes, self.nsteps_cells)
+
+    def cell_corner_indexes(self, cell_indexes):
+        """(N,3) cell ijk -> (N,8,3) corner ijk, corner order 000,100,010,110,001,..."""
+        ci = np.asarray(cell_indexes)
+        off = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 1], [1, 0, 1],
+                        [0, 1, 1], [1, 1, 1]])
+        return ci[:, None, :] + off[None, :, :]
+
+    def node_indexes_to_position(self, node_indexes):
+        ni = np.asarray(node_indexes)
+        shape = ni.shape
+        ni = ni.reshape(-1, 3)
+        xyz = self.origin[None, :] + self.step_vector[None, :] * ni
+        return xyz.reshape(shape)
+
+    # --------------------------------------------------- GPU-backed ------
+    def _lsgrid(self) -> _lib.Grid:
+        return _lib.make_grid(self.origin, self.step_vector, self.nsteps)
+
+    def _locate(self, pos, want_w=False, want_T=False):
+        pts = _as_points(pos)
+        idc, ins, w, T = fs.locate(self._lsgrid(), torch.from_numpy(pts), want_w, want_T)
+        return pts, idc, ins, w, T
+
+    def inside(self, pos):
+        """LC/supports/_3d_base_structured.py:287-293 (on the GPU)."""
+        _, _, ins, _, _ = self._locate(pos)
+        return ins.cpu().numpy()
+
+    def position_to_cell_corners(self, pos):
+        """(N,8) corner node ids (-1 outside) and (N,) inside flags
+        (LC/supports/_3d_base_structured.py:386-412)."""
+        _, idc, ins, _, _ = self._locate(pos)
+        return idc.cpu().numpy(), ins.cpu().numpy()
+
+    def position_to_cell_index(self, pos):
+        """(N,3) cell ijk (0 outside) and inside flags (:251-282)."""
+        idc, ins = self.position_to_cell_corners(pos)
+        g0 = np.where(ins, idc[:, 0], 0)
+        ci = self.global_index_to_node_index(g0)
+        ci[~ins] = 0
+        return ci, ins
+
+    def position_to_cell_vertices(self, pos):
+        gi, inside = self.position_to_cell_corners(pos)
+        gi = np.where(gi < 0, 0, gi)
+        return self.node_indexes_to_position(self.global_index_to_node_index(gi)), inside
+
+
+class StructuredGrid(BaseStructuredSupport):
+    """Regular cartesian grid (LC/supports/_3d_structured_grid.py:18-510)."""
+
+    def __init__(self, origin=None, nsteps_cells=None, step_vector=None, nsteps=None,
+                 rotation_xy=None, properties=None, cell_properties=None,
+                 name="StructuredGrid"):
+        if nsteps_cells is None:
+            nsteps_cells = np.array([10, 10, 10]) if nsteps is None else np.array(nsteps)
+        super().__init__(origin, nsteps_cells, step_vector, rotation_xy=rotation_xy)
+        self.type = SupportType.StructuredGrid
+        self.name = name
+        self.regions = {"everywhere": np.ones(self.n_nodes).astype(bool)}
+
+    @property
+    def barycentre(self):
+        return self.cell_centres(np.arange(self.n_elements))
+
+    def cell_centres(self, global_index):
+        ci = self.global_index_to_cell_index(np.asarray(global_index))
+        return (self.origin[None, :] + self.step_vector[None, :] * ci
+                + self.step_vector[None, :] * 0.5)
+
+    def trilinear(self, local_coords):
+        """Shape functions at local coordinates (closed form, corner order
+        000,100,010,110,001,101,011,111; :90-124).  Host formula helper: the
+        assembly and evaluation paths compute these in the HIP kernels."""
+        lc = np.asarray(local_coords, dtype=float)
+        x, y, z = lc[:, 0], lc[:, 1], lc[:, 2]
+        w = np.zeros((lc.shape[0], 8))
+        w[:, 0] = (1 - x) * (1 - y) * (1 - z)
+        w[:, 1] = x * (1 - y) * (1 - z)
+        w[:, 2] = (1 - x) * y * (1 - z)
+        w[:, 4] = (1 - x) * (1 - y) * z
+        w[:, 5] = x * (1 - y) * z
+        w[:, 6] = (1 - x) * y * z
+        w[:, 3] = x * y * (1 - z)
+        w[:, 7] = x * y * z
+        return w
+
+    def position_to_local_coordinates(self, pos):
+        """clip((p - cell origin)/h, 0, 1) (:126-146), from the GPU cell lookup."""
+        pts = _as_points(pos)
+        ci, _ = self.position_to_cell_index(pts)
+        cell_origin = self.origin[None, :] + self.step_vector[None, :] * ci
+        return np.clip((pts - cell_origin) / self.step_vector[None, :], 0.0, 1.0)
+
+    def position_to_dof_coefs(self, pos):
+        """Trilinear weights (:148-162), computed by the locate kernel."""
+        _, _, _, w, _ = self._locate(pos, want_w=True)
+        return w.cpu().numpy()
+
+    def get_element_gradient_for_location(self, pos):
+        """(vertices, T (N,3,8), elements, inside) (:393-454)."""
+        pts, idc, ins, _, T = self._locate(pos, want_T=True)
+        vertices, inside = self.position_to_cell_vertices(pts)
+        ci, _ = self.position_to_cell_index(pts)
+        return vertices, T.cpu().numpy(), self.global_cell_indices(ci), inside
+
+    def get_element_for_location(self, pos):
+        pts, idc, ins, w, _ = self._locate(pos, want_w=True)
+        vertices, inside = self.position_to_cell_vertices(pts)
+        ci, _ = self.position_to_cell_index(pts)
+        return vertices, w.cpu().numpy(), self.global_cell_indices(ci), inside
+
+    def neighbour_global_indexes(self, mask=None, **kwargs):
+        """Index helper for custom stencils (:164-293)."""
+        if "indexes" in kwargs:
+            indexes = kwargs["indexes"]
+        else:
+            gi = np.arange(self.n_nodes)
+            ix = self.global_index_to_node_index(gi)
+            m = np.all((ix > 0) & (ix < self.nsteps[None, :] - 1), axis=1)
+            indexes = ix[m].T
+        if indexes.ndim != 2:
+            return None
+        if mask is None:
+            mask = np.array([[c - 1 for a in range(3) for b in range(3) for c in range(3)],
+                             [b - 1 for a in range(3) for b in range(3) for c in range(3)],
+                             [a - 1 for a in range(3) for b in range(3) for c in range(3)]])
+        nb = indexes[:, None, :] + mask[:, :, None]
+        return (nb[0] + self.nsteps[0] * nb[1]
+                + self.nsteps[0] * self.nsteps[1] * nb[2]).astype(np.int64)
+
+    def _check_property(self, property_array):
+        if np.asarray(property_array).shape[0] != self.n_nodes:
+            raise ValueError(f"cannot assign {np.asarray(property_array).shape[0]} vlaues "
+                             f"to array of shape {self.n_nodes}")
+
+    def evaluate_value(self, evaluation_points, property_array):
+        """Trilinear interpolation of node values (:295-322); outside -> NaN."""
+        self._check_property(property_array)
+        c = property_array if isinstance(property_array, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(property_array, dtype=float))
+        pts = _as_points(evaluation_points)
+        v, _ = fs.evaluate(self._lsgrid(), c, torch.from_numpy(pts), True, False)
+        v = v.cpu().numpy()
+        # the support-level call has no NaN-coordinate special case: NaN -> outside
+        v[np.any(np.isnan(pts), axis=1)] = np.nan
+        return v
+
+    def evaluate_gradient(self, evaluation_points, property_array):
+        """Gradient of the trilinear interpolant (:324-391); outside -> 0."""
+        self._check_property(property_array)
+        c = property_array if isinstance(property_array, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(property_array, dtype=float))
+        pts = _as_points(evaluation_points)
+        _, g = fs.evaluate(self._lsgrid(), c, torch.from_numpy(pts), False, True)
+        return g.cpu().numpy()
+
+    def get_operators(self, weights):
+        """(mask, weight) per interior family, cross weights / 4 (:489-510)."""
+        return {"dxy": (Operator.Dxy_mask, weights["dxy"] / 4),
+                "dyz": (Operator.Dyz_mask, weights["dyz"] / 4),
+                "dxz": (Operator.Dxz_mask, weights["dxz"] / 4),
+                "dxx": (Operator.Dxx_mask, weights["dxx"] / 1),
+                "dyy": (Operator.Dyy_mask, weights["dyy"] / 1),
+                "dzz": (Operator.Dzz_mask, weights["dzz"] / 1)}
+
+    def to_dict(self):
+        return {"type": int(self.type), **super().to_dict()}
+
+    def vtk(self, node_properties=None, cell_properties=None):
+        try:
+            import pyvista  # noqa: F401
+        except ImportError:
+            raise ImportError("pyvista is required for vtk support")
+        raise NotImplementedError("vtk export is outside this package's scope")

@@ -535,3 +535,23 @@ def field_c1(p):
                   -0.2 * np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y),
                   np.ones_like(x)], axis=1)
     return f, g
+
+
+def compute_weighting(grid_points, constraint_points, alpha=10.0, sigma=1.0):
+    """LI/_finite_difference_interpolator.py:66-88 (scipy KDTree)."""
+    from scipy.spatial import KDTree
+
+    distances, _ = KDTree(constraint_points).query(grid_points, k=1)
+    return 1 + alpha * np.exp(-(distances ** 2) / (2 * sigma ** 2))
+
+
+def regularisation_scale(grid: Grid, normal_points, sigma=None):
+    """reset() ones + one compute_weighting term for the inside normal points
+    (LI/_discrete_interpolator.py:313, LI/_finite_difference_interpolator.py:436-444)."""
+    s = np.ones(grid.n_nodes)
+    if normal_points is not None and len(normal_points):
+        ins = inside(grid, normal_points[:, :3])
+        if sigma is None:
+            sigma = grid.nnodes[0] * 10
+        s = s + compute_weighting(grid.nodes(), normal_points[ins, :3], sigma=sigma)
+    return s

@@ -1,0 +1,199 @@
+"""The drop-in FiniteDifferenceInterpolator on the GPU against the CPU oracle.
+
+Solutions are compared with the unique least-squares minimiser of the
+reference system (spsolve on the oracle's explicit normal equations);
+evaluation of a given node vector is bit-identical to the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import fd as ofd
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(cells, origin=(0.0, 0.0, 0.0), step=None):
+    import loopstructural_amd as L
+
+    step = step if step is not None else (1.0 / cells,) * 3
+    return L.StructuredGrid(origin=np.array(origin), nsteps=np.array([cells] * 3),
+                            step_vector=np.array(step))
+
+
+def _data(seed, nv=200, nn=50):
+    from loopstructural_amd import synthetic
+
+    values, normals = synthetic.constraints(nv, nn, seed)
+    rng = np.random.default_rng(seed)
+    values[:, 4] = rng.uniform(0.5, 2.0, nv)
+    normals[:, 6] = rng.uniform(0.5, 2.0, nn)
+    return values, normals
+
+
+def _oracle(cells, data, **kw):
+    og = ofd.Grid((0, 0, 0), (cells,) * 3, (1.0 / cells,) * 3)
+    return og, ofd.assemble_fd(og, {k: v.copy() for k, v in data.items()}, **kw)
+
+
+def _relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_fdi_value_normal_matches_minimiser():
+    import loopstructural_amd as L
+
+    values, normals = _data(1)
+    interp = L.FiniteDifferenceInterpolator(_grid(14))
+    interp.set_value_constraints(values)
+    interp.set_normal_constraints(normals)
+    report = interp.setup_interpolator(regularisation=0.1)
+    assert report.families["value"].row_count == 200
+    assert report.families["normal"].row_count == 150
+    assert interp.solve_system("cg", solver_kwargs={"rtol": 1e-12, "atol": 0.0})
+    og, ref = _oracle(14, {"value": values, "normal": normals}, regularisation=0.1)
+    cx = ofd.solve_exact(ref)
+    assert _relerr(interp.c, cx) <= 1e-8
+    t = interp.get_last_solve_timing()
+    for k in ("solver", "backend", "assembly_seconds", "preprocess_seconds", "solve_seconds",
+              "total_seconds", "matrix_rows", "matrix_cols", "matrix_nnz", "up_to_date"):
+        assert k in t
+    A, _ = ref.build_matrix()
+    assert t["matrix_rows"] == A.shape[0] and t["matrix_cols"] == og.n_nodes
+    # evaluation: bit-identical to the oracle on the same node vector
+    pts = np.random.default_rng(5).uniform(-0.1, 1.1, (3000, 3))
+    pts[::50, 2] = np.nan
+    v = interp.evaluate_value(pts)
+    rv = ofd.evaluate_value(og, interp.c, pts)
+    assert np.array_equal(np.isnan(v), np.isnan(rv))
+    assert np.array_equal(v[~np.isnan(v)], rv[~np.isnan(rv)])
+    assert np.all(v[::50] == 0.0)  # NaN coordinates -> 0
+    g = interp.evaluate_gradient(pts)
+    assert np.array_equal(g, ofd.evaluate_gradient(og, interp.c, pts))
+
+
+def test_fdi_all_data_families_match_minimiser():
+    import loopstructural_amd as L
+
+    rng = np.random.default_rng(7)
+    values, normals = _data(7, 120, 30)
+    grads = np.column_stack([rng.uniform(0, 1, (25, 3)), rng.standard_normal((25, 3)),
+                             np.ones(25)])
+    tang = np.column_stack([rng.uniform(0, 1, (20, 3)), rng.standard_normal((20, 3)),
+                            np.ones(20)])
+    iface = np.column_stack([rng.uniform(0, 1, (30, 3)), rng.integers(0, 3, 30), np.ones(30)])
+    interp = L.FiniteDifferenceInterpolator(_grid(10))
+    interp.set_value_constraints(values)
+    interp.set_normal_constraints(normals)
+    interp.set_gradient_constraints(grads)
+    interp.set_tangent_constraints(tang)
+    interp.set_interface_constraints(iface)
+    interp.setup_interpolator(regularisation=0.3, cpw=2.0, gpw=0.7, tpw=0.4, ipw=1.5, dx=0.5)
+    names = list(interp.constraints)
+    assert names[:7] == ["norm_0", "norm_1", "norm_2", "gradient", "gradient_0", "value",
+                         "gradient orthogonal"]
+    assert names[-6:] == ["dx_lower", "dx_upper", "dy_lower", "dy_upper", "dz_lower",
+                          "dz_upper"]
+    interp.solve_system(solver_kwargs={"rtol": 1e-12})
+    w = ofd.default_weights()
+    w.update({"cpw": 2.0, "gpw": 0.7, "tpw": 0.4, "ipw": 1.5, "dx": 0.5})
+    og, ref = _oracle(10, {"value": values, "normal": normals, "gradient": grads,
+                           "tangent": tang, "interface": iface}, weights=w, regularisation=0.3)
+    assert list(ref.constraints) == names
+    cx = ofd.solve_exact(ref)
+    assert _relerr(interp.c, cx) <= 1e-8
+    # stored gradient vectors were normalised in place, as in the reference
+    g = interp.get_gradient_constraints()[:, 3:6]
+    assert np.allclose(np.linalg.norm(g, axis=1), 1.0)
+
+
+def test_border_rows_counts():
+    """packages/loop_interpolation/tests/test_discrete_interpolator.py:22-51."""
+    import loopstructural_amd as L
+
+    grid = L.StructuredGrid(origin=np.zeros(3), nsteps=np.array([4, 4, 4]),
+                            step_vector=np.ones(3))
+    interp = L.FiniteDifferenceInterpolator(grid)
+    interp.setup_interpolator(dxy=0.0, dyz=0.0, dxz=0.0, dxx=0.0, dyy=0.0, dzz=0.0, dx=1.0,
+                              dy=1.0, dz=1.0, cpw=0.0, gpw=0.0, npw=0.0, tpw=0.0, ipw=0.0)
+    expected = grid.nsteps[1] * grid.nsteps[2]
+    for name in ["dx_lower", "dx_upper", "dy_lower", "dy_upper", "dz_lower", "dz_upper"]:
+        assert name in interp.constraints
+        assert interp.constraints[name]["matrix"].shape[0] == expected
+
+
+def test_regularisation_weight_scale_matches_oracle():
+    import loopstructural_amd as L
+
+    values, normals = _data(11, 80, 20)
+    interp = L.FiniteDifferenceInterpolator(_grid(9))
+    interp.set_value_constraints(values)
+    interp.set_normal_constraints(normals)
+    interp.setup_interpolator(regularisation=0.2, use_regularisation_weight_scale=True,
+                              regularisation_weight_sigma=0.15)
+    og = ofd.Grid((0, 0, 0), (9,) * 3, (1 / 9,) * 3)
+    scale = ofd.regularisation_scale(og, normals, sigma=0.15)
+    assert np.allclose(interp.regularisation_scale, scale, rtol=1e-13, atol=0)
+    interp.solve_system(solver_kwargs={"rtol": 1e-12})
+    ref = ofd.assemble_fd(og, {"value": values.copy(), "normal": normals.copy()},
+                          regularisation=0.2, scale=scale)
+    assert _relerr(interp.c, ofd.solve_exact(ref)) <= 1e-8
+
+
+def test_unscaled_cg_and_mixed_precision_reach_minimiser():
+    import loopstructural_amd as L
+
+    values, normals = _data(13)
+    og, ref = _oracle(12, {"value": values, "normal": normals}, regularisation=0.1)
+    cx = ofd.solve_exact(ref)
+    interp = L.FiniteDifferenceInterpolator(_grid(12))
+    interp.set_value_constraints(values)
+    interp.set_normal_constraints(normals)
+    interp.setup_interpolator(regularisation=0.1)
+    interp.apply_scaling_matrix = False
+    interp.solve_system(solver_kwargs={"rtol": 1e-13, "maxiter": 200000})
+    assert _relerr(interp.c, cx) <= 1e-6
+    interp.apply_scaling_matrix = True
+    interp.solve_system(solver_kwargs={"rtol": 1e-11, "precision": "fp32"})
+    assert _relerr(interp.c, cx) <= 1e-6
+    t = interp.get_last_solve_timing()
+    assert t["refinement_sweeps"] >= 1
+
+
+def test_lazy_update_and_linear_field_recovery():
+    """A linear field is reproduced: the regulariser annihilates it and the
+    data are consistent (tests/integration/test_interpolator.py:109-118 style)."""
+    import loopstructural_amd as L
+
+    rng = np.random.default_rng(3)
+    p = rng.uniform(0, 1, (300, 3))
+    f = 0.3 * p[:, 0] - 0.7 * p[:, 1] + p[:, 2]
+    interp = L.FiniteDifferenceInterpolator(_grid(12))
+    interp.set_value_constraints(np.column_stack([p, f]))
+    # evaluate triggers update(): setup + solve with the stored weights
+    q = rng.uniform(0.05, 0.95, (500, 3))
+    v = interp.evaluate_value(q)
+    assert interp.up_to_date
+    assert np.allclose(v, 0.3 * q[:, 0] - 0.7 * q[:, 1] + q[:, 2], atol=1e-3)
+    g = interp.evaluate_gradient(q)
+    assert np.allclose(g, [0.3, -0.7, 1.0], atol=1e-2)
+
+
+def test_support_helpers_match_oracle():
+    """StructuredGrid methods on the GPU = the oracle's numpy (reference KATs:
+    tests/unit/interpolator/test_discrete_supports.py:106-112, 33-50)."""
+    import loopstructural_amd as L
+
+    grid = L.StructuredGrid()
+    lc = grid.position_to_local_coordinates(np.array([[1.2, 1.5, 1.7]]))
+    assert np.allclose(lc, [[0.2, 0.5, 0.7]])
+    assert np.sum(grid.barycentre[:, 0] - grid.evaluate_value(grid.barycentre,
+                                                              grid.nodes[:, 0])) == 0
+    vec = grid.evaluate_gradient(grid.barycentre, grid.nodes[:, 1])
+    assert np.sum(vec - np.array([0, 1, 0])) == 0
+    og = ofd.Grid(grid.origin, grid.nsteps_cells, grid.step_vector)
+    pts = np.random.default_rng(0).uniform(-1, 11, (1000, 3))
+    idc, ins = grid.position_to_cell_corners(pts)
+    ridc, rins = ofd.cell_corners(og, pts)
+    assert np.array_equal(idc, ridc) and np.array_equal(ins, rins)
+    assert np.array_equal(grid.position_to_dof_coefs(pts[rins]),
+                          ofd.trilinear(ofd.local_coords(og, pts[rins])))
