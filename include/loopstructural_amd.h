@@ -176,6 +176,11 @@ int lsamd_fd_evaluate(const lsamd_grid* g, const double* c, const double* pts, i
 int lsamd_fd_nearest_d2(const lsamd_grid* g, const double* pts, int64_t m, double* d2,
                         void* stream);
 
+/* A20: out (nnodes) = E^T gv, E the (n x nnodes) trilinear evaluation matrix
+ * of the points (outside points contribute nothing); deterministic. */
+int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
+                              const double* gv, double* out, void* stream);
+
 /* A20: per data row r of the system (set_rows order), with u = lambda . a_r
  * and the row's normalised coefficients a_r, rhs b_r, weight w_r:
  *   g_b[r] = w_r^2 u            (d loss / d b_hat_r)

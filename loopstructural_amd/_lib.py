@@ -76,6 +76,7 @@ SIGNATURES = {
     "lsamd_rows_adjoint": [_P, _P, _P, _P, _P, _P],
     "lsamd_rows_normalise": [_I64, _I32, _P, _P, _P, _P],
     "lsamd_fd_nearest_d2": [_P, _P, _I64, _P, _P],
+    "lsamd_fd_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
     "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],
 }

@@ -446,6 +446,23 @@ __global__ void k_fill_one(double* a, float* b, int64_t n) {
     b[i] = 1.0f;
   }
 }
+void launch_eval_entries(const GridDev& g, const double* pts, const double* gv, int64_t n,
+                         int32_t* key, int32_t* idx, double* val, hipStream_t st);
+
+// one thread per run of equal keys in the sorted entries: out[key] = sum of
+// the run's values in entry order
+__global__ void k_eval_segsum(const int32_t* __restrict__ key, const int32_t* __restrict__ idx,
+                              const double* __restrict__ val, int64_t ne,
+                              double* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ne) return;
+  const int32_t k = key[s];
+  if (s > 0 && key[s - 1] == k) return;
+  double acc = 0.0;
+  for (int64_t t = s; t < ne && key[t] == k; ++t) acc += val[idx[t]];
+  out[k] += acc;
+}
+
 __global__ void k_zero_d(double* v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = 0.0;
@@ -1346,6 +1363,56 @@ int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t 
   hipEventDestroy(e1);
   LSAMD_CHECK_LAUNCH("time_stencil");
   *avg_ms = total / iters;
+  return LSAMD_OK;
+}
+
+int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n, const double* gv,
+                              double* out, void* stream) {
+  // out = E^T gv with E the trilinear evaluation matrix of the points (outside
+  // points contribute nothing).  Deterministic: entries are sorted by node and
+  // summed in point order.
+  if (int rc = injected("evaluate_adjoint")) return rc;
+  if (int rc = check_grid(g)) return rc;
+  if (n < 0 || !out || (n > 0 && (!pts || !gv)))
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  if (nn > 2147483647LL) return fail(LSAMD_ERR_ARG, "grid exceeds int32 node ids");
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, out, nn);
+  if (n == 0) return LSAMD_OK;
+  const int64_t ne = 8 * n;
+  if (ne >= 2147483647LL) return fail(LSAMD_ERR_ARG, "too many points");
+  int32_t *key = nullptr, *key_s = nullptr, *idx = nullptr, *idx_s = nullptr;
+  double* val = nullptr;
+  void* tmp = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)key, (void*)key_s, (void*)idx, (void*)idx_s, (void*)val, tmp})
+      if (p) hipFree(p);
+  };
+  if (hipMalloc(&key, ne * 4) != hipSuccess || hipMalloc(&key_s, ne * 4) != hipSuccess ||
+      hipMalloc(&idx, ne * 4) != hipSuccess || hipMalloc(&idx_s, ne * 4) != hipSuccess ||
+      hipMalloc(&val, ne * 8) != hipSuccess) {
+    cleanup();
+    return fail(LSAMD_ERR_ALLOC, "evaluate_adjoint: hipMalloc failed");
+  }
+  launch_eval_entries(to_dev(*g), pts, gv, n, key, idx, val, st);
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
+  size_t tb = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)ne, 0, bits, st);
+  if (hipMalloc(&tmp, tb + 8) != hipSuccess) {
+    cleanup();
+    return fail(LSAMD_ERR_ALLOC, "evaluate_adjoint: hipMalloc failed");
+  }
+  hipError_t e =
+      hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_s, idx, idx_s, (int)ne, 0, bits, st);
+  if (e == hipSuccess)
+    hipLaunchKernelGGL(k_eval_segsum, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, key_s,
+                       idx_s, val, ne, out);
+  hipError_t e2 = hipGetLastError();
+  hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess || e2 != hipSuccess) return fail(LSAMD_ERR_HIP, "evaluate_adjoint failed");
   return LSAMD_OK;
 }
 

@@ -380,7 +380,35 @@ __global__ void k_rows_normalise(int64_t nrows, int k, double* __restrict__ coef
   if (norm) norm[r] = len;
 }
 
+// (node, point entry, w_k * gv) for the 8 corners of every point; outside
+// points give zero contributions to node 0.
+__global__ void k_eval_entries(GridDev g, const double* __restrict__ pts,
+                               const double* __restrict__ gv, int64_t n,
+                               int32_t* __restrict__ key, int32_t* __restrict__ idx,
+                               double* __restrict__ val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Located L = locate(g, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  int64_t id[8];
+  corner_ids(g, L, id);
+  double a[8];
+  trilinear(L.lx, L.ly, L.lz, a);
+  for (int k = 0; k < 8; ++k) {
+    const int64_t e = 8 * i + k;
+    key[e] = L.inside ? (int32_t)id[k] : 0;
+    idx[e] = (int32_t)e;
+    val[e] = L.inside ? a[k] * gv[i] : 0.0;
+  }
+}
+
 }  // namespace
+
+void launch_eval_entries(const GridDev& g, const double* pts, const double* gv, int64_t n,
+                         int32_t* key, int32_t* idx, double* val, hipStream_t st) {
+  hipLaunchKernelGGL(k_eval_entries, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, g, pts,
+                     gv, n, key, idx, val);
+}
+
 }  // namespace lsamd
 
 using namespace lsamd;
