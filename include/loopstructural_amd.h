@@ -189,6 +189,73 @@ int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
 int lsamd_rows_adjoint(lsamd_fd_system* s, const double* c, const double* lambda, double* g_b,
                        double* g_w, void* stream);
 
+/* ------------------------------------------------------------------ P1 ----
+ * Structured 5-tet mesh on the grid g (nnodes = cells + 1 per axis): tet id
+ * = local + 5 * cell, cells x fastest; node ids as above.
+ * LC/supports/_3d_structured_tetra.py, LI/_p1interpolator.py. */
+
+/* A16: get_element_for_location (_3d_structured_tetra.py:274-365, strict
+ * inside :264-272).  Outputs (any may be NULL): tet (n) int64 (-1 outside),
+ * nodes (n,4) int64 tet corner ids (-1), bc (n,4) barycentric coordinates
+ * (NaN), grad (n,3,4) d bc_k / d x_d (NaN; get_element_gradient_for_location
+ * :502-545), inside (n) uint8.  Tet choice and bc are bit-identical to the
+ * reference; grad agrees to a few ulp (adjugate vs LU inverse). */
+int lsamd_p1_locate(const lsamd_grid* g, const double* pts, int64_t n, int64_t* tet,
+                    int64_t* nodes, double* bc, double* grad, uint8_t* inside, void* stream);
+
+/* A15: get_elements (:376-399) -> elements (ntet,4) and get_neighbours
+ * (:626-740) -> neighbours (ntet,4), -1 = none; either may be NULL. */
+int lsamd_p1_elements(const lsamd_grid* g, int64_t* elements, int64_t* neighbours, void* stream);
+
+/* A16: get_element_gradients(tets) (:421-486) -> grads (n,3,4); size (n,
+ * nullable) = element_size (:80-94).  Negative ids count from the end. */
+int lsamd_p1_tet_gradients(const lsamd_grid* g, const int64_t* tets, int64_t n, double* grads,
+                           double* size, void* stream);
+
+/* A15: number of face-sharing tet pairs (host only; -1 on a bad grid). */
+int64_t lsamd_p1_n_pairs(const lsamd_grid* g);
+
+/* A15/A17: the face table (_init_face_table :120-184) and the edge-jump rows
+ * (minimise_edge_jumps, LI/_p1interpolator.py:115-166), one per pair, in the
+ * reference's pair order.  Any output may be NULL: pairs (P,2) (larger tet
+ * first), faces (P,3) shared node ids ascending, cols (P,8) int32,
+ * coef (P,8) normalised by the 8-entry norm, w (P) = cgw, or with scale (n
+ * nodes, nullable) 0.5 * (mean scale over T1 + mean over T2) * cgw
+ * (:150-154), norm (P) the 8-entry norm.  The face normal is taken over the
+ * ascending nodes; the reference's node order may differ, which flips the
+ * sign of a row and leaves its normal equations unchanged. */
+int lsamd_p1_pairs(const lsamd_grid* g, int64_t* pairs, int64_t* faces, int32_t* cols,
+                   double* coef, double* w, double* norm, const double* scale, double cgw,
+                   void* stream);
+
+/* A17: rows of n points INSIDE the mesh (lsamd_p1_locate), pts (n, ld) FP64:
+ * x y z | value or vector.  VALUE: 1 row of 4 (bc), b = v (:98-113).
+ * NORMAL: 3 rows of 4 per point, point-major (row = 3i + d), b = n_d
+ * (:73-96).  TANGENT: 1 row (unit vector . grad), b = bval, weight scaled by
+ * the tet volume (add_gradient_orthogonal_constraints :248-296).  Rows are
+ * normalised (a/|a|, b/|a|); w = wscale (point weights are ignored, as in
+ * the reference); norm (nullable) = |a|. */
+int lsamd_p1_build_rows(const lsamd_grid* g, int32_t kind, const double* pts, int64_t n,
+                        int64_t ld, double wscale, double bval, int32_t* cols, double* coef,
+                        double* b, double* w, double* norm, void* stream);
+
+/* A18: evaluate_value / evaluate_gradient (:208-262); NaN outside (strict)
+ * for both.  values (n) and grads (n,3) nullable. */
+int lsamd_p1_evaluate(const lsamd_grid* g, const double* c, const double* pts, int64_t n,
+                      double* values, double* grads, void* stream);
+
+/* A17: the P1 normal-equations system N = A^T W^2 A + cgw^2 E^T E + ridge2 I
+ * with E the normalised edge-jump rows, applied matrix-free by a 21-tap table
+ * stencil.  The handle works with every lsamd_fd_system_* call (set_rows,
+ * apply, rhs, diag, solve, time_stencil, rows_adjoint, destroy). */
+int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
+                           lsamd_fd_system** out);
+
+/* Introspection (host memory, no GPU work): the 21-tap coefficient table of
+ * the edge-jump operator, tab[((vz*nv[1] + vy)*nv[0] + vx)*21 + q], tab of at
+ * least 343*21 doubles; nv (3) the virtual axis lengths. */
+int lsamd_p1_operator_table(const lsamd_grid* g, double cgw, double* tab, int32_t* nv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,14 +6,14 @@ from .constraints import (DtypeError, FiniteValueError, ShapeError, UnsupportedC
 from .diagnostics import (ConstraintDiagnosticsReport, ConstraintFamilyDiagnostics,
                           RegionCoverageDiagnostics)
 from .interpolators import (DiscreteInterpolator, FiniteDifferenceInterpolator,
-                            GeologicalInterpolator, InterpolatorType)
-from .supports import Operator, StructuredGrid, SupportType
+                            GeologicalInterpolator, InterpolatorType, P1Interpolator)
+from .supports import Operator, StructuredGrid, SupportType, TetMesh
 
 __all__ = [
     "ConstraintDiagnosticsReport", "ConstraintFamilyDiagnostics", "DiscreteInterpolator",
     "DtypeError", "FiniteDifferenceInterpolator", "FiniteValueError", "GeologicalInterpolator",
-    "InterpolatorType", "Operator", "RegionCoverageDiagnostics", "ShapeError", "StructuredGrid",
-    "SupportType", "UnsupportedCombinationError", "ValidationError", "VectorError",
+    "InterpolatorType", "Operator", "P1Interpolator", "RegionCoverageDiagnostics", "ShapeError", "StructuredGrid",
+    "SupportType", "TetMesh", "UnsupportedCombinationError", "ValidationError", "VectorError",
     "WeightError",
 ]
 __version__ = "0.1.0"

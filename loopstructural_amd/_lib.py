@@ -79,8 +79,17 @@ SIGNATURES = {
     "lsamd_fd_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
     "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],
+    "lsamd_p1_locate": [_P, _P, _I64, _P, _P, _P, _P, _P, _P],
+    "lsamd_p1_elements": [_P, _P, _P, _P],
+    "lsamd_p1_tet_gradients": [_P, _P, _I64, _P, _P, _P],
+    "lsamd_p1_n_pairs": [_P],
+    "lsamd_p1_pairs": [_P, _P, _P, _P, _P, _P, _P, _P, _D, _P],
+    "lsamd_p1_build_rows": [_P, _I32, _P, _I64, _I64, _D, _D, _P, _P, _P, _P, _P, _P],
+    "lsamd_p1_evaluate": [_P, _P, _P, _I64, _P, _P, _P],
+    "lsamd_p1_system_create": [_P, _D, _D, _P],
+    "lsamd_p1_operator_table": [_P, _D, _P, _P],
 }
-_RESTYPE = {"lsamd_last_error": C.c_char_p}
+_RESTYPE = {"lsamd_last_error": C.c_char_p, "lsamd_p1_n_pairs": C.c_int64}
 
 _lib = None
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "fd_operator.hpp"
+#include "p1_mesh.hpp"
 
 namespace lsamd {
 
@@ -293,6 +294,61 @@ __global__ __launch_bounds__(256) void k_st_general(const T* __restrict__ x, T* 
   }
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// P1 edge-jump regulariser: 21-tap table stencil, one node per thread.  The
+// table row of a node is chosen by its virtual index per axis (p1_virtual);
+// taps falling outside the grid have zero coefficients and are not read.
+template <typename T>
+__global__ __launch_bounds__(256) void k_p1_stencil(const T* __restrict__ x, T* __restrict__ y,
+                                                    GridDev g, const T* __restrict__ tab, int nvx,
+                                                    int nvy, T ridge2, double* __restrict__ part,
+                                                    const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double shm[4];
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dot = 0.0;
+  if (id < n) {
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const int k = (int)(id / nxy);
+    const int64_t r = id - (int64_t)k * nxy;
+    const int j = (int)(r / g.nx);
+    const int i = (int)(r - (int64_t)j * g.nx);
+    const int vx = p1_virtual(i, g.nx), vy = p1_virtual(j, g.ny), vz = p1_virtual(k, g.nz);
+    const T* c = tab + (((int64_t)vz * nvy + vy) * nvx + vx) * kP1Taps;
+    const T xc = x[id];
+    T acc = c[0] * xc;
+#pragma unroll
+    for (int q = 1; q < kP1Taps; ++q) {
+      int d[3];
+      p1_tap_offset(q, d);
+      const bool ok = (unsigned)(i + d[0]) < (unsigned)g.nx && (unsigned)(j + d[1]) < (unsigned)g.ny &&
+                      (unsigned)(k + d[2]) < (unsigned)g.nz;
+      const int64_t o = ok ? id + d[0] + (int64_t)g.nx * d[1] + nxy * d[2] : id;
+      const T xv = x[o];
+      acc += c[q] * (ok ? xv : (T)0);
+    }
+    acc += ridge2 * xc;
+    y[id] = acc;
+    dot = (double)xc * (double)acc;
+  }
+  const double bs = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+__global__ void k_p1_diag(GridDev g, const double* __restrict__ tab, int nvx, int nvy,
+                          double ridge2, double* __restrict__ diag) {
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int k = (int)(id / nxy);
+  const int64_t r = id - (int64_t)k * nxy;
+  const int j = (int)(r / g.nx);
+  const int i = (int)(r - (int64_t)j * g.nx);
+  const int vx = p1_virtual(i, g.nx), vy = p1_virtual(j, g.ny), vz = p1_virtual(k, g.nz);
+  diag[id] = tab[(((int64_t)vz * nvy + vy) * nvx + vx) * kP1Taps] + ridge2;
 }
 
 // ----------------------------------------------------------- data rows ----
@@ -671,6 +727,11 @@ struct lsamd_fd_system {
   float* tab32 = nullptr;
   Coef25<double> ku64;
   Coef25<float> ku32;
+  int op = 0;                  // regulariser: 0 FD second derivatives, 1 P1 edge jumps
+  int p1nv[3] = {0, 0, 0};     // P1 virtual axis lengths of the table
+  double* p1tab64 = nullptr;
+  float* p1tab32 = nullptr;
+  double p1_ridge2 = 0.0;
   int zc[2] = {0, 0};          // tuned z-chunk per precision (0: not tuned yet)
   float tune_us[2][3] = {{0}};  // timings of the candidates 8, 16, 32
   // data rows
@@ -753,6 +814,16 @@ RowsDev rows_dev(const lsamd_fd_system* s, bool fp32) {
   return R;
 }
 
+// regulariser (and ridge) part of diag(N)
+void launch_reg_diag(lsamd_fd_system* s, hipStream_t st) {
+  if (s->op == 1)
+    hipLaunchKernelGGL(k_p1_diag, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, s->gd,
+                       (const double*)s->p1tab64, s->p1nv[0], s->p1nv[1], s->p1_ridge2, s->diag);
+  else
+    hipLaunchKernelGGL(k_stencil_diag, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st,
+                       s->gd, make_regw<double>(s->reg), s->scale2, s->diag);
+}
+
 int vec_blocks(int64_t n) {
   int64_t b = ceil_div(n, VB);
   if (b > 4096) b = 4096;
@@ -795,6 +866,13 @@ void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStre
 template <typename T>
 int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st,
                        int zc) {
+  if (s->op == 1) {
+    const bool f32 = sizeof(T) == 4;
+    hipLaunchKernelGGL(k_p1_stencil<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
+                       y, s->gd, f32 ? (const T*)s->p1tab32 : (const T*)s->p1tab64, s->p1nv[0],
+                       s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
+    return ceil_div(s->n, 256);
+  }
   if (!s->regular) {
     hipLaunchKernelGGL(k_st_general<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
                        y, s->gd, make_regw<T>(s->reg), (const double*)s->scale2, s->part, ctl);
@@ -954,6 +1032,31 @@ int run_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const T
   return LSAMD_OK;
 }
 
+// Work buffers, control blocks and the regulariser-only diagonal shared by
+// every system kind.
+int init_common(lsamd_fd_system* s) {
+  s->part_n = std::max<int64_t>(stencil_partials(s, 8), 8192);
+  if (int rc = dalloc(s, &s->part, s->part_n)) return rc;
+  if (int rc = dalloc(s, &s->ctl, 1)) return rc;
+  if (int rc = dalloc(s, &s->ctl_apply, 1)) return rc;
+  LSAMD_HIP(hipMemset(s->ctl, 0, sizeof(CgCtl)));
+  LSAMD_HIP(hipMemset(s->ctl_apply, 0, sizeof(CgCtl)));
+  if (hipHostMalloc((void**)&s->ctl_host, sizeof(CgCtl), hipHostMallocDefault) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed");
+  if (int rc = dalloc(s, &s->rhs, s->n)) return rc;
+  if (int rc = dalloc(s, &s->diag, s->n)) return rc;
+  if (int rc = dalloc(s, &s->minv64, s->n)) return rc;
+  if (int rc = dalloc(s, &s->minv32, s->n)) return rc;
+  // regulariser-only system until rows are set
+  launch_reg_diag(s, 0);
+  hipLaunchKernelGGL(k_minv, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->diag,
+                     s->minv64, s->minv32, s->n);
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->rhs, s->n);
+  LSAMD_CHECK_LAUNCH("system_create");
+  LSAMD_HIP(hipDeviceSynchronize());
+  return LSAMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1018,26 +1121,39 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
       s->ku32.k[q] = (float)tab[bulk + q];
     }
   }
-  s->part_n = std::max<int64_t>(stencil_partials(s, 8), 8192);
-  if (int rc = dalloc(s, &s->part, s->part_n)) return bail(rc);
-  if (int rc = dalloc(s, &s->ctl, 1)) return bail(rc);
-  if (int rc = dalloc(s, &s->ctl_apply, 1)) return bail(rc);
-  LSAMD_HIP(hipMemset(s->ctl, 0, sizeof(CgCtl)));
-  LSAMD_HIP(hipMemset(s->ctl_apply, 0, sizeof(CgCtl)));
-  if (hipHostMalloc((void**)&s->ctl_host, sizeof(CgCtl), hipHostMallocDefault) != hipSuccess)
-    return bail(fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed"));
-  if (int rc = dalloc(s, &s->rhs, s->n)) return bail(rc);
-  if (int rc = dalloc(s, &s->diag, s->n)) return bail(rc);
-  if (int rc = dalloc(s, &s->minv64, s->n)) return bail(rc);
-  if (int rc = dalloc(s, &s->minv32, s->n)) return bail(rc);
-  // regulariser-only system until rows are set
-  hipLaunchKernelGGL(k_stencil_diag, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->gd,
-                     make_regw<double>(s->reg), s->scale2, s->diag);
-  hipLaunchKernelGGL(k_minv, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->diag,
-                     s->minv64, s->minv32, s->n);
-  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->rhs, s->n);
-  LSAMD_CHECK_LAUNCH("system_create");
-  LSAMD_HIP(hipDeviceSynchronize());
+  if (int rc = init_common(s)) return bail(rc);
+  *out = s;
+  return LSAMD_OK;
+}
+
+int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
+                           lsamd_fd_system** out) {
+  if (int rc = injected("p1_system_create")) return rc;
+  if (!out) return fail(LSAMD_ERR_ARG, "lsamd_p1_system_create: NULL argument");
+  *out = nullptr;
+  if (int rc = check_grid(g)) return rc;
+  if (!(cgw >= 0.0) || !(ridge2 >= 0.0))
+    return fail(LSAMD_ERR_ARG, "lsamd_p1_system_create: cgw and ridge2 must be >= 0");
+  lsamd_fd_system* s = new lsamd_fd_system();
+  s->grid = *g;
+  s->gd = to_dev(*g);
+  memset(&s->reg, 0, sizeof(s->reg));
+  s->n = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  s->op = 1;
+  s->p1_ridge2 = ridge2;
+  auto bail = [&](int rc) {
+    lsamd_fd_system_destroy(s);
+    return rc;
+  };
+  std::vector<double> tab(343 * kP1Taps);
+  build_p1_table(*g, cgw * cgw, tab.data(), s->p1nv);
+  const int64_t nt = (int64_t)s->p1nv[0] * s->p1nv[1] * s->p1nv[2] * kP1Taps;
+  std::vector<float> tabf(tab.begin(), tab.begin() + nt);
+  if (int rc = dalloc(s, &s->p1tab64, nt)) return bail(rc);
+  if (int rc = dalloc(s, &s->p1tab32, nt)) return bail(rc);
+  LSAMD_HIP(hipMemcpy(s->p1tab64, tab.data(), nt * 8, hipMemcpyHostToDevice));
+  LSAMD_HIP(hipMemcpy(s->p1tab32, tabf.data(), nt * 4, hipMemcpyHostToDevice));
+  if (int rc = init_common(s)) return bail(rc);
   *out = s;
   return LSAMD_OK;
 }
@@ -1091,8 +1207,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
   s->ndn = 0;
   const int64_t n = s->n;
   // regulariser part of the diagonal, rhs = 0
-  hipLaunchKernelGGL(k_stencil_diag, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->gd,
-                     make_regw<double>(s->reg), s->scale2, s->diag);
+  launch_reg_diag(s, st);
   hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->rhs, n);
   if (nrows > 0) {
     if (int rc = dalloc(s, &s->row_ptr, nrows + 1)) return rc;

@@ -244,3 +244,125 @@ def evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True, g
     _lib.call("lsamd_fd_evaluate", C.byref(grid), _lib.ptr(c), _lib.ptr(pts), n, _lib.ptr(v),
               _lib.ptr(g), _lib.stream())
     return v, g
+
+
+# ------------------------------------------------------------------- P1 ----
+class P1System(FdSystem):
+    """``lsamd_fd_system`` handle whose regulariser is the P1 edge-jump
+    operator cgw^2 E^T E (LI/_p1interpolator.py:115-166), applied by the
+    21-tap table stencil; data rows, solve, apply and adjoint as FdSystem."""
+
+    def __init__(self, grid: _lib.Grid, cgw: float, ridge: float | None):
+        self.dev = _lib.device()
+        self.grid = grid
+        self.reg = None
+        self.n = int(grid.nnodes[0] * grid.nnodes[1] * grid.nnodes[2])
+        self._scale2 = None
+        h = C.c_void_p()
+        _lib.call("lsamd_p1_system_create", C.byref(grid), float(cgw),
+                  float(ridge) ** 2 if ridge else 0.0, C.byref(h))
+        self._h = h
+        self.rows = None
+
+
+def p1_locate(grid: _lib.Grid, pts: torch.Tensor, want_bc=True, want_grad=False):
+    """A16: (tet int64 (n,), nodes (n,4), inside bool, bc (n,4), grad (n,3,4))."""
+    pts = pts.to(_lib.device(), F64).contiguous()
+    n = pts.shape[0]
+    d = pts.device
+    tet = torch.empty(n, dtype=torch.int64, device=d)
+    nodes = torch.empty((n, 4), dtype=torch.int64, device=d)
+    ins = torch.empty(n, dtype=torch.uint8, device=d)
+    bc = torch.empty((n, 4), dtype=F64, device=d) if want_bc else None
+    gr = torch.empty((n, 3, 4), dtype=F64, device=d) if want_grad else None
+    _lib.call("lsamd_p1_locate", C.byref(grid), _lib.ptr(pts), n, _lib.ptr(tet), _lib.ptr(nodes),
+              _lib.ptr(bc), _lib.ptr(gr), _lib.ptr(ins), _lib.stream())
+    return tet, nodes, ins.bool(), bc, gr
+
+
+def p1_elements(grid: _lib.Grid, want_neighbours=False):
+    ntet = 5 * int((grid.nnodes[0] - 1) * (grid.nnodes[1] - 1) * (grid.nnodes[2] - 1))
+    dev = _lib.device()
+    el = torch.empty((ntet, 4), dtype=torch.int64, device=dev)
+    nb = torch.empty((ntet, 4), dtype=torch.int64, device=dev) if want_neighbours else None
+    _lib.call("lsamd_p1_elements", C.byref(grid), _lib.ptr(el), _lib.ptr(nb), _lib.stream())
+    return el, nb
+
+
+def p1_tet_gradients(grid: _lib.Grid, tets: torch.Tensor, want_size=False):
+    tets = tets.to(_lib.device(), torch.int64).contiguous()
+    n = tets.shape[0]
+    g = torch.empty((n, 3, 4), dtype=F64, device=tets.device)
+    s = torch.empty(n, dtype=F64, device=tets.device) if want_size else None
+    if n:
+        _lib.call("lsamd_p1_tet_gradients", C.byref(grid), _lib.ptr(tets), n, _lib.ptr(g),
+                  _lib.ptr(s), _lib.stream())
+    return g, s
+
+
+def p1_n_pairs(grid: _lib.Grid) -> int:
+    n = int(_lib.load().lsamd_p1_n_pairs(C.byref(grid)))
+    if n < 0:
+        _lib.check(_lib.LSAMD_ERR_ARG)
+    return n
+
+
+def p1_pairs(grid: _lib.Grid, want_faces=False):
+    """A15 face table: pairs (P,2) (larger tet first) and faces (P,3)."""
+    P = p1_n_pairs(grid)
+    dev = _lib.device()
+    pairs = torch.empty((P, 2), dtype=torch.int64, device=dev)
+    faces = torch.empty((P, 3), dtype=torch.int64, device=dev) if want_faces else None
+    if P:
+        _lib.call("lsamd_p1_pairs", C.byref(grid), _lib.ptr(pairs), _lib.ptr(faces), None, None,
+                  None, None, None, 0.0, _lib.stream())
+    return pairs, faces
+
+
+def p1_edge_rows(grid: _lib.Grid, cgw: float, dof: int, scale: torch.Tensor | None = None) -> FdRows:
+    """A17 edge-jump rows (8 entries each, normalised, b = 0)."""
+    P = p1_n_pairs(grid)
+    dev = _lib.device()
+    cols = torch.empty((P, 8), dtype=torch.int32, device=dev)
+    coef = torch.empty((P, 8), dtype=F64, device=dev)
+    w = torch.empty(P, dtype=F64, device=dev)
+    norm = torch.empty(P, dtype=F64, device=dev)
+    if scale is not None:
+        scale = scale.to(dev, F64).contiguous()
+    if P:
+        _lib.call("lsamd_p1_pairs", C.byref(grid), None, None, _lib.ptr(cols), _lib.ptr(coef),
+                  _lib.ptr(w), _lib.ptr(norm), _lib.ptr(scale), float(cgw), _lib.stream())
+    b = torch.zeros(P, dtype=F64, device=dev)
+    return fixed_width_rows(cols, coef, b, w, norm, dof)
+
+
+def p1_build_rows(grid: _lib.Grid, kind: int, pts: torch.Tensor, wscale: float, dof: int,
+                  bval: float = 0.0) -> FdRows:
+    """A17 rows (4 entries) for points already known to be inside."""
+    pts = pts.to(_lib.device(), F64).contiguous()
+    n, ld = pts.shape
+    per = 3 if kind == _lib.ROWS_NORMAL else 1
+    R = per * n
+    dev = pts.device
+    cols = torch.empty((R, 4), dtype=torch.int32, device=dev)
+    coef = torch.empty((R, 4), dtype=F64, device=dev)
+    b = torch.empty(R, dtype=F64, device=dev)
+    w = torch.empty(R, dtype=F64, device=dev)
+    norm = torch.empty(R, dtype=F64, device=dev)
+    if n:
+        _lib.call("lsamd_p1_build_rows", C.byref(grid), kind, _lib.ptr(pts), n, ld, float(wscale),
+                  float(bval), _lib.ptr(cols), _lib.ptr(coef), _lib.ptr(b), _lib.ptr(w),
+                  _lib.ptr(norm), _lib.stream())
+    return fixed_width_rows(cols, coef, b, w, norm, dof)
+
+
+def p1_evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True, gradients=False):
+    pts = pts.to(_lib.device(), F64).contiguous()
+    c = c.to(pts.device, F64).contiguous()
+    n = pts.shape[0]
+    v = torch.empty(n, dtype=F64, device=pts.device) if values else None
+    g = torch.empty((n, 3), dtype=F64, device=pts.device) if gradients else None
+    if n:
+        _lib.call("lsamd_p1_evaluate", C.byref(grid), _lib.ptr(c), _lib.ptr(pts), n, _lib.ptr(v),
+                  _lib.ptr(g), _lib.stream())
+    return v, g

@@ -989,3 +989,195 @@ def _row_slice(blk: fs.FdRows, r0: int, r1: int) -> fs.FdRows:
     e0, e1 = r0 * k, r1 * k
     return fs.FdRows(rp, blk.cols[e0:e1], blk.coef[e0:e1], blk.b[r0:r1], blk.w[r0:r1],
                      blk.norm[r0:r1], blk.dof)
+
+
+class EdgeJumpFamily:
+    """The P1 edge-jump regulariser kept matrix-free: one normalised 8-entry
+    row per face-sharing tet pair, uniform weight."""
+
+    def __init__(self, name, weight, nrows, dof):
+        self.name, self.weight, self.nrows, self.dof = name, weight, nrows, dof
+
+    @property
+    def shape(self):
+        return (self.nrows, self.dof)
+
+
+class P1Interpolator(DiscreteInterpolator):
+    """Piecewise-linear interpolator on a TetMesh (LI/_p1interpolator.py:24-300)
+    on the MI355X.  The edge-jump regulariser is applied as a 21-tap stencil
+    (or as explicit rows when per-pair weights are requested)."""
+
+    def __init__(self, mesh):
+        self.shape = "rectangular"
+        DiscreteInterpolator.__init__(self, mesh)
+        self.support = mesh
+        self.interpolation_weights = {"cgw": 0.1, "cpw": 1.0, "npw": 1.0, "gpw": 1.0,
+                                      "tpw": 1.0, "ipw": 1.0}
+        self.type = InterpolatorType.PIECEWISE_LINEAR
+        self.use_regularisation_weight_scale = False
+        self.regularisation_weight_sigma = None
+        self._edge_families = OrderedDict()
+
+    def reset(self):
+        super().reset()
+        self._edge_families = OrderedDict()
+
+    def clear_constraints(self):
+        super().clear_constraints()
+        self._edge_families = OrderedDict()
+
+    def _inside_points(self, points, ncols):
+        """Device copy of the rows of ``points`` inside the mesh (strict box and
+        a containing tet, as evaluate_shape) and the mask."""
+        dev = _lib.device()
+        pts = torch.from_numpy(np.ascontiguousarray(points[:, :ncols], dtype=float)).to(dev)
+        _, _, ins, _, _ = fs.p1_locate(self.support._lsgrid(), pts[:, :3].contiguous(), False)
+        return pts[ins].contiguous(), ins
+
+    def _update_regularisation_scale_from_norm_constraints(self):
+        """:50-68: += 1 + 10 exp(-d^2/2 sigma^2), d to the nearest normal point."""
+        points = self.get_norm_constraints()
+        if points.shape[0] == 0:
+            return
+        pin, ins = self._inside_points(points, 3)
+        if pin.shape[0] == 0:
+            return
+        sigma = self.regularisation_weight_sigma
+        if sigma is None:
+            sigma = getattr(self.support, "nsteps", np.array([1.0], dtype=float))[0] * 10
+        self._scale_sources.append((pin.cpu().numpy(), sigma))
+        self._regularisation_scale = None
+
+    # ------------------------------------------------------------ rows ----
+    def add_gradient_constraints(self, w=1.0):
+        """A no-op in the reference (:70-71)."""
+        pass
+
+    def add_norm_constraints(self, w=1.0):
+        """Three rows grad(lambda)_d per point, b = n_d, weight w (:73-96)."""
+        points = self.get_norm_constraints()
+        if points.shape[0] > 0:
+            pin, _ = self._inside_points(points, 6)
+            blk = fs.p1_build_rows(self.support._lsgrid(), _lib.ROWS_NORMAL, pin, float(w),
+                                   self.dof)
+            self._register_rows(blk, "norm")
+            self.up_to_date = False
+
+    def add_value_constraints(self, w=1.0):
+        """Barycentric rows, b = value, weight w (:98-113)."""
+        points = self.get_value_constraints()
+        if points.shape[0] > 0:
+            pin, _ = self._inside_points(points, 4)
+            blk = fs.p1_build_rows(self.support._lsgrid(), _lib.ROWS_VALUE, pin, float(w),
+                                   self.dof)
+            self._register_rows(blk, "value")
+            self.up_to_date = False
+
+    def add_gradient_orthogonal_constraints(self, points, vectors, w=1.0, b=0,
+                                            name="undefined gradient orthogonal constraint"):
+        """vector . grad = b rows weighted by w * tet volume (:248-296);
+        ``vectors`` is normalised in place."""
+        points = np.asarray(points)
+        if points.shape[0] > 0:
+            vectors = np.asarray(vectors)
+            norm = np.linalg.norm(vectors, axis=1)
+            vectors[norm > 0, :] /= norm[norm > 0, None]
+            arr = np.hstack([np.asarray(points[:, :3], dtype=float),
+                             np.asarray(vectors[:, :3], dtype=float)])
+            pin, ins = self._inside_points(arr, 6)
+            blk = fs.p1_build_rows(self.support._lsgrid(), _lib.ROWS_TANGENT, pin, float(w),
+                                   self.dof, bval=float(b))
+            self._register_rows(blk, name)
+            if pin.shape[0] <= 0:
+                logger.warning(f"{points.shape[0]} gradient constraints not added: outside of "
+                               f"model bounding box")
+            self.up_to_date = False
+
+    def add_interface_constraints(self, w: float = 1):
+        raise NotImplementedError
+
+    def minimise_edge_jumps(self, w=0.1, vector_func=None, vector=None, name="edge jump"):
+        """(n . grad)|T1 - (n . grad)|T2 = 0 over every shared face (:115-166)."""
+        if vector_func is not None or vector is not None:
+            raise NotImplementedError("directional regularisation is not part of this package")
+        grid = self.support._lsgrid()
+        P = fs.p1_n_pairs(grid)
+        if self.use_regularisation_weight_scale:
+            self.regularisation_scale  # materialise on the device
+            blk = fs.p1_edge_rows(grid, float(w), self.dof, self._regularisation_scale)
+            self._register_rows(blk, name)
+        else:
+            name = self._unique_name(name)
+            fam = EdgeJumpFamily(name, float(w), P, self.dof)
+            self._edge_families[name] = fam
+            self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(P), w=np.full(P, float(w)))
+        self.up_to_date = False
+
+    def get_regularisation_sample_points(self):
+        f = self.support.shared_elements
+        return self.support.nodes[f].mean(axis=1)
+
+    # ------------------------------------------------------------ setup --
+    def setup_interpolator(self, **kwargs):
+        """LI/_p1interpolator.py:180-246."""
+        self.reset()
+        reg = kwargs.get("regularisation", None)
+        directional = kwargs.get("directional_regularisation", None)
+        isotropic = None
+        if reg is not None:
+            if np.isscalar(reg):
+                isotropic = float(reg)
+            elif isinstance(reg, dict) and ("isotropic" in reg or "weight" in reg) and \
+                    "direction" not in reg and "vector" not in reg:
+                isotropic = float(reg.get("isotropic", reg.get("weight")))
+                directional = directional or reg.get("directional") or None
+            else:
+                directional = reg
+        if directional:
+            raise NotImplementedError("directional regularisation is not part of this package")
+        self._apply_isotropic_regularisation_weight(isotropic, ("cgw",))
+        self._apply_interpolation_weight_kwargs(
+            kwargs, skip_keys=("regularisation", "directional_regularisation"))
+        self.use_regularisation_weight_scale = kwargs.get("use_regularisation_weight_scale", False)
+        self.regularisation_weight_sigma = kwargs.get("regularisation_weight_sigma", None)
+        if self.use_regularisation_weight_scale:
+            self._update_regularisation_scale_from_norm_constraints()
+        if self.interpolation_weights["cgw"] > 0.0:
+            self.up_to_date = False
+            self.minimise_edge_jumps(self.interpolation_weights["cgw"])
+        self.add_gradient_constraints(self.interpolation_weights["gpw"])
+        self.add_norm_constraints(self.interpolation_weights["npw"])
+        self.add_value_constraints(self.interpolation_weights["cpw"])
+        self.add_tangent_constraints(self.interpolation_weights["tpw"])
+        return self.finalize_setup_diagnostics_report()
+
+    def add_tangent_constraints(self, w=1.0):
+        """LI/_discrete_interpolator.py:668-682."""
+        points = self.get_tangent_constraints()
+        if points.shape[0] > 0:
+            self.add_gradient_orthogonal_constraints(points[:, :3], points[:, 3:6], w)
+
+    # ------------------------------------------------------------ solve --
+    def _build_system(self):
+        rows = self._system_rows()
+        w2 = sum(f.weight ** 2 for f in self._edge_families.values())
+        ridge = float(self.ridge_factor) if self.add_ridge_regulatisation else 0.0
+        system = fs.P1System(self.support._lsgrid(), float(np.sqrt(w2)), ridge)
+        system.set_rows(rows)
+        self._system = system
+        nreg = sum(f.nrows for f in self._edge_families.values())
+        ndata = rows.nrows if rows is not None else 0
+        nnz = (rows.nnz if rows is not None else 0) + 8 * nreg
+        return system, ndata + nreg, nnz
+
+    def _evaluate(self, pts, values, gradients):
+        v, g = fs.p1_evaluate(self.support._lsgrid(), self._c_device(), torch.from_numpy(pts),
+                              values, gradients)
+        v = v.cpu().numpy() if v is not None else None
+        if v is not None:
+            v[np.any(np.isnan(pts[:, :3]), axis=1)] = 0.0
+        return v, (g.cpu().numpy() if g is not None else None)
+
+    def copy(self):
+        return P1Interpolator(self.support)

@@ -441,3 +441,239 @@ class StructuredGrid(BaseStructuredSupport):
         except ImportError:
             raise ImportError("pyvista is required for vtk support")
         raise NotImplementedError("vtk export is outside this package's scope")
+
+
+def _dev_points(pos) -> torch.Tensor:
+    return torch.from_numpy(_as_points(pos)).to(_lib.device())
+
+
+class TetMesh(BaseStructuredSupport):
+    """Structured tetrahedral mesh, 5 tets per cell
+    (LC/supports/_3d_structured_tetra.py:16-740).
+
+    Topology (elements, neighbours, face table), location, barycentric
+    coordinates, element gradients and evaluation come from the P1 HIP
+    kernels; arrays the reference returns as numpy are returned as numpy.
+    """
+
+    tetra_mask_even = np.array([[7, 1, 2, 4], [6, 2, 4, 7], [5, 1, 4, 7], [0, 1, 2, 4],
+                                [3, 1, 2, 7]])
+    tetra_mask = np.array([[0, 6, 5, 3], [7, 3, 5, 6], [4, 0, 5, 6], [2, 0, 3, 6], [1, 0, 3, 5]])
+
+    def __init__(self, origin=None, nsteps_cells=None, step_vector=None, nsteps=None):
+        if nsteps_cells is None:
+            nsteps_cells = np.ones(3) * 10 if nsteps is None else np.array(nsteps)
+        super().__init__(origin, np.asarray(nsteps_cells).astype(int), step_vector)
+        self.type = SupportType.TetMesh
+        self.cg = None
+        self._elements = None
+        self._faces = None
+
+    def onGeometryChange(self):
+        self._elements = None
+        self._faces = None
+        if self.interpolator is not None:
+            self.interpolator.reset()
+
+    # ---------------------------------------------------------- topology --
+    @property
+    def ntetra(self) -> int:
+        return int(np.prod(self.nsteps_cells)) * 5
+
+    @property
+    def n_elements(self) -> int:
+        return self.ntetra
+
+    @property
+    def n_cells(self) -> int:
+        return int(np.prod(self.nsteps_cells))
+
+    @property
+    def elements(self):
+        """(ntetra, 4) node ids (:376-399)."""
+        if self._elements is None:
+            el, _ = fs.p1_elements(self._lsgrid())
+            self._elements = el.cpu().numpy()
+        return self._elements
+
+    def get_elements(self):
+        return self.elements
+
+    def get_neighbours(self) -> np.ndarray:
+        """(ntetra, 4) face neighbours, -1 = none (:626-740)."""
+        _, nb = fs.p1_elements(self._lsgrid(), want_neighbours=True)
+        return nb.cpu().numpy()
+
+    @property
+    def neighbours(self):
+        return self.get_neighbours()
+
+    def _face_table(self):
+        if self._faces is None:
+            pairs, faces = fs.p1_pairs(self._lsgrid(), want_faces=True)
+            self._faces = (pairs, faces)
+        return self._faces
+
+    @property
+    def shared_element_relationships(self):
+        """(P,2) face-sharing tet pairs, larger id first (:120-184)."""
+        return self._face_table()[0].cpu().numpy()
+
+    @property
+    def shared_elements(self):
+        """(P,3) node ids of each shared face (ascending)."""
+        return self._face_table()[1].cpu().numpy()
+
+    def _node_pos(self, ids: torch.Tensor) -> torch.Tensor:
+        """Support node coordinates (np.linspace per axis) of node ids."""
+        n = torch.as_tensor(self.nsteps, device=ids.device)
+        ijk = torch.stack([ids % n[0], ids // n[0] % n[1], ids // n[0] // n[1]], dim=-1)
+        out = []
+        for a in range(3):
+            o = float(self.origin[a])
+            stop = o + float(self.nsteps_cells[a]) * float(self.step_vector[a])
+            step = (stop - o) / float(self.nsteps[a] - 1)
+            v = ijk[..., a].to(torch.float64) * step + o
+            out.append(torch.where(ijk[..., a] == self.nsteps[a] - 1,
+                                   torch.full_like(v, stop), v))
+        return torch.stack(out, dim=-1)
+
+    @property
+    def shared_element_norm(self):
+        """(v1-v0) x (v2-v0) of each shared face (:186-194)."""
+        f = self._face_table()[1]
+        p = self._node_pos(f)
+        return torch.linalg.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]).cpu().numpy()
+
+    @property
+    def shared_element_size(self):
+        return 0.5 * np.linalg.norm(self.shared_element_norm, axis=1)
+
+    @property
+    def shared_element_scale(self):
+        s = self.shared_element_size
+        return s / np.mean(s)
+
+    @property
+    def element_size(self):
+        """|det| / 6 of every tet (:80-94)."""
+        t = torch.arange(self.ntetra, dtype=torch.int64, device=_lib.device())
+        _, s = fs.p1_tet_gradients(self._lsgrid(), t, want_size=True)
+        return s.cpu().numpy()
+
+    @property
+    def element_scale(self):
+        size = self.element_size
+        size -= np.min(size)
+        size /= np.max(size)
+        size += 1.0
+        return size
+
+    @property
+    def barycentre(self):
+        el, _ = fs.p1_elements(self._lsgrid())
+        return (self._node_pos(el).sum(dim=1) / 4.0).cpu().numpy()
+
+    def tetra_global_index(self, indices, tetra_index):
+        """:401-419."""
+        indices = np.asarray(indices)
+        nc = self.nsteps_cells
+        return (tetra_index + indices[:, 0] * 5 + nc[0] * indices[:, 1] * 5
+                + nc[0] * nc[1] * indices[:, 2] * 5)
+
+    # ---------------------------------------------------------- location --
+    def inside(self, pos):
+        """Strict box test (:264-272)."""
+        p = _dev_points(pos)
+        o = torch.as_tensor(self.origin, device=p.device)
+        mx = o + torch.as_tensor(self.step_vector * self.nsteps_cells, device=p.device)
+        return torch.all((p > o) & (p < mx), dim=1).cpu().numpy()
+
+    def position_to_cell_index(self, pos):
+        """Cell ijk of strictly inside points (0 elsewhere) and the flags."""
+        p = _dev_points(pos)
+        o = torch.as_tensor(self.origin, device=p.device)
+        h = torch.as_tensor(self.step_vector, device=p.device)
+        nc = torch.as_tensor(self.nsteps_cells, device=p.device)
+        mx = o + h * nc
+        ins = torch.all((p > o) & (p < mx), dim=1)
+        ci = torch.floor((p - o) / h)
+        ci = torch.where(ins[:, None], ci, torch.zeros_like(ci)).to(torch.int64)
+        ci = torch.minimum(torch.clamp(ci, min=0), nc[None, :] - 1)
+        return ci.cpu().numpy(), ins.cpu().numpy()
+
+    def _p1_locate(self, pos, want_grad=False):
+        return fs.p1_locate(self._lsgrid(), _dev_points(pos), True, want_grad)
+
+    def get_element_for_location(self, pos):
+        """(vertices (N,4,3), bc (N,4), tet (N,), inside (N,)); NaN / -1
+        outside (:274-365)."""
+        tet, nodes, ins, bc, _ = self._p1_locate(pos)
+        nd = nodes.clamp(min=0)
+        ijk = torch.stack([nd % int(self.nsteps[0]), nd // int(self.nsteps[0]) % int(self.nsteps[1]),
+                           nd // int(self.nsteps[0]) // int(self.nsteps[1])], dim=-1)
+        o = torch.as_tensor(self.origin, device=nd.device)
+        h = torch.as_tensor(self.step_vector, device=nd.device)
+        vert = o + h * ijk.to(torch.float64)
+        vert[~ins] = float("nan")
+        return vert.cpu().numpy(), bc.cpu().numpy(), tet.cpu().numpy(), ins.cpu().numpy()
+
+    def evaluate_shape(self, locations):
+        """(bc, tet, inside) (:367-374)."""
+        tet, _, ins, bc, _ = self._p1_locate(locations)
+        return bc.cpu().numpy(), tet.cpu().numpy(), ins.cpu().numpy()
+
+    def get_element_gradients(self, elements=None):
+        """(n,3,4) gradients of the barycentric coordinates (:421-486)."""
+        if elements is None:
+            elements = np.arange(0, self.ntetra)
+        t = torch.as_tensor(np.asarray(elements, dtype=np.int64), device=_lib.device())
+        g, _ = fs.p1_tet_gradients(self._lsgrid(), t.reshape(-1))
+        return g.reshape(*t.shape, 3, 4).cpu().numpy()
+
+    def evaluate_shape_derivatives(self, pos, elements=None):
+        """(gradients, elements, inside) (:488-500)."""
+        if elements is not None:
+            inside = np.ones(np.asarray(elements).shape[0], dtype=bool)
+        else:
+            _, _, elements, inside = self.get_element_for_location(pos)
+        return self.get_element_gradients(elements), elements, inside
+
+    def get_element_gradient_for_location(self, pos):
+        """(vertices, gradients (N,3,4) NaN outside, tet, inside) (:502-545)."""
+        vert, _, tet, ins = self.get_element_for_location(pos)
+        _, _, _, _, g = self._p1_locate(pos, want_grad=True)
+        return vert, g.cpu().numpy(), tet, ins
+
+    # -------------------------------------------------------- evaluation --
+    def _check_property(self, property_array):
+        if np.asarray(property_array).shape[0] != self.n_nodes:
+            raise ValueError(f"cannot assign {np.asarray(property_array).shape[0]} vlaues "
+                             f"to array of shape {self.n_nodes}")
+
+    def _prop(self, property_array):
+        if isinstance(property_array, torch.Tensor):
+            return property_array
+        return torch.from_numpy(np.ascontiguousarray(property_array, dtype=float))
+
+    def evaluate_value(self, pos, property_array):
+        """Barycentric interpolation (:208-229); NaN outside (strict)."""
+        v, _ = fs.p1_evaluate(self._lsgrid(), self._prop(property_array), _dev_points(pos),
+                              True, False)
+        return v.cpu().numpy()
+
+    def evaluate_gradient(self, pos, property_array):
+        """Element gradient (:231-262); NaN outside."""
+        _, g = fs.p1_evaluate(self._lsgrid(), self._prop(property_array), _dev_points(pos),
+                              False, True)
+        return g.cpu().numpy()
+
+    def to_dict(self):
+        return {"type": int(self.type), **super().to_dict()}
+
+    def vtk(self, node_properties=None, cell_properties=None):
+        try:
+            import pyvista  # noqa: F401
+        except ImportError:
+            raise ImportError("pyvista is required for vtk support")
+        raise NotImplementedError("vtk export is outside this package's scope")
