@@ -251,6 +251,31 @@ int lsamd_p1_evaluate(const lsamd_grid* g, const double* c, const double* pts, i
 int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
                            lsamd_fd_system** out);
 
+/* ---------------------------------------------------------------- batch ----
+ * An ensemble of FD systems with different grids (every axis >= 7 nodes) and
+ * one set of regulariser weights, solved together: the node vectors are
+ * concatenated (system s owns [off_s, off_s+1)), every launch covers all
+ * systems, CG scalars and stopping tests are per system and converged
+ * systems drop out.  No reference counterpart (the reference solves one
+ * system per call, LI/_discrete_interpolator.py:1221-1319); each system's
+ * result equals its single-system solve.  The handle also accepts
+ * lsamd_fd_system_apply / _rhs / _diag / _time_stencil / _destroy. */
+int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg* reg,
+                       lsamd_fd_system** out);
+/* node_off (nsys+1, host): first node of each system, total last. */
+int lsamd_batch_offsets(lsamd_fd_system* s, int64_t* node_off);
+/* Rows of all systems as one CSR with GLOBAL node ids, system s owning rows
+ * [sys_rows[s], sys_rows[s+1]) (host array, nsys+1); a row may only address
+ * its own system's nodes (checked: LSAMD_ERR_ARG). */
+int lsamd_batch_set_rows(lsamd_fd_system* s, const int64_t* sys_rows, int64_t nrows, int64_t nnz,
+                         const int64_t* row_ptr, const int32_t* cols, const double* coef,
+                         const double* b, const double* w, void* stream);
+/* Solve every system (opts as lsamd_fd_system_solve, jacobi must be 1;
+ * maxiter <= 0: 10 x the largest system); c (total nodes) FP64; info (nsys,
+ * host, nullable) per system. */
+int lsamd_batch_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, const double* rhs,
+                      double* c, lsamd_solve_info* info, void* stream);
+
 /* Introspection (host memory, no GPU work): the 21-tap coefficient table of
  * the edge-jump operator, tab[((vz*nv[1] + vy)*nv[0] + vx)*21 + q], tab of at
  * least 343*21 doubles; nv (3) the virtual axis lengths. */

@@ -88,6 +88,10 @@ SIGNATURES = {
     "lsamd_p1_evaluate": [_P, _P, _P, _I64, _P, _P, _P],
     "lsamd_p1_system_create": [_P, _D, _D, _P],
     "lsamd_p1_operator_table": [_P, _D, _P, _P],
+    "lsamd_batch_create": [_I32, _P, _P, _P],
+    "lsamd_batch_offsets": [_P, _P],
+    "lsamd_batch_set_rows": [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P],
+    "lsamd_batch_solve": [_P, _P, _P, _P, _P, _P],
 }
 _RESTYPE = {"lsamd_last_error": C.c_char_p, "lsamd_p1_n_pairs": C.c_int64}
 

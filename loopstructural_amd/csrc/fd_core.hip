@@ -708,6 +708,25 @@ __global__ void k_copy_d(const double* __restrict__ s, double* __restrict__ d, i
 
 }  // namespace lsamd
 
+// entry columns must address nodes of the system ([lo, hi)); *bad set otherwise
+__global__ void k_check_cols(const int32_t* __restrict__ cols, int64_t nnz, int64_t lo, int64_t hi,
+                             int* __restrict__ bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz && (cols[e] < lo || cols[e] >= hi)) *bad = 1;
+}
+
+#include "batch_kernels.hpp"
+
+// rows of system s may only address its own nodes; one workgroup per system
+namespace lsamd {
+__global__ void k_b_check_rows(RowsDev R, const BSys* __restrict__ sys, int* __restrict__ bad) {
+  const BSys S = sys[blockIdx.x];
+  for (int64_t r = S.r0 + threadIdx.x; r < S.r1; r += blockDim.x)
+    for (int64_t e = R.row_ptr[r]; e < R.row_ptr[r + 1]; ++e)
+      if (R.cols[e] < S.off || R.cols[e] >= S.off + S.n) *bad = 1;
+}
+}  // namespace lsamd
+
 // ================================================================ host side ==
 using namespace lsamd;
 
@@ -732,6 +751,21 @@ struct lsamd_fd_system {
   double* p1tab64 = nullptr;
   float* p1tab32 = nullptr;
   double p1_ridge2 = 0.0;
+  // batched ensemble (op == 2): per-system descriptors and node-block table
+  int nsys = 0;
+  std::vector<BSys> bsys_h;
+  BSys* bsys = nullptr;
+  int* blk_sys = nullptr;
+  int nblk = 0;
+  CgCtl* bctl = nullptr;        // per-system PCG control
+  CgCtl* bctl_apply = nullptr;  // all-zero (no system done): standalone apply
+  CgCtl* bctl64 = nullptr;      // FP64 thresholds of the mixed-precision solve
+  CgCtl* bctl_host = nullptr;   // pinned mirror
+  double* u_rows = nullptr;     // per data row: t_r (a_r . p)
+  int* odone = nullptr;         // per system: outer refinement converged
+  double* bres = nullptr;       // per system: outer residual
+  hipGraphExec_t bgraph[2] = {nullptr, nullptr};
+  int bgraph_chunk[2] = {0, 0};
   int zc[2] = {0, 0};          // tuned z-chunk per precision (0: not tuned yet)
   float tune_us[2][3] = {{0}};  // timings of the candidates 8, 16, 32
   // data rows
@@ -816,7 +850,11 @@ RowsDev rows_dev(const lsamd_fd_system* s, bool fp32) {
 
 // regulariser (and ridge) part of diag(N)
 void launch_reg_diag(lsamd_fd_system* s, hipStream_t st) {
-  if (s->op == 1)
+  if (s->op == 2)
+    hipLaunchKernelGGL(k_b_diag, dim3((unsigned)s->nblk), dim3(256), 0, st,
+                       (const BSys*)s->bsys, (const int*)s->blk_sys, (const double*)s->tab64,
+                       s->diag);
+  else if (s->op == 1)
     hipLaunchKernelGGL(k_p1_diag, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, s->gd,
                        (const double*)s->p1tab64, s->p1nv[0], s->p1nv[1], s->p1_ridge2, s->diag);
   else
@@ -834,6 +872,7 @@ int vec_blocks(int64_t n) {
 constexpr int kZcCand[3] = {8, 16, 32};
 
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
+  if (s->op == 2) return s->nblk;
   if (!s->regular) return ceil_div(s->n, 256);
   const int64_t nbz = ceil_div(s->gd.nz - 6, zc);
   return (int64_t)(s->n_int + s->n_edge) * nbz + 2 * (int64_t)s->nbx * s->nby;
@@ -866,6 +905,14 @@ void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStre
 template <typename T>
 int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st,
                        int zc) {
+  if (s->op == 2) {
+    const bool f32 = sizeof(T) == 4;
+    hipLaunchKernelGGL(k_b_stencil<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, x, y,
+                       (const BSys*)s->bsys, (const int*)s->blk_sys,
+                       f32 ? (const T*)s->tab32 : (const T*)s->tab64, s->part,
+                       (const CgCtl*)s->bctl_apply);
+    return s->nblk;
+  }
   if (s->op == 1) {
     const bool f32 = sizeof(T) == 4;
     hipLaunchKernelGGL(k_p1_stencil<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
@@ -1057,6 +1104,105 @@ int init_common(lsamd_fd_system* s) {
   return LSAMD_OK;
 }
 
+
+// ------------------------------------------------------------- batch ----
+template <typename T>
+int launch_b_iteration(lsamd_fd_system* s, T* x, T* r, T* p, T* q, T* t, const T* minv,
+                       hipStream_t st) {
+  const bool f32 = sizeof(T) == 4;
+  const BSys* sys = s->bsys;
+  const int* bs = s->blk_sys;
+  hipLaunchKernelGGL(k_b_dir<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, (const T*)r, minv, p,
+                     sys, bs, (const CgCtl*)s->bctl);
+  RowsDev R = rows_dev(s, f32);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL(k_b_rows_t<T>, dim3((unsigned)ceil_div(s->nrows, 256)), dim3(256), 0, st,
+                       (const T*)p, R, t, s->u_rows);
+  hipLaunchKernelGGL(k_b_stencil<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, (const T*)p, q,
+                     sys, bs, f32 ? (const T*)s->tab32 : (const T*)s->tab64, s->part,
+                     (const CgCtl*)s->bctl);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL((k_rows_scatter<T, T>), dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256))),
+                       dim3(256), 0, st, R, (const T*)t, q, 1, s->ctl_apply, 0,
+                       (const double*)nullptr, 0, (const double*)nullptr, 0);
+  hipLaunchKernelGGL(k_b_alpha, dim3((unsigned)s->nsys), dim3(256), 0, st, (const double*)s->part,
+                     (const double*)s->u_rows, sys, s->bctl);
+  hipLaunchKernelGGL(k_b_upd<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, x, r, (const T*)p,
+                     (const T*)q, minv, sys, bs, s->part, (const CgCtl*)s->bctl);
+  hipLaunchKernelGGL(k_b_rho, dim3((unsigned)s->nsys), dim3(256), 0, st, (const double*)s->part,
+                     sys, s->bctl);
+  LSAMD_CHECK_LAUNCH("batch iteration");
+  return LSAMD_OK;
+}
+
+template <typename T>
+int launch_b_init(lsamd_fd_system* s, const double* b, const T* minv, T* x, T* r, double rtol,
+                  double atol, int maxiter, const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(k_b_init<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, b, minv, x, r,
+                     (const BSys*)s->bsys, (const int*)s->blk_sys, skip, s->part);
+  hipLaunchKernelGGL(k_b_init_fin, dim3((unsigned)s->nsys), dim3(256), 0, st,
+                     (const double*)s->part, (const BSys*)s->bsys, rtol, atol, maxiter, skip,
+                     s->bctl);
+  LSAMD_CHECK_LAUNCH("batch init");
+  return LSAMD_OK;
+}
+
+int b_all_done(lsamd_fd_system* s, hipStream_t st, bool* done) {
+  LSAMD_HIP(hipMemcpyAsync(s->bctl_host, s->bctl, s->nsys * sizeof(CgCtl), hipMemcpyDeviceToHost,
+                           st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  bool all = true;
+  for (int i = 0; i < s->nsys && all; ++i) all = s->bctl_host[i].done != 0;
+  *done = all;
+  return LSAMD_OK;
+}
+
+// chunks of `chunk` batched iterations (one captured graph) until every
+// system is done
+template <typename T>
+int run_b_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const T* minv,
+              hipStream_t st) {
+  const int gi = sizeof(T) == 4 ? 1 : 0;
+  if (s->bgraph[gi] && s->bgraph_chunk[gi] != chunk) {
+    hipGraphExecDestroy(s->bgraph[gi]);
+    s->bgraph[gi] = nullptr;
+  }
+  if (!s->bgraph[gi]) {
+    hipStream_t cs;
+    LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    LSAMD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = LSAMD_OK;
+    for (int i = 0; i < chunk && rc == LSAMD_OK; ++i)
+      rc = launch_b_iteration<T>(s, x, r, p, q, t, minv, cs);
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    hipStreamDestroy(cs);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "batch graph capture failed");
+    e = hipGraphInstantiate(&s->bgraph[gi], graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "batch graph instantiate failed");
+    s->bgraph_chunk[gi] = chunk;
+  }
+  for (;;) {
+    bool done = false;
+    if (int rc = b_all_done(s, st, &done)) return rc;
+    if (done) break;
+    LSAMD_HIP(hipGraphLaunch(s->bgraph[gi], st));
+  }
+  return LSAMD_OK;
+}
+
+int ensure_batch_work(lsamd_fd_system* s) {
+  if (int rc = ensure_work(s, false)) return rc;
+  if (int rc = ensure_work(s, true)) return rc;
+  if (s->nrows > 0 && !s->t64) {
+    if (int rc = dalloc(s, &s->t64, s->nrows)) return rc;
+    if (int rc = dalloc(s, &s->t32, s->nrows)) return rc;
+  }
+  return LSAMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1160,10 +1306,13 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
 
 int lsamd_fd_system_destroy(lsamd_fd_system* s) {
   if (!s) return LSAMD_OK;
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
     if (s->graph[i]) hipGraphExecDestroy(s->graph[i]);
+    if (s->bgraph[i]) hipGraphExecDestroy(s->bgraph[i]);
+  }
   for (void* p : s->owned) hipFree(p);
   if (s->ctl_host) hipHostFree(s->ctl_host);
+  if (s->bctl_host) hipHostFree(s->bctl_host);
   delete s;
   return LSAMD_OK;
 }
@@ -1180,15 +1329,21 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     return fail(LSAMD_ERR_ARG, "lsamd_fd_system_set_rows: NULL row array");
   hipStream_t st = (hipStream_t)stream;
   LSAMD_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
     if (s->graph[i]) {
       hipGraphExecDestroy(s->graph[i]);
       s->graph[i] = nullptr;
     }
+    if (s->bgraph[i]) {
+      hipGraphExecDestroy(s->bgraph[i]);
+      s->bgraph[i] = nullptr;
+    }
+  }
   void* olds[] = {s->row_ptr, s->cols,  s->coef64,   s->coef32,  s->b,     s->w,
                   s->w2_64,   s->w2_32, s->seg_ptr,  s->seg_entry, s->seg_row, s->dnode,
-                  s->t64,     s->t32,   s->part_rows};
+                  s->t64,     s->t32,   s->part_rows, s->u_rows};
   for (void* p : olds) dfree(s, p);
+  s->u_rows = nullptr;
   s->row_ptr = nullptr;
   s->cols = nullptr;
   s->coef64 = nullptr;
@@ -1221,11 +1376,27 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     if (int rc = dalloc(s, &s->t64, nrows)) return rc;
     if (int rc = dalloc(s, &s->t32, nrows)) return rc;
     if (int rc = dalloc(s, &s->part_rows, ceil_div(nrows, 256))) return rc;
+    if (int rc = dalloc(s, &s->u_rows, nrows)) return rc;
     LSAMD_HIP(hipMemcpyAsync(s->row_ptr, row_ptr, (nrows + 1) * 8, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->cols, cols, nnz * 4, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->coef64, coef, nnz * 8, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->b, b, nrows * 8, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->w, w, nrows * 8, hipMemcpyDefault, st));
+    {
+      int* bad = nullptr;
+      if (int rc = dalloc(s, &bad, 1)) return rc;
+      LSAMD_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
+      hipLaunchKernelGGL(k_check_cols, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, st,
+                         (const int32_t*)s->cols, nnz, (int64_t)0, s->n, bad);
+      int hb = 0;
+      LSAMD_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+      LSAMD_HIP(hipStreamSynchronize(st));
+      dfree(s, bad);
+      if (hb) {
+        s->nrows = s->nnz = 0;
+        return fail(LSAMD_ERR_ARG, "lsamd_fd_system_set_rows: a column id is outside the grid");
+      }
+    }
     const int64_t m = std::max(nrows, nnz);
     hipLaunchKernelGGL(k_w2, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, s->w, s->w2_64,
                        s->w2_32, s->coef64, s->coef32, nrows, nnz);
@@ -1562,6 +1733,217 @@ int lsamd_rows_adjoint(lsamd_fd_system* s, const double* c, const double* lambda
   hipLaunchKernelGGL(k_adjoint, dim3((unsigned)ceil_div(s->nrows, 256)), dim3(256), 0,
                      (hipStream_t)stream, R, s->coef64, s->w, s->b, c, lambda, g_b, g_w);
   LSAMD_CHECK_LAUNCH("k_adjoint");
+  return LSAMD_OK;
+}
+
+
+// ------------------------------------------------------------- batch ----
+int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg* reg,
+                       lsamd_fd_system** out) {
+  if (int rc = injected("batch_create")) return rc;
+  if (!out || !grids || !reg) return fail(LSAMD_ERR_ARG, "lsamd_batch_create: NULL argument");
+  *out = nullptr;
+  if (nsys < 1) return fail(LSAMD_ERR_ARG, "lsamd_batch_create: nsys < 1");
+  std::vector<BSys> hs(nsys);
+  int64_t off = 0;
+  int64_t nb = 0;
+  for (int i = 0; i < nsys; ++i) {
+    if (int rc = check_grid(&grids[i])) return rc;
+    for (int a = 0; a < 3; ++a)
+      if (grids[i].nnodes[a] < 7)
+        return fail(LSAMD_ERR_ARG, "lsamd_batch_create: every grid needs >= 7 nodes per axis");
+    const int64_t n = grids[i].nnodes[0] * grids[i].nnodes[1] * grids[i].nnodes[2];
+    hs[i].g = to_dev(grids[i]);
+    hs[i].off = off;
+    hs[i].n = n;
+    hs[i].r0 = hs[i].r1 = 0;
+    hs[i].b0 = (int)nb;
+    nb += ceil_div(n, kBNodes);
+    hs[i].b1 = (int)nb;
+    off += n;
+    if (nb > 2000000000LL || off > ((int64_t)1 << 31) - 1)
+      return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
+  }
+  lsamd_fd_system* s = new lsamd_fd_system();
+  s->grid = grids[0];
+  s->gd = to_dev(grids[0]);
+  s->reg = *reg;
+  s->n = off;
+  s->op = 2;
+  s->nsys = nsys;
+  s->bsys_h = hs;
+  s->nblk = (int)nb;
+  auto bail = [&](int rc) {
+    lsamd_fd_system_destroy(s);
+    return rc;
+  };
+  std::vector<int> bsys_tab(nb);
+  for (int i = 0; i < nsys; ++i)
+    for (int b = hs[i].b0; b < hs[i].b1; ++b) bsys_tab[b] = i;
+  if (int rc = dalloc(s, &s->bsys, nsys)) return bail(rc);
+  if (int rc = dalloc(s, &s->blk_sys, nb)) return bail(rc);
+  if (int rc = dalloc(s, &s->bctl, nsys)) return bail(rc);
+  if (int rc = dalloc(s, &s->bctl_apply, nsys)) return bail(rc);
+  if (int rc = dalloc(s, &s->bctl64, nsys)) return bail(rc);
+  if (int rc = dalloc(s, &s->odone, nsys)) return bail(rc);
+  if (int rc = dalloc(s, &s->bres, nsys)) return bail(rc);
+  LSAMD_HIP(hipMemcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
+  LSAMD_HIP(hipMemcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+  LSAMD_HIP(hipMemset(s->bctl, 0, nsys * sizeof(CgCtl)));
+  LSAMD_HIP(hipMemset(s->bctl_apply, 0, nsys * sizeof(CgCtl)));
+  LSAMD_HIP(hipMemset(s->bctl64, 0, nsys * sizeof(CgCtl)));
+  if (hipHostMalloc((void**)&s->bctl_host, nsys * sizeof(CgCtl), hipHostMallocDefault) !=
+      hipSuccess)
+    return bail(fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed"));
+  std::vector<double> tab(kClasses * kTaps);
+  build_class_table(s->reg, tab.data());
+  std::vector<float> tabf(tab.begin(), tab.end());
+  if (int rc = dalloc(s, &s->tab64, kClasses * kTaps)) return bail(rc);
+  if (int rc = dalloc(s, &s->tab32, kClasses * kTaps)) return bail(rc);
+  LSAMD_HIP(hipMemcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+  LSAMD_HIP(hipMemcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
+  if (int rc = init_common(s)) return bail(rc);
+  *out = s;
+  return LSAMD_OK;
+}
+
+int lsamd_batch_offsets(lsamd_fd_system* s, int64_t* node_off) {
+  if (!s || !node_off || s->op != 2) return fail(LSAMD_ERR_ARG, "lsamd_batch_offsets: not a batch");
+  for (int i = 0; i < s->nsys; ++i) node_off[i] = s->bsys_h[i].off;
+  node_off[s->nsys] = s->n;
+  return LSAMD_OK;
+}
+
+int lsamd_batch_set_rows(lsamd_fd_system* s, const int64_t* sys_rows, int64_t nrows, int64_t nnz,
+                         const int64_t* row_ptr, const int32_t* cols, const double* coef,
+                         const double* b, const double* w, void* stream) {
+  if (int rc = injected("batch_set_rows")) return rc;
+  if (!s || s->op != 2 || !sys_rows) return fail(LSAMD_ERR_ARG, "lsamd_batch_set_rows: not a batch");
+  if (sys_rows[0] != 0 || sys_rows[s->nsys] != nrows)
+    return fail(LSAMD_ERR_ARG, "lsamd_batch_set_rows: sys_rows must run from 0 to nrows");
+  for (int i = 0; i < s->nsys; ++i)
+    if (sys_rows[i + 1] < sys_rows[i])
+      return fail(LSAMD_ERR_ARG, "lsamd_batch_set_rows: sys_rows must be non-decreasing");
+  if (int rc = lsamd_fd_system_set_rows(s, nrows, nnz, row_ptr, cols, coef, b, w, stream))
+    return rc;
+  for (int i = 0; i < s->nsys; ++i) {
+    s->bsys_h[i].r0 = sys_rows[i];
+    s->bsys_h[i].r1 = sys_rows[i + 1];
+  }
+  LSAMD_HIP(hipMemcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys), hipMemcpyHostToDevice));
+  if (nrows > 0) {
+    hipStream_t st = (hipStream_t)stream;
+    int* bad = nullptr;
+    if (int rc = dalloc(s, &bad, 1)) return rc;
+    LSAMD_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_b_check_rows, dim3((unsigned)s->nsys), dim3(256), 0, st,
+                       rows_dev(s, false), (const BSys*)s->bsys, bad);
+    int hb = 0;
+    LSAMD_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    dfree(s, bad);
+    if (hb) {
+      lsamd_fd_system_set_rows(s, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, stream);
+      for (auto& q : s->bsys_h) q.r0 = q.r1 = 0;
+      LSAMD_HIP(hipMemcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys),
+                          hipMemcpyHostToDevice));
+      return fail(LSAMD_ERR_ARG, "lsamd_batch_set_rows: a row addresses another system's nodes");
+    }
+  }
+  return LSAMD_OK;
+}
+
+int lsamd_batch_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* rhs, double* c,
+                      lsamd_solve_info* info, void* stream) {
+  if (int rc = injected("batch_solve")) return rc;
+  if (!s || !o || !c || s->op != 2) return fail(LSAMD_ERR_ARG, "lsamd_batch_solve: bad argument");
+  if (!(o->rtol >= 0.0) || !(o->atol >= 0.0))
+    return fail(LSAMD_ERR_ARG, "rtol/atol must be non-negative");
+  if (!o->jacobi) return fail(LSAMD_ERR_ARG, "lsamd_batch_solve: only the Jacobi form is batched");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = s->n;
+  const int ns = s->nsys;
+  const double* b = rhs ? rhs : s->rhs;
+  if (int rc = ensure_batch_work(s)) return rc;
+  int64_t nmax = 0;
+  for (const BSys& q : s->bsys_h) nmax = std::max(nmax, q.n);
+  const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * nmax;
+  const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
+  const int chunk = o->check_every > 0 ? o->check_every : 64;
+  std::vector<lsamd_solve_info> inf(ns);
+  memset(inf.data(), 0, ns * sizeof(lsamd_solve_info));
+  if (o->precision == LSAMD_FP64) {
+    if (int rc = launch_b_init<double>(s, b, s->minv64, s->x64, s->r64, o->rtol, o->atol, maxiter,
+                                       nullptr, st))
+      return rc;
+    if (int rc = run_b_pcg<double>(s, chunk, s->x64, s->r64, s->p64, s->q64, s->t64, s->minv64, st))
+      return rc;
+    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->x64, c, n);
+    LSAMD_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < ns; ++i) {
+      inf[i].iterations = s->bctl_host[i].iter;
+      inf[i].converged = s->bctl_host[i].converged;
+      inf[i].residual = sqrt(s->bctl_host[i].rho);
+      inf[i].threshold = s->bctl_host[i].thr;
+    }
+  } else if (o->precision == LSAMD_FP32) {
+    const double inner = o->inner_rtol > 0 ? o->inner_rtol : 1e-4;
+    const int64_t max_outer = o->max_outer > 0 ? o->max_outer : 50;
+    hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, c, n);
+    // FP64 thresholds and initial residual r64 = rhs (c = 0)
+    if (int rc = launch_b_init<double>(s, b, s->minv64, s->x64, s->r64, o->rtol, o->atol, maxiter,
+                                       nullptr, st))
+      return rc;
+    LSAMD_HIP(hipMemcpyAsync(s->bctl64, s->bctl, ns * sizeof(CgCtl), hipMemcpyDeviceToDevice, st));
+    std::vector<int> od(ns);
+    LSAMD_HIP(hipMemcpyAsync(s->bctl_host, s->bctl, ns * sizeof(CgCtl), hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < ns; ++i) {
+      od[i] = s->bctl_host[i].converged;
+      inf[i].threshold = s->bctl_host[i].thr;
+      inf[i].residual = sqrt(s->bctl_host[i].rho);
+    }
+    LSAMD_HIP(hipMemcpyAsync(s->odone, od.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+    int64_t outer = 0;
+    bool all = true;
+    for (int i = 0; i < ns; ++i) all = all && od[i];
+    std::vector<double> res(ns);
+    while (!all && outer < max_outer) {
+      if (int rc = launch_b_init<float>(s, s->r64, s->minv32, s->x32, s->r32, inner, 0.0, maxiter,
+                                        s->odone, st))
+        return rc;
+      if (int rc = run_b_pcg<float>(s, chunk, s->x32, s->r32, s->p32, s->q32, s->t32, s->minv32,
+                                    st))
+        return rc;
+      for (int i = 0; i < ns; ++i)
+        if (!od[i]) {
+          inf[i].iterations += s->bctl_host[i].iter;
+          inf[i].outer += 1;
+        }
+      ++outer;
+      hipLaunchKernelGGL(k_b_accum, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->x32, c,
+                         n);
+      if (int rc = launch_apply<double>(s, c, s->q64, s->t64, s->ctl_apply, st)) return rc;
+      hipLaunchKernelGGL(k_b_residual, dim3((unsigned)s->nblk), dim3(256), 0, st, b, s->q64,
+                         s->minv64, s->r64, (const BSys*)s->bsys, (const int*)s->blk_sys, s->part);
+      hipLaunchKernelGGL(k_b_outer_fin, dim3((unsigned)ns), dim3(256), 0, st,
+                         (const double*)s->part, (const BSys*)s->bsys, (const CgCtl*)s->bctl64,
+                         s->bres, s->odone);
+      LSAMD_CHECK_LAUNCH("batch refinement");
+      LSAMD_HIP(hipMemcpyAsync(od.data(), s->odone, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      LSAMD_HIP(hipMemcpyAsync(res.data(), s->bres, ns * sizeof(double), hipMemcpyDeviceToHost, st));
+      LSAMD_HIP(hipStreamSynchronize(st));
+      all = true;
+      for (int i = 0; i < ns; ++i) {
+        if (inf[i].outer == outer) inf[i].residual = res[i];
+        all = all && od[i];
+      }
+    }
+    for (int i = 0; i < ns; ++i) inf[i].converged = od[i];
+  } else {
+    return fail(LSAMD_ERR_ARG, "unknown precision");
+  }
+  if (info) memcpy(info, inf.data(), ns * sizeof(lsamd_solve_info));
   return LSAMD_OK;
 }
 

@@ -366,3 +366,100 @@ def p1_evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True
         _lib.call("lsamd_p1_evaluate", C.byref(grid), _lib.ptr(c), _lib.ptr(pts), n, _lib.ptr(v),
                   _lib.ptr(g), _lib.stream())
     return v, g
+
+
+# ----------------------------------------------------------------- batch ----
+class BatchSystem:
+    """An ensemble of FD systems of different grid sizes solved together
+    (``lsamd_batch_*``): node vectors concatenated per system, one shared
+    regulariser class table, per-system PCG scalars and stopping tests."""
+
+    def __init__(self, grids, reg: _lib.FdReg):
+        self.dev = _lib.device()
+        self.grids = list(grids)
+        self.nsys = len(self.grids)
+        arr = (_lib.Grid * self.nsys)(*self.grids)
+        h = C.c_void_p()
+        _lib.call("lsamd_batch_create", self.nsys, arr, C.byref(reg), C.byref(h))
+        self._h = h
+        off = (C.c_int64 * (self.nsys + 1))()
+        _lib.call("lsamd_batch_offsets", self._h, off)
+        self.offsets = [int(v) for v in off]
+        self.n = self.offsets[-1]
+        self.rows = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.load().lsamd_fd_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rows(self, blocks):
+        """``blocks[s]``: the FdRows of system s (local node ids)."""
+        shifted, sys_rows = [], [0]
+        for s, blk in enumerate(blocks):
+            if blk is not None and blk.nrows:
+                shifted.append(FdRows(blk.row_ptr, blk.cols + self.offsets[s], blk.coef, blk.b,
+                                      blk.w, blk.norm, self.n))
+            sys_rows.append(sys_rows[-1] + (blk.nrows if blk is not None else 0))
+        rows = concat_rows(shifted, self.n)
+        self.rows = rows
+        sr = (C.c_int64 * (self.nsys + 1))(*sys_rows)
+        s = _lib.stream()
+        if rows is None:
+            _lib.call("lsamd_batch_set_rows", self._h, sr, 0, 0, None, None, None, None, None, s)
+            return
+        _lib.call("lsamd_batch_set_rows", self._h, sr, rows.nrows, rows.nnz, _lib.ptr(rows.row_ptr),
+                  _lib.ptr(rows.cols), _lib.ptr(rows.coef), _lib.ptr(rows.b), _lib.ptr(rows.w), s)
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        prec = _lib.FP32 if x.dtype == torch.float32 else _lib.FP64
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        _lib.call("lsamd_fd_system_apply", self._h, prec, _lib.ptr(x), _lib.ptr(y), _lib.stream())
+        return y
+
+    def rhs(self) -> torch.Tensor:
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_rhs", self._h, _lib.ptr(out), _lib.stream())
+        return out
+
+    def diag(self) -> torch.Tensor:
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_diag", self._h, _lib.ptr(out), _lib.stream())
+        return out
+
+    def solve(self, rtol=1e-5, atol=0.0, maxiter=None, precision="fp64", rhs=None,
+              check_every=64, inner_rtol=1e-4, max_outer=50):
+        o = _lib.SolveOpts()
+        o.rtol, o.atol = float(rtol), float(atol)
+        o.maxiter = int(maxiter) if maxiter else 0
+        o.precision = _lib.FP32 if precision == "fp32" else _lib.FP64
+        o.check_every = int(check_every)
+        o.inner_rtol = float(inner_rtol)
+        o.max_outer = int(max_outer)
+        o.jacobi = 1
+        infos = (_lib.SolveInfo * self.nsys)()
+        c = torch.empty(self.n, dtype=F64, device=self.dev)
+        if rhs is not None:
+            rhs = rhs.to(self.dev, F64).contiguous()
+        _lib.call("lsamd_batch_solve", self._h, C.byref(o), _lib.ptr(rhs), _lib.ptr(c), infos,
+                  _lib.stream())
+        return c, [{"iterations": int(i.iterations), "outer": int(i.outer),
+                    "converged": bool(i.converged), "residual": float(i.residual),
+                    "threshold": float(i.threshold)} for i in infos]
+
+    def split(self, v: torch.Tensor):
+        return [v[self.offsets[s]:self.offsets[s + 1]] for s in range(self.nsys)]
+
+    def time_stencil(self, precision="fp32", iters=20) -> float:
+        ms = C.c_double(0.0)
+        _lib.call("lsamd_fd_system_time_stencil", self._h,
+                  _lib.FP32 if precision == "fp32" else _lib.FP64, int(iters), C.byref(ms),
+                  _lib.stream())
+        return float(ms.value)
