@@ -9,12 +9,19 @@ location + row assembly + transpose/diagonal/rhs + PCG to the tolerance that
 meets rel. L2 error <= 1e-6 against a tightly converged FP64 solution.
 
 Forms timed: 'plain' = FP64 Jacobi-PCG (one system); 'mixed' = FP32 stencil
-and vectors with FP64 dots, FP64 residual recomputation and outer refinement.
-``value`` is the best form that meets the accuracy bar.
+and vectors with FP64 dots, FP64 residual recomputation and outer refinement;
+'batched' = B perturbed realisations of the same workload (value noise 0.02,
+normals tilted 5 degrees) solved together in one batched mixed-precision
+solve, each realisation checked against its own FP64 solution.  ``value`` is
+the best form that meets the accuracy bar.
 
-Prints ONE JSON line (see the round contract) with ``roofline`` for the
-dominant kernel (the regulariser stencil, HIP-event timed) and
-``cpu_baseline`` (the CPU oracle: the reference's scipy CG path, restated).
+Also reported (not ``value``): the 256-system uneven-grid ensemble
+(32^3..96^3 nodes), the P1 solve on 128^3 cells, FD and P1 evaluation
+throughput, the adjoint gradient cost, and the stencil at 512^3 (HBM-resident).
+
+Prints ONE JSON line with ``roofline`` for the dominant kernel (the regulariser
+stencil, HIP-event timed) and ``cpu_baseline`` (the CPU oracle: the
+reference's scipy CG path, restated).
 """
 from __future__ import annotations
 
@@ -28,6 +35,8 @@ import numpy as np
 
 METRIC = "FD solves/sec at 128³ (rel.err≤1e-6 vs FP64) batched vs plain; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+W = {"dxx": 0.1, "dyy": 0.1, "dzz": 0.1, "dxy": 0.1, "dxz": 0.1, "dyz": 0.1,
+     "dx": 1.0, "dy": 1.0, "dz": 1.0}
 
 
 def parse():
@@ -39,14 +48,27 @@ def parse():
     ap.add_argument("--values", type=int, default=5000)
     ap.add_argument("--normals", type=int, default=1000)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--batch", type=int, default=8, help="realisations in the batched form")
+    ap.add_argument("--ensemble", type=int, default=256, help="systems in the uneven ensemble")
+    ap.add_argument("--forms", default="plain,mixed,batched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hbm-sweep", action="store_true")
-    ap.add_argument("--forms", default="plain,mixed")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip ensemble, P1, evaluation and adjoint measurements")
     return ap.parse_args()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def fd_rows(fs, _lib, grid, vals_d, nrms_d, n):
+    blocks = []
+    for pts, kind, wcol, ws in ((nrms_d, _lib.ROWS_NORMAL, 6, 10.0 / 3),
+                                (vals_d, _lib.ROWS_VALUE, 4, 1.0)):
+        _, ins, _, _ = fs.locate(grid, pts[:, :3].contiguous(), want_w=False)
+        blocks.append(fs.build_rows(grid, kind, pts[ins].contiguous(), wcol, ws, n))
+    return fs.concat_rows(blocks, n)
 
 
 def main():
@@ -63,25 +85,19 @@ def main():
 
     dev = _lib.device()
     cells = args.cells
-    grid = _lib.make_grid((0.0, 0.0, 0.0), (1.0 / cells,) * 3, (cells + 1,) * 3)
-    n = (cells + 1) ** 3
+    nn = cells + 1
+    grid = _lib.make_grid((0.0, 0.0, 0.0), (1.0 / cells,) * 3, (nn,) * 3)
+    n = nn ** 3
     values, normals = synthetic.constraints(args.values, args.normals, 1005)
     vals_d = torch.from_numpy(values).to(dev)
     nrms_d = torch.from_numpy(normals).to(dev)
-    w = {"dxx": 0.1, "dyy": 0.1, "dzz": 0.1, "dxy": 0.1, "dxz": 0.1, "dyz": 0.1,
-         "dx": 1.0, "dy": 1.0, "dz": 1.0}
-    system = fs.FdSystem(grid, fs.reg_struct(w, (cells + 1,) * 3, 1e-8))
+    reg = fs.reg_struct(W, (nn,) * 3, 1e-8)
+    system = fs.FdSystem(grid, reg)
 
     def assemble():
-        blocks = []
-        for pts, kind, wcol, ws in ((nrms_d, _lib.ROWS_NORMAL, 6, 10.0 / 3),
-                                    (vals_d, _lib.ROWS_VALUE, 4, 1.0)):
-            _, ins, _, _ = fs.locate(grid, pts[:, :3].contiguous(), want_w=False)
-            blocks.append(fs.build_rows(grid, kind, pts[ins].contiguous(), wcol, ws, n))
-        system.set_rows(fs.concat_rows(blocks, n))
+        system.set_rows(fd_rows(fs, _lib, grid, vals_d, nrms_d, n))
 
-    log(f"workload 128^3-class: cells={cells} nodes={n} values={args.values} "
-        f"normals={args.normals}")
+    log(f"workload: {nn}^3 nodes, values={args.values} normals={args.normals}")
     assemble()
     t = time.perf_counter()
     c_ref, info_ref = system.solve(rtol=1e-13, precision="fp64", check_every=256)
@@ -91,7 +107,8 @@ def main():
     ref_norm = float(torch.linalg.vector_norm(c_ref))
 
     forms = {}
-    for form in args.forms.split(","):
+    want = args.forms.split(",")
+    for form in [f for f in want if f in ("plain", "mixed")]:
         prec = "fp64" if form == "plain" else "fp32"
         kw = dict(rtol=args.rtol, precision=prec, check_every=128)
         if prec == "fp32":
@@ -117,39 +134,36 @@ def main():
         log(f"{form}: {dt * 1e3:.1f} ms/solve, it={info['iterations']} outer={info['outer']} "
             f"relerr={err:.2e}")
 
+    if "batched" in want and args.batch > 0:
+        forms["batched"] = batched_form(args, torch, _lib, fs, synthetic, grid, reg, values,
+                                        normals, n, system)
+
     ok = {k: v for k, v in forms.items() if v["meets_1e-6"]}
     best = max(ok, key=lambda k: ok[k]["solves_per_s"]) if ok else "plain"
-    prec = "fp64" if best == "plain" else "fp32"
+    prec = "fp64" if forms[best]["dtype"] == "f64" else "fp32"
     sb = 8 if prec == "fp64" else 4
-    # roofline of the dominant kernel: regulariser stencil, HIP events on the
-    # launch stream; algorithmic bytes = n * (read x + write y)
+    # roofline of the dominant kernel: regulariser stencil of the single-system
+    # plan, HIP events on the launch stream; algorithmic bytes = n*(read x + write y)
     avg_ms = system.time_stencil(prec, iters=50)
     bytes_alg = n * 2 * sb
     achieved = bytes_alg / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_stencil<{'double' if prec == 'fp64' else 'float'}>",
+                "kernel": f"regulariser stencil ({prec}): k_st_bulk<uniform> + "
+                          f"k_st_bulk<edge> + k_st_zb launch group",
                 "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes": bytes_alg,
-                "note": "128^3 working set is Infinity-Cache (MALL) resident",
+                "note": f"{nn}^3 working set is Infinity-Cache (MALL) resident; the HBM-bound "
+                        f"figure is roofline_hbm_512",
                 "autotune": system.stencil_info(prec)}
     log(f"stencil {prec}: {avg_ms * 1e3:.2f} us, {achieved:.0f} GB/s")
-    hbm = None
-    if not args.no_hbm_sweep:
-        big = 511
-        g2 = _lib.make_grid((0.0, 0.0, 0.0), (1.0 / big,) * 3, (big + 1,) * 3)
-        s2 = fs.FdSystem(g2, fs.reg_struct(w, (big + 1,) * 3, 1e-8))
-        n2 = (big + 1) ** 3
-        res = {}
-        for p2, sb2 in (("fp32", 4), ("fp64", 8)):
-            ms2 = s2.time_stencil(p2, iters=20)
-            gbs = n2 * 2 * sb2 / (ms2 * 1e-3) / 1e9
-            res[p2] = {"avg_us": round(ms2 * 1e3, 1), "achieved_GBs": round(gbs, 1),
-                       "frac": round(gbs / HBM_PEAK_GBS, 4), "autotune": s2.stencil_info(p2)}
-            log(f"512^3 stencil {p2}: {ms2 * 1e3:.1f} us, {gbs:.0f} GB/s")
-        s2.close()
-        del s2
-        torch.cuda.empty_cache()
-        hbm = {"grid": "512^3 nodes, HBM-resident (x+y > 256 MB)", **res}
+    hbm = None if args.no_hbm_sweep else hbm_sweep(torch, _lib, fs)
+
+    extras = {}
+    if not args.no_extras:
+        extras["fd_evaluation"] = fd_evaluation(torch, _lib, fs, grid, c_ref, n)
+        extras["ensemble"] = ensemble_c4(args, torch, _lib, fs, synthetic)
+        extras["p1"] = p1_c3(torch, _lib, fs, synthetic)
+        extras["adjoint"] = adjoint_c5(torch, synthetic)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -169,17 +183,231 @@ def main():
         "vs_baseline": None,
         "dtype": forms[best]["dtype"],
         "data": "synthetic",
-        "config": {"workload": f"FD interpolation solve, {cells + 1}^3 nodes unit box, "
+        "config": {"workload": f"FD interpolation solve, {nn}^3 nodes unit box, "
                                f"{args.values} value + {args.normals} normal constraints, "
                                f"regularisation 0.1, rel.err<=1e-6 vs FP64",
                    "form": best, "rtol": args.rtol, "parallelism": "single GPU"},
         "forms": forms,
+        "batched_vs_plain": (round(forms["batched"]["solves_per_s"] /
+                                   forms["plain"]["solves_per_s"], 3)
+                             if "batched" in forms and "plain" in forms else None),
         "fp64_reference_iterations": info_ref["iterations"],
         "roofline": roofline,
         "roofline_hbm_512": hbm,
+        "extras": extras,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n, single):
+    """B perturbed realisations of the 128^3 workload in one batched solve."""
+    B = args.batch
+    dev = _lib.device()
+    reals = [synthetic.perturb(values, normals, 2000 + b) for b in range(B)]
+    vd = [(torch.from_numpy(v).to(dev), torch.from_numpy(m).to(dev)) for v, m in reals]
+    # per-realisation FP64 references (single-system solves, rtol 1e-13)
+    refs = []
+    for v, m in vd:
+        single.set_rows(fd_rows(fs, _lib, grid, v, m, n))
+        c, _ = single.solve(rtol=1e-13, precision="fp64", check_every=256)
+        refs.append(c.clone())
+    bs = fs.BatchSystem([grid] * B, reg)
+
+    def step():
+        bs.set_rows([fd_rows(fs, _lib, grid, v, m, n) for v, m in vd])
+        return bs.solve(rtol=args.rtol, precision="fp32", inner_rtol=1e-3, check_every=128)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c, infos = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    errs = [float(torch.linalg.vector_norm(p - r) / torch.linalg.vector_norm(r))
+            for p, r in zip(bs.split(c), refs)]
+    sb_ms = bs.time_stencil("fp32", iters=20)
+    res = {"ms_per_solve": dt * 1e3 / B, "solves_per_s": B / dt, "batch": B,
+           "ms_per_batch": dt * 1e3,
+           "iterations_max": max(i["iterations"] for i in infos),
+           "refinement_sweeps_max": max(i["outer"] for i in infos),
+           "rel_err_vs_fp64_max": max(errs), "meets_1e-6": max(errs) <= 1e-6,
+           "dtype": "f32+f64",
+           "stencil_fp32_us": round(sb_ms * 1e3, 1),
+           "stencil_fp32_GBs": round(B * n * 8 / (sb_ms * 1e-3) / 1e9, 1)}
+    log(f"batched x{B}: {dt * 1e3:.0f} ms/batch = {B / dt:.2f} solves/s, max relerr "
+        f"{max(errs):.2e}, batched stencil {res['stencil_fp32_GBs']} GB/s")
+    bs.close()
+    del bs
+    torch.cuda.empty_cache()
+    return res
+
+
+def hbm_sweep(torch, _lib, fs):
+    big = 511
+    g2 = _lib.make_grid((0.0, 0.0, 0.0), (1.0 / big,) * 3, (big + 1,) * 3)
+    s2 = fs.FdSystem(g2, fs.reg_struct(W, (big + 1,) * 3, 1e-8))
+    n2 = (big + 1) ** 3
+    res = {}
+    for p2, sb2 in (("fp32", 4), ("fp64", 8)):
+        ms2 = s2.time_stencil(p2, iters=20)
+        gbs = n2 * 2 * sb2 / (ms2 * 1e-3) / 1e9
+        res[p2] = {"avg_us": round(ms2 * 1e3, 1), "achieved_GBs": round(gbs, 1),
+                   "frac": round(gbs / HBM_PEAK_GBS, 4), "autotune": s2.stencil_info(p2)}
+        log(f"512^3 stencil {p2}: {ms2 * 1e3:.1f} us, {gbs:.0f} GB/s")
+    s2.close()
+    del s2
+    torch.cuda.empty_cache()
+    return {"grid": "512^3 nodes, HBM-resident (x+y > 256 MB)", **res}
+
+
+def _timed(torch, fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def fd_evaluation(torch, _lib, fs, grid, c, n):
+    npts = 10_000_000
+    g = torch.Generator(device="cuda").manual_seed(7)
+    pts = torch.rand((npts, 3), dtype=torch.float64, device="cuda", generator=g)
+    dt, _ = _timed(torch, lambda: fs.evaluate(grid, c, pts, True, True))
+    bytes_alg = npts * (24 + 8 + 24) + 8 * n
+    res = {"points": npts, "ms": round(dt * 1e3, 2), "Mpts_per_s": round(npts / dt / 1e6, 1),
+           "GBs": round(bytes_alg / dt / 1e9, 1), "what": "values + gradients"}
+    log(f"FD evaluation: {res}")
+    return res
+
+
+def ensemble_c4(args, torch, _lib, fs, synthetic):
+    """256 realisations, 32^3..96^3 nodes, FP32 + FP64 refinement."""
+    dev = _lib.device()
+    ens = synthetic.ensemble(args.ensemble, 1004)
+    grids, blocks, ns = [], [], []
+    for nnode, v, m in ens:
+        c = nnode - 1
+        g = _lib.make_grid((0, 0, 0), (1.0 / c,) * 3, (nnode,) * 3)
+        grids.append(g)
+        ns.append(nnode ** 3)
+    reg = fs.reg_struct(W, (32, 32, 32), 1e-8)
+    bs = fs.BatchSystem(grids, reg)
+    vd = [(torch.from_numpy(v).to(dev), torch.from_numpy(m).to(dev)) for _, v, m in ens]
+
+    def step():
+        bs.set_rows([fd_rows(fs, _lib, g, v, m, k) for g, (v, m), k in zip(grids, vd, ns)])
+        return bs.solve(rtol=args.rtol, precision="fp32", inner_rtol=1e-3, check_every=128)
+
+    t0 = time.perf_counter()
+    c, infos = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # accuracy on 8 realisations: FP64 single-system solves at rtol 1e-13
+    errs = []
+    parts = bs.split(c)
+    for s in np.linspace(0, len(ens) - 1, 8).astype(int):
+        single = fs.FdSystem(grids[s], fs.reg_struct(W, (ens[s][0],) * 3, 1e-8))
+        single.set_rows(fd_rows(fs, _lib, grids[s], vd[s][0], vd[s][1], ns[s]))
+        r, _ = single.solve(rtol=1e-13, precision="fp64", check_every=256)
+        errs.append(float(torch.linalg.vector_norm(parts[s] - r) / torch.linalg.vector_norm(r)))
+        single.close()
+    sb_ms = bs.time_stencil("fp32", iters=10)
+    tot = sum(ns)
+    res = {"systems": len(ens), "total_dof": tot, "seconds": round(dt, 3),
+           "solves_per_s": round(len(ens) / dt, 2),
+           "converged": sum(i["converged"] for i in infos),
+           "iterations_max": max(i["iterations"] for i in infos),
+           "rel_err_vs_fp64_max_of_8": max(errs),
+           "stencil_fp32_us": round(sb_ms * 1e3, 1),
+           "stencil_fp32_GBs": round(tot * 8 / (sb_ms * 1e-3) / 1e9, 1),
+           "stencil_frac": round(tot * 8 / (sb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"ensemble: {res}")
+    bs.close()
+    del bs
+    torch.cuda.empty_cache()
+    return res
+
+
+def p1_c3(torch, _lib, fs, synthetic):
+    """P1 on 128^3 cells: 10k values (the 2k gradient constraints are a no-op
+    in the reference), cgw 0.1, FP64 and mixed solves, 1e7-point evaluation."""
+    dev = _lib.device()
+    cells = 128
+    g = _lib.make_grid((0, 0, 0), (1.0 / cells,) * 3, (cells + 1,) * 3)
+    n = (cells + 1) ** 3
+    rng = np.random.default_rng(1003)
+    pv = rng.uniform(0, 1, (10000, 3))
+    f, _ = synthetic.field2(pv)
+    f = f + rng.normal(0, 0.01, 10000)
+    vals = torch.from_numpy(np.column_stack([pv, f])).to(dev)
+    sysm = fs.P1System(g, 0.1, 1e-8)
+
+    def assemble():
+        _, _, ins, _, _ = fs.p1_locate(g, vals[:, :3].contiguous(), False)
+        sysm.set_rows(fs.p1_build_rows(g, _lib.ROWS_VALUE, vals[ins].contiguous(), 1.0, n))
+
+    assemble()
+    t0 = time.perf_counter()
+    c_ref, iref = sysm.solve(rtol=1e-12, precision="fp64", check_every=256)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    res = {"nodes": n, "tets": 5 * cells ** 3, "fp64_ref_iterations": iref["iterations"],
+           "fp64_ref_seconds": round(t_ref, 3)}
+    for prec in ("fp64", "fp32"):
+        kw = dict(rtol=1e-10, precision=prec, check_every=128)
+        if prec == "fp32":
+            kw["inner_rtol"] = 1e-3
+        dt, (c, info) = _timed(torch, lambda: (assemble(), sysm.solve(**kw))[1], reps=2)
+        err = float(torch.linalg.vector_norm(c - c_ref) / torch.linalg.vector_norm(c_ref))
+        res[prec] = {"ms_per_solve": round(dt * 1e3, 1), "iterations": info["iterations"],
+                     "rel_err_vs_fp64": err}
+    for prec, sb in (("fp64", 8), ("fp32", 4)):
+        ms = sysm.time_stencil(prec, iters=20)
+        res[f"stencil_{prec}_GBs"] = round(n * 2 * sb / (ms * 1e-3) / 1e9, 1)
+        res[f"stencil_{prec}_us"] = round(ms * 1e3, 1)
+    npts = 10_000_000
+    gen = torch.Generator(device="cuda").manual_seed(8)
+    pts = torch.rand((npts, 3), dtype=torch.float64, device="cuda", generator=gen)
+    dt, _ = _timed(torch, lambda: fs.p1_evaluate(g, c_ref, pts, True, False))
+    res["evaluation_Mpts_per_s"] = round(npts / dt / 1e6, 1)
+    log(f"P1: {res}")
+    sysm.close()
+    return res
+
+
+def adjoint_c5(torch, synthetic):
+    """Forward solve + adjoint gradient w.r.t. 5k values, 6k weights, 3k normal
+    components at 128^3 (mixed precision), loss at 2k held-out points."""
+    from loopstructural_amd.adjoint import FDProblem
+
+    values, normals = synthetic.constraints(5000, 1000, 1005)
+    rng = np.random.default_rng(1005)
+    held = rng.uniform(0.02, 0.98, (2000, 3))
+    f_held, _ = synthetic.field1(held)
+    prob = FDProblem((0, 0, 0), (127,) * 3, (1 / 127,) * 3, values[:, :3], normals[:, :3],
+                     regularisation=0.1, precision="fp32", rtol=1e-10, inner_rtol=1e-3)
+    dev = prob.dev
+    target = torch.tensor(f_held, device=dev)
+
+    def run():
+        ps = [torch.tensor(values[:, 3], device=dev, requires_grad=True),
+              torch.ones(5000, dtype=torch.float64, device=dev, requires_grad=True),
+              torch.tensor(normals[:, 3:6], device=dev, requires_grad=True),
+              torch.ones(1000, dtype=torch.float64, device=dev, requires_grad=True)]
+        loss = torch.mean((prob.evaluate(prob.solve(*ps), held) - target) ** 2)
+        loss.backward()
+        return loss
+
+    dt, loss = _timed(torch, run, reps=2)
+    res = {"ms_forward_backward": round(dt * 1e3, 1), "loss": float(loss),
+           "parameters": 5000 + 5000 + 3000 + 1000}
+    log(f"adjoint: {res}")
+    return res
 
 
 def cpu_baseline(cells, values, normals, gpu_iterations):
