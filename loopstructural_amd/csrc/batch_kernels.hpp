@@ -21,7 +21,17 @@ struct BSys {
   int64_t off, n;  // node range in the concatenated vectors
   int64_t r0, r1;  // data rows
   int b0, b1;      // node blocks
+  int t0, t1;      // stencil tiles
 };
+
+// Stencil work item of the batched plane-streaming kernel: system, xy tile
+// and z part.  kind >= 0: bulk z-chunk `kind` of an xy tile whose class
+// coefficients vary per lane; kind >= kUniTile: same for an xy tile that is
+// interior in x and y (one coefficient set); kZbLow / kZbHigh: output planes
+// [0,3) / [nz-3,nz).
+constexpr int kUniTile = 1 << 20;
+constexpr int kZbLow = -1, kZbHigh = -2;
+constexpr int kBZc = 16;  // z-chunk of the batched bulk tiles
 
 constexpr int kBNodes = 1024;  // nodes per block (256 threads x 4)
 
@@ -32,8 +42,34 @@ __constant__ signed char kBTap[kTaps][3] = {
     {0, 0, 1},  {0, 0, -2}, {2, 0, -2}, {-2, 0, -2}, {0, 2, -2},  {0, -2, -2}, {0, 0, 2},
     {2, 0, 2},  {-2, 0, 2}, {0, 2, 2},  {0, -2, 2}};
 
-// y = R x (+ ridge) for every node of every active system; per-block
-// partials of x.y.
+// Plane-streaming stencil over every tile of every active system (one
+// launch for the whole ensemble); per-block partials of x.y.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __restrict__ y,
+                                                const BSys* __restrict__ sys,
+                                                const int4* __restrict__ tiles,
+                                                const T* __restrict__ tab, Coef25<T> ku,
+                                                double* __restrict__ part,
+                                                const CgCtl* __restrict__ ctl) {
+  __shared__ double shm[NT / 64];
+  const int4 t = tiles[blockIdx.x];
+  if (ctl[t.x].done) return;
+  const BSys S = sys[t.x];
+  double dot;
+  if (t.w >= kUniTile)
+    dot = st_bulk_body<T, kBZc, 4, true>(x + S.off, y + S.off, S.g, tab, ku,
+                                         make_int2(t.y, t.z), t.w - kUniTile);
+  else if (t.w >= 0)
+    dot = st_bulk_body<T, kBZc, 4, false>(x + S.off, y + S.off, S.g, tab, ku,
+                                          make_int2(t.y, t.z), t.w);
+  else
+    dot = st_zb_body<T>(x + S.off, y + S.off, S.g, tab, t.y, t.z, t.w == kZbLow ? 0 : 1);
+  const double bs = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// y = R x (+ ridge) for every node of every active system, one node per
+// thread (reference form of k_b_tiles); per-block partials of x.y.
 template <typename T>
 __global__ __launch_bounds__(256) void k_b_stencil(const T* __restrict__ x, T* __restrict__ y,
                                                    const BSys* __restrict__ sys,
@@ -186,7 +222,7 @@ __global__ __launch_bounds__(256) void k_b_alpha(const double* __restrict__ part
   const int s = blockIdx.x;
   if (ctl[s].done) return;
   const BSys S = sys[s];
-  const double a = reduce_partials(part + S.b0, S.b1 - S.b0, sh);
+  const double a = reduce_partials(part + S.t0, S.t1 - S.t0, sh);
   const double b = reduce_partials(u + S.r0, (int)(S.r1 - S.r0), sh);
   if (threadIdx.x == 0) {
     ctl[s].pq_st = a;

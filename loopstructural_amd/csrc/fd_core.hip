@@ -92,18 +92,14 @@ __device__ __forceinline__ T inplane(const T* k, const T* b) {
          k[10] * b[2 * PX - 2] + k[11] * b[-2 * PX + 2] + k[12] * b[-2 * PX - 2];
 }
 
+// Output planes [3 + bz*ZC, min(3 + (bz+1)*ZC, nz-3)) of the 64x8 xy tile tb;
+// returns this thread's share of x.y.
 template <typename T, int ZC, int D, bool UNI>
-__global__ __launch_bounds__(NT) void k_st_bulk(const T* __restrict__ x, T* __restrict__ y,
-                                                GridDev g, const T* __restrict__ tab,
-                                                Coef25<T> ku, const int2* __restrict__ tiles,
-                                                int ntile, double* __restrict__ part,
-                                                const CgCtl* ctl) {
-  if (ctl->done) return;
+__device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __restrict__ y,
+                                               const GridDev& g, const T* __restrict__ tab,
+                                               const Coef25<T>& ku, int2 tb, int bz) {
   constexpr int NS = ZC + 4;
   __shared__ T pl[2][PSZ];
-  __shared__ double shm[NT / 64];
-  const int txy = blockIdx.x % ntile, bz = blockIdx.x / ntile;
-  const int2 tb = tiles[txy];
   const int x0 = tb.x * TX, y0 = tb.y * TY, z0 = 3 + bz * ZC;
   const int zend = min(z0 + ZC, g.nz - 3);
   const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
@@ -192,21 +188,32 @@ __global__ __launch_bounds__(NT) void k_st_bulk(const T* __restrict__ x, T* __re
     }
     __syncthreads();
   }
+  return dot;
+}
+
+template <typename T, int ZC, int D, bool UNI>
+__global__ __launch_bounds__(NT) void k_st_bulk(const T* __restrict__ x, T* __restrict__ y,
+                                                GridDev g, const T* __restrict__ tab,
+                                                Coef25<T> ku, const int2* __restrict__ tiles,
+                                                int ntile, double* __restrict__ part,
+                                                const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double shm[NT / 64];
+  const int txy = blockIdx.x % ntile, bz = blockIdx.x / ntile;
+  const double dot = st_bulk_body<T, ZC, D, UNI>(x, y, g, tab, ku, tiles[txy], bz);
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // output planes [0,3) (bz = 0) and [nz-3, nz) (bz = 1) of every xy tile
+// Output planes [0,3) (bz = 0) or [nz-3, nz) (bz = 1) of xy tile (tx, ty);
+// coefficients re-read per plane.  Returns this thread's share of x.y.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_st_zb(const T* __restrict__ x, T* __restrict__ y,
-                                              GridDev g, const T* __restrict__ tab, int nbx,
-                                              int nby, double* __restrict__ part,
-                                              const CgCtl* ctl) {
-  if (ctl->done) return;
+__device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restrict__ y,
+                                             const GridDev& g, const T* __restrict__ tab, int tx,
+                                             int ty, int bz) {
   __shared__ T pl[2][PSZ];
-  __shared__ double shm[NT / 64];
-  const int tile = blockIdx.x % (nbx * nby), bz = blockIdx.x / (nbx * nby);
-  const int x0 = (tile % nbx) * TX, y0 = (tile / nbx) * TY, z0 = bz == 0 ? 0 : g.nz - 3;
+  const int x0 = tx * TX, y0 = ty * TY, z0 = bz == 0 ? 0 : g.nz - 3;
   const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
   const int ix = x0 + lx, iy = y0 + ly;
   const bool colok = ix < g.nx && iy < g.ny;
@@ -268,6 +275,18 @@ __global__ __launch_bounds__(NT) void k_st_zb(const T* __restrict__ x, T* __rest
     v1 = v0;
     __syncthreads();
   }
+  return dot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_st_zb(const T* __restrict__ x, T* __restrict__ y,
+                                              GridDev g, const T* __restrict__ tab, int nbx,
+                                              int nby, double* __restrict__ part,
+                                              const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double shm[NT / 64];
+  const int tile = blockIdx.x % (nbx * nby), bz = blockIdx.x / (nbx * nby);
+  const double dot = st_zb_body<T>(x, y, g, tab, tile % nbx, tile / nbx, bz);
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
@@ -757,6 +776,10 @@ struct lsamd_fd_system {
   BSys* bsys = nullptr;
   int* blk_sys = nullptr;
   int nblk = 0;
+  int4* btiles = nullptr;       // stencil tiles of all systems
+  int nbtile = 0;
+  Coef25<double> bku64;         // bulk coefficients (shared by all systems)
+  Coef25<float> bku32;
   CgCtl* bctl = nullptr;        // per-system PCG control
   CgCtl* bctl_apply = nullptr;  // all-zero (no system done): standalone apply
   CgCtl* bctl64 = nullptr;      // FP64 thresholds of the mixed-precision solve
@@ -872,7 +895,7 @@ int vec_blocks(int64_t n) {
 constexpr int kZcCand[3] = {8, 16, 32};
 
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
-  if (s->op == 2) return s->nblk;
+  if (s->op == 2) return std::max(s->nblk, s->nbtile);
   if (!s->regular) return ceil_div(s->n, 256);
   const int64_t nbz = ceil_div(s->gd.nz - 6, zc);
   return (int64_t)(s->n_int + s->n_edge) * nbz + 2 * (int64_t)s->nbx * s->nby;
@@ -901,17 +924,23 @@ void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStre
                      s->gd, tab, s->nbx, s->nby, s->part + off, ctl);
 }
 
+template <typename T>
+void launch_b_tiles(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
+  const bool f32 = sizeof(T) == 4;
+  Coef25<T> ku;
+  for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->bku32.k[q] : (T)s->bku64.k[q];
+  hipLaunchKernelGGL(k_b_tiles<T>, dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
+                     (const BSys*)s->bsys, (const int4*)s->btiles,
+                     f32 ? (const T*)s->tab32 : (const T*)s->tab64, ku, s->part, ctl);
+}
+
 // Launch the regulariser stencil; returns the number of block partials.
 template <typename T>
 int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st,
                        int zc) {
   if (s->op == 2) {
-    const bool f32 = sizeof(T) == 4;
-    hipLaunchKernelGGL(k_b_stencil<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, x, y,
-                       (const BSys*)s->bsys, (const int*)s->blk_sys,
-                       f32 ? (const T*)s->tab32 : (const T*)s->tab64, s->part,
-                       (const CgCtl*)s->bctl_apply);
-    return s->nblk;
+    launch_b_tiles<T>(s, x, y, s->bctl_apply, st);
+    return s->nbtile;
   }
   if (s->op == 1) {
     const bool f32 = sizeof(T) == 4;
@@ -1118,9 +1147,7 @@ int launch_b_iteration(lsamd_fd_system* s, T* x, T* r, T* p, T* q, T* t, const T
   if (s->nrows > 0)
     hipLaunchKernelGGL(k_b_rows_t<T>, dim3((unsigned)ceil_div(s->nrows, 256)), dim3(256), 0, st,
                        (const T*)p, R, t, s->u_rows);
-  hipLaunchKernelGGL(k_b_stencil<T>, dim3((unsigned)s->nblk), dim3(256), 0, st, (const T*)p, q,
-                     sys, bs, f32 ? (const T*)s->tab32 : (const T*)s->tab64, s->part,
-                     (const CgCtl*)s->bctl);
+  launch_b_tiles<T>(s, (const T*)p, q, s->bctl, st);
   if (s->nrows > 0)
     hipLaunchKernelGGL((k_rows_scatter<T, T>), dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256))),
                        dim3(256), 0, st, R, (const T*)t, q, 1, s->ctl_apply, 0,
@@ -1757,6 +1784,7 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
     hs[i].off = off;
     hs[i].n = n;
     hs[i].r0 = hs[i].r1 = 0;
+    hs[i].t0 = hs[i].t1 = 0;
     hs[i].b0 = (int)nb;
     nb += ceil_div(n, kBNodes);
     hs[i].b1 = (int)nb;
@@ -1771,8 +1799,8 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   s->n = off;
   s->op = 2;
   s->nsys = nsys;
-  s->bsys_h = hs;
   s->nblk = (int)nb;
+  s->nbtile = 0;
   auto bail = [&](int rc) {
     lsamd_fd_system_destroy(s);
     return rc;
@@ -1780,6 +1808,26 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   std::vector<int> bsys_tab(nb);
   for (int i = 0; i < nsys; ++i)
     for (int b = hs[i].b0; b < hs[i].b1; ++b) bsys_tab[b] = i;
+  // stencil tiles: per system, bulk z-chunks of every xy tile (uniform-
+  // coefficient tiles flagged) then the two z-boundary parts
+  std::vector<int4> tiles;
+  for (int i = 0; i < nsys; ++i) {
+    const GridDev& g = hs[i].g;
+    const int nbx = (int)ceil_div(g.nx, TX), nby = (int)ceil_div(g.ny, TY);
+    const int nbz = (int)ceil_div(g.nz - 6, kBZc);
+    hs[i].t0 = (int)tiles.size();
+    for (int ty = 0; ty < nby; ++ty)
+      for (int tx = 0; tx < nbx; ++tx) {
+        const bool inter = tx * TX >= 3 && tx * TX + TX - 1 <= g.nx - 4 && ty * TY >= 3 &&
+                           ty * TY + TY - 1 <= g.ny - 4;
+        for (int bz = 0; bz < nbz; ++bz) tiles.push_back(make_int4(i, tx, ty, inter ? kUniTile + bz : bz));
+        tiles.push_back(make_int4(i, tx, ty, kZbLow));
+        tiles.push_back(make_int4(i, tx, ty, kZbHigh));
+      }
+    hs[i].t1 = (int)tiles.size();
+    if (tiles.size() > 2000000000ULL)
+      return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
+  }
   if (int rc = dalloc(s, &s->bsys, nsys)) return bail(rc);
   if (int rc = dalloc(s, &s->blk_sys, nb)) return bail(rc);
   if (int rc = dalloc(s, &s->bctl, nsys)) return bail(rc);
@@ -1787,6 +1835,10 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   if (int rc = dalloc(s, &s->bctl64, nsys)) return bail(rc);
   if (int rc = dalloc(s, &s->odone, nsys)) return bail(rc);
   if (int rc = dalloc(s, &s->bres, nsys)) return bail(rc);
+  s->bsys_h = hs;  // with the tile ranges
+  s->nbtile = (int)tiles.size();
+  if (int rc = dalloc(s, &s->btiles, tiles.size())) return bail(rc);
+  LSAMD_HIP(hipMemcpy(s->btiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
   LSAMD_HIP(hipMemcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
   LSAMD_HIP(hipMemcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   LSAMD_HIP(hipMemset(s->bctl, 0, nsys * sizeof(CgCtl)));
@@ -1802,6 +1854,11 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   if (int rc = dalloc(s, &s->tab32, kClasses * kTaps)) return bail(rc);
   LSAMD_HIP(hipMemcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
   LSAMD_HIP(hipMemcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
+  const int bulk = ((3 * 7 + 3) * 7 + 3) * kTaps;
+  for (int q = 0; q < kTaps; ++q) {
+    s->bku64.k[q] = tab[bulk + q];
+    s->bku32.k[q] = (float)tab[bulk + q];
+  }
   if (int rc = init_common(s)) return bail(rc);
   *out = s;
   return LSAMD_OK;
