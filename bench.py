@@ -8,8 +8,10 @@ one complete solve from constraint arrays already resident in HBM: point
 location + row assembly + transpose/diagonal/rhs + PCG to the tolerance that
 meets rel. L2 error <= 1e-6 against a tightly converged FP64 solution.
 
-Forms timed: 'plain' = FP64 Jacobi-PCG (one system); 'mixed' = FP32 stencil
-and vectors with FP64 dots, FP64 residual recomputation and outer refinement;
+Forms timed: 'plain' = FP64 Jacobi-PCG (one system, the reference's column
+scaling); 'mixed' = FP32 stencil and vectors with FP64 dots, FP64 residual
+recomputation and outer refinement; 'mg' = geometric-multigrid preconditioned
+flexible CG (FP32 V-cycle, FP64 outer iteration and stopping test);
 'batched' = B perturbed realisations of the same workload (value noise 0.02,
 normals tilted 5 degrees) solved together in one batched mixed-precision
 solve, each realisation checked against its own FP64 solution.  ``value`` is
@@ -50,7 +52,7 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=8, help="realisations in the batched form")
     ap.add_argument("--ensemble", type=int, default=256, help="systems in the uneven ensemble")
-    ap.add_argument("--forms", default="plain,mixed,batched")
+    ap.add_argument("--forms", default="plain,mixed,mg,batched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hbm-sweep", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -108,11 +110,13 @@ def main():
 
     forms = {}
     want = args.forms.split(",")
-    for form in [f for f in want if f in ("plain", "mixed")]:
-        prec = "fp64" if form == "plain" else "fp32"
+    for form in [f for f in want if f in ("plain", "mixed", "mg")]:
+        prec = "fp64" if form in ("plain", "mg") else "fp32"
         kw = dict(rtol=args.rtol, precision=prec, check_every=128)
-        if prec == "fp32":
+        if form == "mixed":
             kw["inner_rtol"] = 1e-3
+        if form == "mg":
+            kw.update(multigrid=True, check_every=8)
 
         def step():
             assemble()
@@ -130,7 +134,7 @@ def main():
         forms[form] = {"ms_per_solve": dt * 1e3, "solves_per_s": 1.0 / dt,
                        "iterations": info["iterations"], "refinement_sweeps": info["outer"],
                        "rel_err_vs_fp64": err, "meets_1e-6": err <= 1e-6,
-                       "dtype": "f64" if prec == "fp64" else "f32+f64"}
+                       "dtype": "f64" if form == "plain" else "f32+f64"}
         log(f"{form}: {dt * 1e3:.1f} ms/solve, it={info['iterations']} outer={info['outer']} "
             f"relerr={err:.2e}")
 

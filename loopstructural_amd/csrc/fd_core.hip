@@ -56,7 +56,7 @@ int check_grid(const lsamd_grid* g) {
 
 // ------------------------------------------------------- CG control block --
 struct CgCtl {
-  double rho, rho_prev, pq_st, pq_rows, thr, bnorm, acc, alpha;
+  double rho, rho_prev, pq_st, pq_rows, thr, bnorm, acc, alpha, res2;
   int iter, done, maxiter, converged;
 };
 
@@ -735,6 +735,7 @@ __global__ void k_check_cols(const int32_t* __restrict__ cols, int64_t nnz, int6
 }
 
 #include "batch_kernels.hpp"
+#include "mg.hpp"
 
 // rows of system s may only address its own nodes; one workgroup per system
 namespace lsamd {
@@ -749,7 +750,13 @@ __global__ void k_b_check_rows(RowsDev R, const BSys* __restrict__ sys, int* __r
 // ================================================================ host side ==
 using namespace lsamd;
 
+namespace lsamd {
+struct Mg;
+void mg_destroy(Mg* m);
+}  // namespace lsamd
+
 struct lsamd_fd_system {
+  lsamd::Mg* mg = nullptr;  // multigrid hierarchy (built on first MG solve after set_rows)
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -1232,6 +1239,480 @@ int ensure_batch_work(lsamd_fd_system* s) {
 
 }  // namespace
 
+// ------------------------------------------------------------ multigrid ----
+namespace lsamd {
+
+struct MgLevel {
+  lsamd_fd_system* sys = nullptr;  // level 0: the system itself (not owned)
+  bool owned = false;
+  // coarse levels: data term as an assembled Gram matrix (node rows, FP32)
+  int64_t* gptr = nullptr;
+  int32_t* gcol = nullptr;
+  float* gval = nullptr;
+  GridDev g;
+  int64_t n = 0;
+  float *b = nullptr, *x = nullptr, *q = nullptr, *d = nullptr;
+  float c0 = 0.f, c1[kMgDeg] = {0.f}, c2[kMgDeg] = {0.f};
+  double lambda = 0.0;
+};
+
+struct Mg {
+  std::vector<MgLevel> lv;
+  float* cinv = nullptr;  // dense inverse of the coarsest operator
+  int nc = 0;
+  double* z64 = nullptr;
+  double* rp64 = nullptr;
+  double* scal = nullptr;  // power-iteration scalars
+  std::vector<void*> mem;
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+};
+
+void mg_destroy(Mg* m) {
+  if (!m) return;
+  if (m->graph) hipGraphExecDestroy(m->graph);
+  for (auto& L : m->lv)
+    if (L.owned && L.sys) lsamd_fd_system_destroy(L.sys);
+  for (void* p : m->mem) hipFree(p);
+  delete m;
+}
+
+namespace {
+
+template <typename P>
+int mg_alloc(Mg* m, P** p, int64_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<int64_t>(count, 1) * sizeof(P)) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid)");
+  m->mem.push_back(q);
+  *p = (P*)q;
+  return LSAMD_OK;
+}
+
+// y = N x of a level, without the alpha reduction; `ctl` only gates on done
+template <typename T>
+void mg_apply(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
+  const bool f32 = sizeof(T) == 4;
+  RowsDev R = rows_dev(s, f32);
+  T* t = f32 ? (T*)s->t32 : (T*)s->t64;
+  const int nrb = s->nrows > 0 ? (int)ceil_div(s->nrows, 256) : 0;
+  if (nrb > 0)
+    hipLaunchKernelGGL(k_rows_t<T>, dim3((unsigned)nrb), dim3(256), 0, st, x, R, t, s->part_rows,
+                       const_cast<CgCtl*>(ctl));
+  launch_stencil<T>(s, x, y, ctl, st, s->zc[f32 ? 1 : 0]);
+  if (nrb > 0)
+    hipLaunchKernelGGL((k_rows_scatter<T, T>),
+                       dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256))), dim3(256), 0,
+                       st, R, (const T*)t, y, 1, const_cast<CgCtl*>(ctl), 0,
+                       (const double*)nullptr, 0, (const double*)nullptr, 0);
+}
+
+// level apply in FP32: the fine level uses the data rows, coarse levels the
+// assembled Gram matrix of their data term (rows there are many per node)
+void mg_level_apply(const MgLevel& L, const float* x, float* y, const CgCtl* ctl,
+                    hipStream_t st) {
+  if (!L.gptr) {
+    mg_apply<float>(L.sys, x, y, ctl, st);
+    return;
+  }
+  lsamd_fd_system* s = L.sys;
+  launch_stencil<float>(s, x, y, ctl, st, s->zc[1]);
+  hipLaunchKernelGGL(k_gram_spmv, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+                     (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const float*)L.gval, x, y,
+                     L.n, ctl);
+}
+
+lsamd_grid mg_coarse_grid(const lsamd_grid& f) {
+  lsamd_grid c = f;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t cells = f.nnodes[a] - 1;
+    c.nnodes[a] = (cells + 1) / 2 + 1;
+    c.step[a] = 2.0 * f.step[a];
+  }
+  return c;
+}
+
+int mg_prepare_level(lsamd_fd_system* sys, hipStream_t st) {
+  if (int rc = ensure_work(sys, false)) return rc;
+  if (int rc = ensure_work(sys, true)) return rc;
+  if (int rc = ensure_tuned<double>(sys, sys->p64, sys->q64, st)) return rc;
+  if (int rc = ensure_tuned<float>(sys, sys->p32, sys->q32, st)) return rc;
+  return LSAMD_OK;
+}
+
+// lambda_max(D^-1 N) by power iteration (FP64), on the level's work vectors
+int mg_lambda(Mg* m, lsamd_fd_system* sys, double* out, hipStream_t st) {
+  const int64_t n = sys->n;
+  const int vb = vec_blocks(n);
+  double *v = sys->p64, *y = sys->q64, *w = sys->r64;
+  const dim3 nb((unsigned)ceil_div(n, 256));
+  hipLaunchKernelGGL(k_mg_start, nb, dim3(256), 0, st, v, n);
+  for (int it = 0; it < 20; ++it) {
+    if (int rc = launch_apply<double>(sys, v, y, sys->t64, sys->ctl_apply, st)) return rc;
+    hipLaunchKernelGGL(k_mg_power, dim3(vb), dim3(256), 0, st, (const double*)y,
+                       (const double*)sys->minv64, (const double*)v, w, n, sys->part);
+    hipLaunchKernelGGL(k_mg_norm_fin, dim3(1), dim3(256), 0, st, (const double*)sys->part, vb,
+                       m->scal);
+    hipLaunchKernelGGL(k_mg_scale, nb, dim3(256), 0, st, (const double*)w, v, n,
+                       (const double*)m->scal);
+  }
+  LSAMD_CHECK_LAUNCH("multigrid power iteration");
+  double sc[2] = {0.0, 0.0};
+  LSAMD_HIP(hipMemcpyAsync(sc, m->scal, 16, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  *out = sc[0];
+  return LSAMD_OK;
+}
+
+// dense inverse (Gauss-Jordan, partial pivoting) of the n x n matrix a
+bool mg_invert(std::vector<double>& a, int n, std::vector<double>& inv) {
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(a[(size_t)r * n + c]) > fabs(a[(size_t)piv * n + c])) piv = r;
+    if (!(fabs(a[(size_t)piv * n + c]) > 0.0)) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const double d = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  return true;
+}
+
+// G = A^T W^2 A of a coarse level's (fixed-width) data rows: pairs, stable
+// radix sort by (row, col), reduce by key in sorted order (deterministic)
+int mg_gram(Mg* m, MgLevel& L, lsamd_fd_system* cs, int wc, hipStream_t st) {
+  const int64_t nr = cs->nrows;
+  const int64_t np = nr * wc * wc;
+  if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: data term too large");
+  std::vector<void*> tmp;
+  auto talloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    tmp.push_back(*p);
+    return true;
+  };
+  auto cleanup = [&]() {
+    for (void* p : tmp) hipFree(p);
+  };
+  uint64_t *key = nullptr, *key_s = nullptr, *ukey = nullptr;
+  double *val = nullptr, *val_s = nullptr, *uval = nullptr;
+  int32_t* nrun = nullptr;
+  if (!talloc((void**)&key, np * 8) || !talloc((void**)&key_s, np * 8) ||
+      !talloc((void**)&val, np * 8) || !talloc((void**)&val_s, np * 8) ||
+      !talloc((void**)&ukey, np * 8) || !talloc((void**)&uval, np * 8) ||
+      !talloc((void**)&nrun, 8)) {
+    cleanup();
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+  }
+  hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
+                     (const int32_t*)cs->cols, (const double*)cs->coef64,
+                     (const double*)cs->w2_64, nr, wc, key, val);
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) < cs->n) ++bits;
+  size_t tb1 = 0, tb2 = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, key, key_s, val, val_s, (int)np, 0, 32 + bits,
+                                     st);
+  hipcub::DeviceReduce::ReduceByKey(nullptr, tb2, key_s, ukey, val_s, uval, nrun, hipcub::Sum(),
+                                    (int)np, st);
+  void* work = nullptr;
+  const size_t tb = std::max(tb1, tb2);
+  if (!talloc(&work, tb)) {
+    cleanup();
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+  }
+  size_t t1 = tb, t2 = tb;
+  hipcub::DeviceRadixSort::SortPairs(work, t1, key, key_s, val, val_s, (int)np, 0, 32 + bits, st);
+  hipcub::DeviceReduce::ReduceByKey(work, t2, key_s, ukey, val_s, uval, nrun, hipcub::Sum(),
+                                    (int)np, st);
+  int32_t nrun_h = 0;
+  LSAMD_HIP(hipMemcpyAsync(&nrun_h, nrun, 4, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  const int64_t m_ = nrun_h;
+  if (int rc = mg_alloc(m, &L.gptr, L.n + 1)) {
+    cleanup();
+    return rc;
+  }
+  if (int rc = mg_alloc(m, &L.gcol, m_)) {
+    cleanup();
+    return rc;
+  }
+  if (int rc = mg_alloc(m, &L.gval, m_)) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_gram_ptr, dim3((unsigned)ceil_div(L.n + 1, 256)), dim3(256), 0, st,
+                     (const uint64_t*)ukey, m_, L.n, L.gptr);
+  hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)ceil_div(std::max<int64_t>(m_, 1), 256)),
+                     dim3(256), 0, st, (const uint64_t*)ukey, (const double*)uval, m_, L.gcol,
+                     L.gval);
+  const hipError_t e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid Gram assembly failed");
+  LSAMD_CHECK_LAUNCH("multigrid Gram");
+  return LSAMD_OK;
+}
+
+int mg_level_vectors(Mg* m, MgLevel& L) {
+  if (int rc = mg_alloc(m, &L.b, L.n)) return rc;
+  if (int rc = mg_alloc(m, &L.x, L.n)) return rc;
+  if (int rc = mg_alloc(m, &L.q, L.n)) return rc;
+  if (int rc = mg_alloc(m, &L.d, L.n)) return rc;
+  for (float* v : {L.b, L.x, L.q, L.d}) LSAMD_HIP(hipMemset(v, 0, L.n * 4));
+  return LSAMD_OK;
+}
+
+int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
+  if (int rc = mg_prepare_level(s, st)) return rc;
+  if (int rc = mg_alloc(m, &m->scal, 2)) return rc;
+  MgLevel L0;
+  L0.sys = s;
+  L0.g = s->gd;
+  L0.n = s->n;
+  if (int rc = mg_level_vectors(m, L0)) return rc;
+  m->lv.push_back(L0);
+  lsamd_grid gprev = s->grid;
+  lsamd_fd_system* prev = s;
+  for (int l = 1;; ++l) {
+    if (gprev.nnodes[0] <= kMgCoarse && gprev.nnodes[1] <= kMgCoarse &&
+        gprev.nnodes[2] <= kMgCoarse)
+      break;
+    const lsamd_grid gc = mg_coarse_grid(gprev);
+    lsamd_fd_reg rc_reg = s->reg;
+    const double sc = ldexp(1.0, -l);
+    for (double* f : {&rc_reg.w2_xx, &rc_reg.w2_yy, &rc_reg.w2_zz, &rc_reg.w2_xy, &rc_reg.w2_xz,
+                      &rc_reg.w2_yz, &rc_reg.w2_bx, &rc_reg.w2_by, &rc_reg.w2_bz, &rc_reg.ridge2})
+      *f *= sc;
+    lsamd_fd_system* cs = nullptr;
+    if (int rc = lsamd_fd_system_create(&gc, &rc_reg, nullptr, &cs)) return rc;
+    MgLevel L;
+    L.sys = cs;
+    L.owned = true;
+    L.g = cs->gd;
+    L.n = cs->n;
+    m->lv.push_back(L);
+    if (int rc = mg_level_vectors(m, m->lv.back())) return rc;
+    if (prev->nrows > 0) {
+      const int64_t nr = prev->nrows;
+      int32_t* cnt = nullptr;
+      if (int rc = mg_alloc(m, &cnt, nr)) return rc;
+      const GridDev gf = prev->gd, gcd = cs->gd;
+      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
+                         (const int64_t*)prev->row_ptr, (const int32_t*)prev->cols,
+                         (const double*)prev->coef64, nr, gf, gcd, 1, 0, cnt, (int32_t*)nullptr,
+                         (double*)nullptr);
+      std::vector<int32_t> hc(nr);
+      LSAMD_HIP(hipMemcpyAsync(hc.data(), cnt, nr * 4, hipMemcpyDeviceToHost, st));
+      LSAMD_HIP(hipStreamSynchronize(st));
+      int wc = 1;
+      for (int32_t c : hc) wc = std::max(wc, (int)c);
+      if (wc > 64) return fail(LSAMD_ERR_ARG, "multigrid: a data row spans too many cells");
+      int32_t* ccols = nullptr;
+      double* ccoef = nullptr;
+      int64_t* rptr = nullptr;
+      if (int rc = mg_alloc(m, &ccols, nr * wc)) return rc;
+      if (int rc = mg_alloc(m, &ccoef, nr * wc)) return rc;
+      if (int rc = mg_alloc(m, &rptr, nr + 1)) return rc;
+      std::vector<int64_t> hr(nr + 1);
+      for (int64_t r = 0; r <= nr; ++r) hr[r] = r * wc;
+      LSAMD_HIP(hipMemcpyAsync(rptr, hr.data(), (nr + 1) * 8, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
+                         (const int64_t*)prev->row_ptr, (const int32_t*)prev->cols,
+                         (const double*)prev->coef64, nr, gf, gcd, 0, wc, (int32_t*)nullptr,
+                         ccols, ccoef);
+      LSAMD_CHECK_LAUNCH("multigrid rows");
+      if (int rc = lsamd_fd_system_set_rows(cs, nr, nr * wc, rptr, ccols, ccoef, prev->b, prev->w,
+                                            st))
+        return rc;
+      if (int rc = mg_gram(m, m->lv.back(), cs, wc, st)) return rc;
+    }
+    if (int rc = mg_prepare_level(cs, st)) return rc;
+    prev = cs;
+    gprev = gc;
+  }
+  // coarsest level: dense inverse of its FP64 operator
+  MgLevel& C = m->lv.back();
+  const int nc = (int)C.n;
+  if (nc > 4096) return fail(LSAMD_ERR_ARG, "multigrid: coarsest level too large");
+  m->nc = nc;
+  std::vector<double> a((size_t)nc * nc), inv;
+  for (int j = 0; j < nc; ++j) {
+    hipLaunchKernelGGL(k_mg_unit, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, st,
+                       C.sys->p64, (int64_t)nc, (int64_t)j);
+    if (int rc = launch_apply<double>(C.sys, C.sys->p64, C.sys->q64, C.sys->t64,
+                                      C.sys->ctl_apply, st))
+      return rc;
+    LSAMD_HIP(hipMemcpyAsync(&a[(size_t)j * nc], C.sys->q64, nc * 8, hipMemcpyDeviceToHost, st));
+  }
+  LSAMD_HIP(hipStreamSynchronize(st));
+  if (!mg_invert(a, nc, inv)) return fail(LSAMD_ERR_ARG, "multigrid: singular coarsest operator");
+  std::vector<float> invf(inv.begin(), inv.end());
+  if (int rc = mg_alloc(m, &m->cinv, (int64_t)nc * nc)) return rc;
+  LSAMD_HIP(hipMemcpy(m->cinv, invf.data(), invf.size() * 4, hipMemcpyHostToDevice));
+  // smoother coefficients of the other levels
+  for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
+    MgLevel& L = m->lv[l];
+    if (int rc = mg_lambda(m, L.sys, &L.lambda, st)) return rc;
+    const double lmax = 1.15 * L.lambda, lmin = L.lambda / 30.0;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    L.c0 = (float)(1.0 / theta);
+    for (int k = 1; k < kMgDeg; ++k) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      L.c1[k] = (float)(rn * rho);
+      L.c2[k] = (float)(2.0 * rn / delta);
+      rho = rn;
+    }
+  }
+  if (int rc = mg_alloc(m, &m->z64, s->n)) return rc;
+  if (int rc = mg_alloc(m, &m->rp64, s->n)) return rc;
+  LSAMD_HIP(hipMemset(m->z64, 0, s->n * 8));
+  LSAMD_HIP(hipMemset(m->rp64, 0, s->n * 8));
+  return LSAMD_OK;
+}
+
+void mg_vcycle(Mg* m, int l, const CgCtl* ctl, hipStream_t st) {
+  MgLevel& L = m->lv[l];
+  if (l == (int)m->lv.size() - 1) {
+    hipLaunchKernelGGL(k_mg_dense, dim3((unsigned)m->nc), dim3(64), 0, st,
+                       (const float*)m->cinv, (const float*)L.b, L.x, m->nc, ctl);
+    return;
+  }
+  lsamd_fd_system* S = L.sys;
+  const dim3 nb((unsigned)ceil_div(L.n, 256));
+  hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b, (const float*)L.q,
+                     (const float*)S->minv32, L.x, L.d, L.n, L.c0, 1, ctl);
+  for (int k = 1; k < kMgDeg; ++k) {
+    mg_level_apply(L, L.x, L.q, ctl, st);
+    hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
+                       (const float*)L.q, (const float*)S->minv32, L.x, L.d, L.n, L.c1[k],
+                       L.c2[k], ctl);
+  }
+  mg_level_apply(L, L.x, L.q, ctl, st);
+  MgLevel& C = m->lv[l + 1];
+  hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)ceil_div(C.n, 256)), dim3(256), 0, st,
+                     (const float*)L.b, (const float*)L.q, L.g, C.b, C.g, ctl);
+  mg_vcycle(m, l + 1, ctl, st);
+  hipLaunchKernelGGL(k_mg_prolong, nb, dim3(256), 0, st, (const float*)C.x, C.g, L.x, L.g, ctl);
+  mg_level_apply(L, L.x, L.q, ctl, st);
+  hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b, (const float*)L.q,
+                     (const float*)S->minv32, L.x, L.d, L.n, L.c0, 0, ctl);
+  for (int k = 1; k < kMgDeg; ++k) {
+    mg_level_apply(L, L.x, L.q, ctl, st);
+    hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
+                       (const float*)L.q, (const float*)S->minv32, L.x, L.d, L.n, L.c1[k],
+                       L.c2[k], ctl);
+  }
+}
+
+int launch_fcg_iteration(lsamd_fd_system* s, hipStream_t st) {
+  Mg* m = s->mg;
+  const int64_t n = s->n;
+  const int vb = vec_blocks(n);
+  MgLevel& L0 = m->lv[0];
+  const dim3 nb((unsigned)ceil_div(n, 256));
+  hipLaunchKernelGGL(k_mg_to_f32, nb, dim3(256), 0, st, (const double*)s->r64, L0.b, n,
+                     (const CgCtl*)s->ctl);
+  mg_vcycle(m, 0, s->ctl, st);
+  hipLaunchKernelGGL(k_mg_to_f64, nb, dim3(256), 0, st, (const float*)L0.x, m->z64, n,
+                     (const CgCtl*)s->ctl);
+  hipLaunchKernelGGL(k_fcg_dots, dim3(vb), dim3(256), 0, st, (const double*)m->z64,
+                     (const double*)s->r64, (const double*)m->rp64, n, s->part,
+                     (const CgCtl*)s->ctl);
+  hipLaunchKernelGGL(k_fcg_dots_fin, dim3(1), dim3(256), 0, st, (const double*)s->part, vb,
+                     s->ctl);
+  hipLaunchKernelGGL(k_fcg_dir, dim3(vb), dim3(256), 0, st, (const double*)m->z64, s->p64, n,
+                     (const CgCtl*)s->ctl);
+  if (int rc = launch_apply<double>(s, s->p64, s->q64, s->t64, s->ctl, st)) return rc;
+  hipLaunchKernelGGL(k_fcg_upd, dim3(vb), dim3(256), 0, st, s->x64, s->r64, m->rp64,
+                     (const double*)s->p64, (const double*)s->q64, (const double*)s->minv64, n,
+                     s->part, (const CgCtl*)s->ctl);
+  hipLaunchKernelGGL(k_fcg_upd_fin, dim3(1), dim3(256), 0, st, (const double*)s->part, vb,
+                     s->ctl);
+  LSAMD_CHECK_LAUNCH("multigrid PCG iteration");
+  return LSAMD_OK;
+}
+
+int run_fcg(lsamd_fd_system* s, int chunk, hipStream_t st) {
+  Mg* m = s->mg;
+  if (m->graph && m->graph_chunk != chunk) {
+    hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+  }
+  if (!m->graph) {
+    hipStream_t cs;
+    LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    LSAMD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = LSAMD_OK;
+    for (int i = 0; i < chunk && rc == LSAMD_OK; ++i) rc = launch_fcg_iteration(s, cs);
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    hipStreamDestroy(cs);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid graph capture failed");
+    e = hipGraphInstantiate(&m->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid graph instantiate failed");
+    m->graph_chunk = chunk;
+  }
+  for (;;) {
+    LSAMD_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    if (s->ctl_host->done) break;
+    LSAMD_HIP(hipGraphLaunch(m->graph, st));
+  }
+  return LSAMD_OK;
+}
+
+}  // namespace
+
+int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, double* c,
+             lsamd_solve_info* inf, hipStream_t st) {
+  if (s->op != 0) return fail(LSAMD_ERR_ARG, "multigrid is available for FD systems only");
+  if (!s->mg) {
+    Mg* m = new Mg();
+    if (int rc = mg_build(s, m, st)) {
+      mg_destroy(m);
+      return rc;
+    }
+    s->mg = m;
+  }
+  const int64_t n = s->n;
+  const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * n;
+  const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
+  const int chunk = std::max(1, std::min(o->check_every > 0 ? o->check_every : 8, 8));
+  if (int rc = launch_init<double>(s, b, s->minv64, s->x64, s->r64, o->rtol, o->atol, maxiter, st))
+    return rc;
+  if (int rc = run_fcg(s, chunk, st)) return rc;
+  hipLaunchKernelGGL(k_copy_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->x64, c, n);
+  LSAMD_HIP(hipStreamSynchronize(st));
+  inf->iterations = s->ctl_host->iter;
+  inf->outer = 0;
+  inf->converged = s->ctl_host->converged;
+  inf->residual = s->ctl_host->iter > 0 ? sqrt(s->ctl_host->res2) : sqrt(s->ctl_host->rho);
+  inf->threshold = s->ctl_host->thr;
+  return LSAMD_OK;
+}
+
+}  // namespace lsamd
+
 extern "C" {
 
 const char* lsamd_last_error(void) { return g_last_error.c_str(); }
@@ -1337,6 +1818,7 @@ int lsamd_fd_system_destroy(lsamd_fd_system* s) {
     if (s->graph[i]) hipGraphExecDestroy(s->graph[i]);
     if (s->bgraph[i]) hipGraphExecDestroy(s->bgraph[i]);
   }
+  if (s->mg) mg_destroy(s->mg);
   for (void* p : s->owned) hipFree(p);
   if (s->ctl_host) hipHostFree(s->ctl_host);
   if (s->bctl_host) hipHostFree(s->bctl_host);
@@ -1356,6 +1838,10 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     return fail(LSAMD_ERR_ARG, "lsamd_fd_system_set_rows: NULL row array");
   hipStream_t st = (hipStream_t)stream;
   LSAMD_HIP(hipStreamSynchronize(st));
+  if (s->mg) {
+    mg_destroy(s->mg);
+    s->mg = nullptr;
+  }
   for (int i = 0; i < 2; ++i) {
     if (s->graph[i]) {
       hipGraphExecDestroy(s->graph[i]);
@@ -1552,6 +2038,13 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
   // preconditioner: Jacobi (= the reference's column scaling) or identity
   const double* minv64 = s->minv64;
   const float* minv32 = s->minv32;
+  if (o->jacobi == 2) {
+    lsamd_solve_info inf;
+    memset(&inf, 0, sizeof(inf));
+    if (int rc = mg_solve(s, o, rhs ? rhs : s->rhs, c, &inf, (hipStream_t)stream)) return rc;
+    if (info) *info = inf;
+    return LSAMD_OK;
+  }
   if (!o->jacobi) {
     if (!s->ones64) {
       if (int rc = dalloc(s, &s->ones64, n)) return rc;

@@ -1,0 +1,389 @@
+// Geometric multigrid preconditioner and flexible PCG for one FD system.
+//
+// Hierarchy: level 0 is the system itself; level l+1 is the grid with every
+// second node of level l (cells halved, rounded up, so the coarse grid may
+// reach one fine cell beyond the fine one), step doubled.  Prolongation P is
+// trilinear ([1/2, 1, 1/2] per axis), restriction is P^T.  Coarse operators:
+// the data term is the exact Galerkin product (A_d P): each data row is
+// mapped through P and its repeated coarse columns merged; the regulariser is
+// re-discretised on the coarse grid with every squared weight scaled by
+// 2^-l (a smooth field's second-difference energy halves per coarsening).
+// The coarsest level (<= 5 nodes per axis) is solved with a dense inverse.
+// Smoother: Chebyshev polynomial of degree kMgDeg in D^-1 N on
+// [lambda_max/30, 1.1 lambda_max], lambda_max by power iteration.  The
+// V-cycle runs in FP32; the outer flexible CG (Polak-Ribiere beta) runs in
+// FP64 with the same FP64 stopping test as the Jacobi form.
+#pragma once
+
+namespace lsamd {
+
+constexpr int kMgDeg = 2;
+constexpr int kMgCoarse = 5;  // stop coarsening at <= 5 nodes per axis
+
+// --------------------------------------------------------------- kernels ----
+__global__ void k_mg_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n,
+                            const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_mg_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n,
+                            const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+// first Chebyshev step: d = c0 D^-1 (b - q) (q unused when from_zero), x (+)= d
+__global__ void k_mg_cheb_first(const float* __restrict__ b, const float* __restrict__ q,
+                                const float* __restrict__ minv, float* __restrict__ x,
+                                float* __restrict__ d, int64_t n, float c0, int from_zero,
+                                const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r = from_zero ? b[i] : b[i] - q[i];
+  const float dv = c0 * minv[i] * r;
+  d[i] = dv;
+  x[i] = from_zero ? dv : x[i] + dv;
+}
+
+// later steps: d = c1 d + c2 D^-1 (b - q), x += d
+__global__ void k_mg_cheb_step(const float* __restrict__ b, const float* __restrict__ q,
+                               const float* __restrict__ minv, float* __restrict__ x,
+                               float* __restrict__ d, int64_t n, float c1, float c2,
+                               const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float dv = c1 * d[i] + c2 * minv[i] * (b[i] - q[i]);
+  d[i] = dv;
+  x[i] += dv;
+}
+
+__device__ inline void mg_ijk(int64_t id, int nx, int ny, int& i, int& j, int& k) {
+  const int64_t nxy = (int64_t)nx * ny;
+  k = (int)(id / nxy);
+  const int64_t r = id - (int64_t)k * nxy;
+  j = (int)(r / nx);
+  i = (int)(r - (int64_t)j * nx);
+}
+
+// bc = P^T (bf - qf): per coarse node, the fine nodes 2J-1, 2J, 2J+1 per axis
+// with weights 1/2, 1, 1/2 (those inside the fine grid)
+__global__ void k_mg_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
+                              GridDev gf, float* __restrict__ bc, GridDev gc, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  int I, J, K;
+  mg_ijk(id, gc.nx, gc.ny, I, J, K);
+  const int64_t fnxy = (int64_t)gf.nx * gf.ny;
+  float acc = 0.f;
+  for (int dk = -1; dk <= 1; ++dk) {
+    const int k = 2 * K + dk;
+    if (k < 0 || k >= gf.nz) continue;
+    const float wk = dk == 0 ? 1.f : 0.5f;
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j >= gf.ny) continue;
+      const float wj = wk * (dj == 0 ? 1.f : 0.5f);
+      for (int di = -1; di <= 1; ++di) {
+        const int i = 2 * I + di;
+        if (i < 0 || i >= gf.nx) continue;
+        const float w = wj * (di == 0 ? 1.f : 0.5f);
+        const int64_t f = (int64_t)i + (int64_t)j * gf.nx + (int64_t)k * fnxy;
+        acc += w * (bf[f] - qf[f]);
+      }
+    }
+  }
+  bc[id] = acc;
+}
+
+// xf += P xc
+__global__ void k_mg_prolong(const float* __restrict__ xc, GridDev gc, float* __restrict__ xf,
+                             GridDev gf, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nf) return;
+  int i, j, k;
+  mg_ijk(id, gf.nx, gf.ny, i, j, k);
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+  const float wi = ni == 2 ? 0.5f : 1.f, wj = nj == 2 ? 0.5f : 1.f, wk = nk == 2 ? 0.5f : 1.f;
+  const int64_t cnxy = (int64_t)gc.nx * gc.ny;
+  float acc = 0.f;
+  for (int c = 0; c < nk; ++c)
+    for (int b = 0; b < nj; ++b)
+      for (int a = 0; a < ni; ++a)
+        acc += xc[(int64_t)(i0 + a) + (int64_t)(j0 + b) * gc.nx + (int64_t)(k0 + c) * cnxy];
+  xf[id] += wi * wj * wk * acc;
+}
+
+// x = Cinv b, dense (coarsest level): one wave per row
+__global__ __launch_bounds__(64) void k_mg_dense(const float* __restrict__ cinv,
+                                                 const float* __restrict__ b,
+                                                 float* __restrict__ x, int n, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int r = blockIdx.x;
+  float acc = 0.f;
+  for (int c = threadIdx.x; c < n; c += 64) acc += cinv[(int64_t)r * n + c] * b[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (threadIdx.x == 0) x[r] = acc;
+}
+
+// Gram matrix of the data rows of a coarse level, G = A^T W^2 A, as sorted
+// (row << 32 | col, value) pairs: every row of width wc emits wc^2 pairs.
+__global__ void k_gram_pairs(const int32_t* __restrict__ cols, const double* __restrict__ coef,
+                             const double* __restrict__ w2, int64_t nrows, int wc,
+                             uint64_t* __restrict__ key, double* __restrict__ val) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int64_t e0 = r * wc;
+  const double w = w2[r];
+  for (int a = 0; a < wc; ++a)
+    for (int b = 0; b < wc; ++b) {
+      const int64_t o = e0 * wc + (int64_t)a * wc + b;
+      key[o] = ((uint64_t)(uint32_t)cols[e0 + a] << 32) | (uint32_t)cols[e0 + b];
+      val[o] = w * coef[e0 + a] * coef[e0 + b];
+    }
+}
+
+// CSR row pointers of the reduced Gram entries (keys sorted): first entry of
+// each node row by binary search
+__global__ void k_gram_ptr(const uint64_t* __restrict__ ukey, int64_t m, int64_t n,
+                           int64_t* __restrict__ ptr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)(ukey[mid] >> 32) < i)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  ptr[i] = lo;
+}
+
+__global__ void k_gram_cols(const uint64_t* __restrict__ ukey, const double* __restrict__ v,
+                            int64_t m, int32_t* __restrict__ col, float* __restrict__ v32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  col[e] = (int32_t)(ukey[e] & 0xffffffffu);
+  v32[e] = (float)v[e];
+}
+
+// y += G x (FP32), one node per thread
+__global__ void k_gram_spmv(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const float* __restrict__ val, const float* __restrict__ x,
+                            float* __restrict__ y, int64_t n, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) acc += val[e] * x[col[e]];
+  y[i] += acc;
+}
+
+// Coarse data rows A P: row r of width wf (columns on grid gf) mapped through
+// P onto grid gc, repeated coarse columns merged.  count_only: write the
+// number of distinct coarse columns; else write them (padded to wc with zero
+// coefficients on the first column).
+__global__ void k_mg_rows(const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
+                          const double* __restrict__ coef, int64_t nrows, GridDev gf, GridDev gc,
+                          int count_only, int wc, int32_t* __restrict__ ccount,
+                          int32_t* __restrict__ ccols, double* __restrict__ ccoef) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  constexpr int kMax = 64;
+  int32_t uc[kMax];
+  double ua[kMax];
+  int nu = 0;
+  bool overflow = false;
+  const int64_t cnxy = (int64_t)gc.nx * gc.ny;
+  for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    int i, j, k;
+    mg_ijk(cols[e], gf.nx, gf.ny, i, j, k);
+    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const double w = coef[e] * (ni == 2 ? 0.5 : 1.0) * (nj == 2 ? 0.5 : 1.0) * (nk == 2 ? 0.5 : 1.0);
+    for (int c = 0; c < nk; ++c)
+      for (int b = 0; b < nj; ++b)
+        for (int a = 0; a < ni; ++a) {
+          const int32_t col = (int32_t)((i >> 1) + a + (int64_t)((j >> 1) + b) * gc.nx +
+                                        (int64_t)((k >> 1) + c) * cnxy);
+          int u = 0;
+          while (u < nu && uc[u] != col) ++u;
+          if (u == nu) {
+            if (nu == kMax) {
+              overflow = true;
+              continue;
+            }
+            uc[nu] = col;
+            ua[nu++] = 0.0;
+          }
+          ua[u] += w;
+        }
+  }
+  if (count_only) {
+    ccount[r] = overflow ? kMax + 1 : nu;
+    return;
+  }
+  for (int u = 0; u < wc; ++u) {
+    ccols[r * wc + u] = u < nu ? uc[u] : uc[0];
+    ccoef[r * wc + u] = u < nu ? ua[u] : 0.0;
+  }
+}
+
+// power iteration step: w = D^-1 y (y = N v), partial |w|^2 and v.w
+__global__ __launch_bounds__(256) void k_mg_power(const double* __restrict__ y,
+                                                  const double* __restrict__ minv,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ w, int64_t n,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double wv = minv[i] * y[i];
+    w[i] = wv;
+    a += wv * wv;
+    b += v[i] * wv;
+  }
+  const double sa = block_sum(a, sh);
+  const double sb = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sa;
+    part[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+// one block: scal[0] = |w| (the eigenvalue estimate), scal[1] = 1/|w|
+__global__ __launch_bounds__(256) void k_mg_norm_fin(const double* __restrict__ part, int np,
+                                                     double* __restrict__ scal) {
+  __shared__ double sh[4];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) a += part[2 * i];
+  const double sa = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    const double wn = sqrt(sa);
+    scal[0] = wn;
+    scal[1] = wn > 0.0 ? 1.0 / wn : 0.0;
+  }
+}
+
+__global__ void k_mg_scale(const double* __restrict__ w, double* __restrict__ v, int64_t n,
+                           const double* __restrict__ scal) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = w[i] * scal[1];
+}
+
+// deterministic positive start vector of the power iteration
+__global__ void k_mg_start(double* __restrict__ v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+  z ^= z >> 31;
+  z *= 0xBF58476D1CE4E5B9ULL;
+  z ^= z >> 29;
+  v[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_mg_unit(double* v, int64_t n, int64_t j) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i == j ? 1.0 : 0.0;
+}
+
+// ------------------------------------------------------ flexible PCG ----
+// partials of z.r, z.r_prev, r.(minv r)
+__global__ __launch_bounds__(256) void k_fcg_dots(const double* __restrict__ z,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ rp, int64_t n,
+                                                  double* __restrict__ part, const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    a += z[i] * r[i];
+    b += z[i] * rp[i];
+  }
+  const double sa = block_sum(a, sh);
+  const double sb = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sa;
+    part[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+// one block: rho = z.r; beta = (z.r - z.r_prev) / rho_prev (0 on the first step)
+__global__ __launch_bounds__(256) void k_fcg_dots_fin(const double* __restrict__ part, int np,
+                                                      CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  const double sa = block_sum(a, sh);
+  const double sb = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    const double beta = ctl->iter == 0 ? 0.0 : (sa - sb) / ctl->rho_prev;
+    ctl->rho = sa;
+    ctl->acc = beta;
+  }
+}
+
+// p = z + beta p
+__global__ void k_fcg_dir(const double* __restrict__ z, double* __restrict__ p, int64_t n,
+                          const CgCtl* ctl) {
+  if (ctl->done) return;
+  const double beta = ctl->acc;
+  const int first = ctl->iter == 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// x += alpha p ; r_prev = r ; r -= alpha q ; partials of r.(minv r)
+__global__ __launch_bounds__(256) void k_fcg_upd(double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ rp,
+                                                 const double* __restrict__ p,
+                                                 const double* __restrict__ q,
+                                                 const double* __restrict__ minv, int64_t n,
+                                                 double* __restrict__ part, const CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double sh[4];
+  const double alpha = ctl->alpha;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    x[i] += alpha * p[i];
+    const double rv = r[i];
+    rp[i] = rv;
+    const double rn = rv - alpha * q[i];
+    r[i] = rn;
+    acc += rn * (minv[i] * rn);
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// one block: rho_prev = rho ; iter++ ; stopping test on |S r|
+__global__ __launch_bounds__(256) void k_fcg_upd_fin(const double* __restrict__ part, int np,
+                                                     CgCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double sh[4];
+  const double tot = reduce_partials(part, np, sh);
+  if (threadIdx.x == 0) {
+    ctl->rho_prev = ctl->rho;
+    const int it = ctl->iter + 1;
+    ctl->iter = it;
+    ctl->res2 = tot;  // |S r|^2 of the current iterate
+    const int conv = sqrt(tot) < ctl->thr;
+    ctl->converged = conv;
+    ctl->done = conv || it >= ctl->maxiter;
+  }
+}
+
+}  // namespace lsamd

@@ -528,7 +528,9 @@ class DiscreteInterpolator(GeologicalInterpolator):
         ``solver_kwargs`` follow scipy's cg: rtol (default 1e-5), atol
         (default 0; ``tol`` sets atol unless both rtol and atol are given),
         maxiter (default 10*dof).  Extra keys: ``precision`` ('fp64' |
-        'fp32': FP32 operator + FP64 refinement), ``inner_rtol``.
+        'fp32': FP32 operator + FP64 refinement), ``inner_rtol``,
+        ``preconditioner`` ('jacobi', the reference's column scaling, or
+        'mg': geometric multigrid, FD only; same stopping test and minimiser).
         """
         if not self._pre_solve():
             raise ValueError("Pre solve failed")
@@ -569,11 +571,16 @@ class DiscreteInterpolator(GeologicalInterpolator):
             if kw.pop(k, None) is not None:
                 logger.warning("solver_kwargs[%r] is not supported by the GPU solver; ignored", k)
         t1 = perf_counter()
+        pre = str(kw.pop("preconditioner", "jacobi")).lower()
+        if pre not in ("jacobi", "mg", "multigrid"):
+            raise ValueError(f"unknown preconditioner {pre!r}: use 'jacobi' or 'mg'")
+        mg = pre in ("mg", "multigrid") and isinstance(system, fs.FdSystem) and \
+            not isinstance(system, fs.P1System)
         c, info = system.solve(rtol=kw.pop("rtol", 1e-5), atol=kw.pop("atol", 0.0),
                                maxiter=kw.pop("maxiter", None),
                                precision=kw.pop("precision", "fp64"),
                                inner_rtol=kw.pop("inner_rtol", 1e-4),
-                               jacobi=self.apply_scaling_matrix)
+                               jacobi=self.apply_scaling_matrix, multigrid=mg)
         torch.cuda.synchronize()
         timing["solve_seconds"] = perf_counter() - t1
         timing.update({"iterations": info["iterations"], "refinement_sweeps": info["outer"],
