@@ -33,6 +33,25 @@ constexpr int kUniTile = 1 << 20;
 constexpr int kZbLow = -1, kZbHigh = -2;
 constexpr int kBZc = 16;  // z-chunk of the batched bulk tiles
 
+// Work items of one system (index sys) for z-chunk zc: bulk chunks of every
+// 64x8 xy tile (interior tiles flagged kUniTile), then both z-boundary parts.
+inline void stencil_tiles(int sys, const GridDev& g, int zc, std::vector<int4>& out) {
+  const int nbx = (int)ceil_div(g.nx, TX), nby = (int)ceil_div(g.ny, TY);
+  const int nbz = (int)ceil_div(g.nz - 6, zc);
+  for (int ty = 0; ty < nby; ++ty)
+    for (int tx = 0; tx < nbx; ++tx) {
+      const bool inter = tx * TX >= 3 && tx * TX + TX - 1 <= g.nx - 4 && ty * TY >= 3 &&
+                         ty * TY + TY - 1 <= g.ny - 4;
+      for (int bz = 0; bz < nbz; ++bz)
+        out.push_back(make_int4(sys, tx, ty, inter ? kUniTile + bz : bz));
+    }
+  for (int ty = 0; ty < nby; ++ty)
+    for (int tx = 0; tx < nbx; ++tx) {
+      out.push_back(make_int4(sys, tx, ty, kZbLow));
+      out.push_back(make_int4(sys, tx, ty, kZbHigh));
+    }
+}
+
 constexpr int kBNodes = 1024;  // nodes per block (256 threads x 4)
 
 // tap offsets in the order of build_class_table
@@ -42,9 +61,10 @@ __constant__ signed char kBTap[kTaps][3] = {
     {0, 0, 1},  {0, 0, -2}, {2, 0, -2}, {-2, 0, -2}, {0, 2, -2},  {0, -2, -2}, {0, 0, 2},
     {2, 0, 2},  {-2, 0, 2}, {0, 2, 2},  {0, -2, 2}};
 
-// Plane-streaming stencil over every tile of every active system (one
-// launch for the whole ensemble); per-block partials of x.y.
-template <typename T>
+// Plane-streaming stencil over a table of work items (one launch for a
+// whole system or a whole ensemble); per-block partials of x.y.  The
+// single-system plan is the same table with one system.
+template <typename T, int ZC>
 __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __restrict__ y,
                                                 const BSys* __restrict__ sys,
                                                 const int4* __restrict__ tiles,
@@ -57,11 +77,11 @@ __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __re
   const BSys S = sys[t.x];
   double dot;
   if (t.w >= kUniTile)
-    dot = st_bulk_body<T, kBZc, 4, true>(x + S.off, y + S.off, S.g, tab, ku,
-                                         make_int2(t.y, t.z), t.w - kUniTile);
+    dot = st_bulk_body<T, ZC, 4, true>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
+                                       t.w - kUniTile);
   else if (t.w >= 0)
-    dot = st_bulk_body<T, kBZc, 4, false>(x + S.off, y + S.off, S.g, tab, ku,
-                                          make_int2(t.y, t.z), t.w);
+    dot = st_bulk_body<T, ZC, 4, false>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
+                                        t.w);
   else
     dot = st_zb_body<T>(x + S.off, y + S.off, S.g, tab, t.y, t.z, t.w == kZbLow ? 0 : 1);
   const double bs = block_sum(dot, shm);

@@ -63,16 +63,16 @@ struct CgCtl {
 // ------------------------------------------------------------- stencil ----
 // y = (R^T W R + ridge2 I) x with per-block partials of x.y.
 //
-// Regular grids (>= 7 nodes per axis, uniform row weights) use three
-// launches over disjoint node sets, all plane-streaming 64x8 xy tiles
-// through a double-buffered LDS plane with D planes of global loads kept in
-// flight in registers:
-//   * k_st_bulk<UNI=true>  tiles interior in x and y, output planes
-//     [3, nz-4]: one coefficient set, passed as kernel arguments;
-//   * k_st_bulk<UNI=false> edge tiles, same planes: per-lane coefficients
-//     of the node's (x, y) class, loaded once before the first prefetch;
-//   * k_st_zb              output planes [0,3) and [nz-3,nz): coefficients
-//     re-read per plane.
+// Regular grids (>= 7 nodes per axis, uniform row weights) use one launch
+// (k_b_tiles, in batch_kernels.hpp) over a table of work items, each
+// plane-streaming a 64x8 xy tile through a double-buffered LDS plane with D
+// planes of global loads kept in flight in registers:
+//   * bulk z-chunks [3 + ZC*b, ...) of tiles interior in x and y: one
+//     coefficient set, passed as kernel arguments (st_bulk_body<UNI=true>);
+//   * bulk z-chunks of edge tiles: per-lane coefficients of the node's (x, y)
+//     class, loaded once before the first prefetch (st_bulk_body<UNI=false>);
+//   * output planes [0,3) and [nz-3,nz): coefficients re-read per plane
+//     (st_zb_body).
 // Other grids (an axis with < 7 nodes, or a per-node weight scale) use
 // k_st_general: one node per thread, exact two-pass formula.
 constexpr int TX = 64, TY = 8, NT = TX * TY;
@@ -191,21 +191,6 @@ __device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __res
   return dot;
 }
 
-template <typename T, int ZC, int D, bool UNI>
-__global__ __launch_bounds__(NT) void k_st_bulk(const T* __restrict__ x, T* __restrict__ y,
-                                                GridDev g, const T* __restrict__ tab,
-                                                Coef25<T> ku, const int2* __restrict__ tiles,
-                                                int ntile, double* __restrict__ part,
-                                                const CgCtl* ctl) {
-  if (ctl->done) return;
-  __shared__ double shm[NT / 64];
-  const int txy = blockIdx.x % ntile, bz = blockIdx.x / ntile;
-  const double dot = st_bulk_body<T, ZC, D, UNI>(x, y, g, tab, ku, tiles[txy], bz);
-  const double bs = block_sum(dot, shm);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
-}
-
-// output planes [0,3) (bz = 0) and [nz-3, nz) (bz = 1) of every xy tile
 // Output planes [0,3) (bz = 0) or [nz-3, nz) (bz = 1) of xy tile (tx, ty);
 // coefficients re-read per plane.  Returns this thread's share of x.y.
 template <typename T>
@@ -276,19 +261,6 @@ __device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restr
     __syncthreads();
   }
   return dot;
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT) void k_st_zb(const T* __restrict__ x, T* __restrict__ y,
-                                              GridDev g, const T* __restrict__ tab, int nbx,
-                                              int nby, double* __restrict__ part,
-                                              const CgCtl* ctl) {
-  if (ctl->done) return;
-  __shared__ double shm[NT / 64];
-  const int tile = blockIdx.x % (nbx * nby), bz = blockIdx.x / (nbx * nby);
-  const double dot = st_zb_body<T>(x, y, g, tab, tile % nbx, tile / nbx, bz);
-  const double bs = block_sum(dot, shm);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // any grid: one node per thread, exact two-pass formula
@@ -765,9 +737,9 @@ struct lsamd_fd_system {
   // stencil plan
   bool regular = false;   // class-table kernels usable
   int nbx = 0, nby = 0;   // 64x8 xy tiles
-  int2* tiles_int = nullptr;
-  int2* tiles_edge = nullptr;
-  int n_int = 0, n_edge = 0;
+  lsamd::BSys* sys1 = nullptr;           // this system as a one-entry table
+  int4* stiles[3] = {nullptr, nullptr, nullptr};  // work items per z-chunk 8/16/32
+  int nstile[3] = {0, 0, 0};
   double* tab64 = nullptr;
   float* tab32 = nullptr;
   Coef25<double> ku64;
@@ -901,34 +873,24 @@ int vec_blocks(int64_t n) {
 
 constexpr int kZcCand[3] = {8, 16, 32};
 
+int zc_index(int zc) { return zc == 8 ? 0 : (zc == 16 ? 1 : 2); }
+
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
   if (s->op == 2) return std::max(s->nblk, s->nbtile);
   if (!s->regular) return ceil_div(s->n, 256);
-  const int64_t nbz = ceil_div(s->gd.nz - 6, zc);
-  return (int64_t)(s->n_int + s->n_edge) * nbz + 2 * (int64_t)s->nbx * s->nby;
+  return s->nstile[zc_index(zc)];
 }
 
+// the regular single-system stencil: one launch over the system's work items
 template <typename T, int ZC>
 void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
   const bool f32 = sizeof(T) == 4;
   const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
-  const int64_t nbz = ceil_div(s->gd.nz - 6, ZC);
-  int64_t off = 0;
-  if (s->n_int > 0) {
-    hipLaunchKernelGGL((k_st_bulk<T, ZC, 4, true>), dim3((unsigned)(s->n_int * nbz)), dim3(NT), 0,
-                       st, x, y, s->gd, tab, ku, (const int2*)s->tiles_int, s->n_int, s->part, ctl);
-    off += s->n_int * nbz;
-  }
-  if (s->n_edge > 0) {
-    hipLaunchKernelGGL((k_st_bulk<T, ZC, 4, false>), dim3((unsigned)(s->n_edge * nbz)), dim3(NT),
-                       0, st, x, y, s->gd, tab, ku, (const int2*)s->tiles_edge, s->n_edge,
-                       s->part + off, ctl);
-    off += s->n_edge * nbz;
-  }
-  hipLaunchKernelGGL(k_st_zb<T>, dim3((unsigned)(2 * s->nbx * s->nby)), dim3(NT), 0, st, x, y,
-                     s->gd, tab, s->nbx, s->nby, s->part + off, ctl);
+  const int zi = zc_index(ZC);
+  hipLaunchKernelGGL((k_b_tiles<T, ZC>), dim3((unsigned)s->nstile[zi]), dim3(NT), 0, st, x, y,
+                     (const BSys*)s->sys1, (const int4*)s->stiles[zi], tab, ku, s->part, ctl);
 }
 
 template <typename T>
@@ -936,7 +898,7 @@ void launch_b_tiles(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipS
   const bool f32 = sizeof(T) == 4;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->bku32.k[q] : (T)s->bku64.k[q];
-  hipLaunchKernelGGL(k_b_tiles<T>, dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
+  hipLaunchKernelGGL((k_b_tiles<T, kBZc>), dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
                      (const BSys*)s->bsys, (const int4*)s->btiles,
                      f32 ? (const T*)s->tab32 : (const T*)s->tab64, ku, s->part, ctl);
 }
@@ -1746,22 +1708,19 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
   if (s->regular) {
     s->nbx = (int)ceil_div(s->gd.nx, TX);
     s->nby = (int)ceil_div(s->gd.ny, TY);
-    std::vector<int2> ti, te;
-    for (int j = 0; j < s->nby; ++j)
-      for (int i = 0; i < s->nbx; ++i) {
-        const bool inter = i * TX >= 3 && i * TX + TX - 1 <= s->gd.nx - 4 && j * TY >= 3 &&
-                           j * TY + TY - 1 <= s->gd.ny - 4;
-        (inter ? ti : te).push_back(make_int2(i, j));
-      }
-    s->n_int = (int)ti.size();
-    s->n_edge = (int)te.size();
-    if (int rc = dalloc(s, &s->tiles_int, std::max<int64_t>(1, s->n_int))) return bail(rc);
-    if (int rc = dalloc(s, &s->tiles_edge, std::max<int64_t>(1, s->n_edge))) return bail(rc);
-    if (s->n_int)
-      LSAMD_HIP(hipMemcpy(s->tiles_int, ti.data(), ti.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (s->n_edge)
-      LSAMD_HIP(hipMemcpy(s->tiles_edge, te.data(), te.size() * sizeof(int2),
-                          hipMemcpyHostToDevice));
+    BSys one;
+    memset(&one, 0, sizeof(one));
+    one.g = s->gd;
+    one.n = s->n;
+    if (int rc = dalloc(s, &s->sys1, 1)) return bail(rc);
+    LSAMD_HIP(hipMemcpy(s->sys1, &one, sizeof(BSys), hipMemcpyHostToDevice));
+    for (int c = 0; c < 3; ++c) {
+      std::vector<int4> t;
+      stencil_tiles(0, s->gd, kZcCand[c], t);
+      s->nstile[c] = (int)t.size();
+      if (int rc = dalloc(s, &s->stiles[c], t.size())) return bail(rc);
+      LSAMD_HIP(hipMemcpy(s->stiles[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     std::vector<double> tab(kClasses * kTaps);
     build_class_table(s->reg, tab.data());
     std::vector<float> tabf(tab.begin(), tab.end());
@@ -2305,18 +2264,8 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   // coefficient tiles flagged) then the two z-boundary parts
   std::vector<int4> tiles;
   for (int i = 0; i < nsys; ++i) {
-    const GridDev& g = hs[i].g;
-    const int nbx = (int)ceil_div(g.nx, TX), nby = (int)ceil_div(g.ny, TY);
-    const int nbz = (int)ceil_div(g.nz - 6, kBZc);
     hs[i].t0 = (int)tiles.size();
-    for (int ty = 0; ty < nby; ++ty)
-      for (int tx = 0; tx < nbx; ++tx) {
-        const bool inter = tx * TX >= 3 && tx * TX + TX - 1 <= g.nx - 4 && ty * TY >= 3 &&
-                           ty * TY + TY - 1 <= g.ny - 4;
-        for (int bz = 0; bz < nbz; ++bz) tiles.push_back(make_int4(i, tx, ty, inter ? kUniTile + bz : bz));
-        tiles.push_back(make_int4(i, tx, ty, kZbLow));
-        tiles.push_back(make_int4(i, tx, ty, kZbHigh));
-      }
+    stencil_tiles(i, hs[i].g, kBZc, tiles);
     hs[i].t1 = (int)tiles.size();
     if (tiles.size() > 2000000000ULL)
       return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
