@@ -57,7 +57,46 @@ def parse():
     ap.add_argument("--no-hbm-sweep", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip ensemble, P1, evaluation and adjoint measurements")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a 512 MiB copy and the stencil at 128^3 and 512^3 (the "
+                         "workload of the FETCH_SIZE / WRITE_SIZE counter runs)")
     return ap.parse_args()
+
+
+def traffic_probe():
+    """Copy kernel of known size (counter calibration), then the regulariser
+    stencil at 128^3 and 512^3 in FP32 and FP64 (autotuned z-chunk)."""
+    import torch
+
+    from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
+
+    x = torch.rand(512 ** 3, device=_lib.device(), dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)  # 512 MiB read + 512 MiB written
+    torch.cuda.synchronize()
+    out = {}
+    for n1 in (128, 512):
+        g = _lib.make_grid((0, 0, 0), (1 / (n1 - 1),) * 3, (n1,) * 3)
+        s = fs.FdSystem(g, fs.reg_struct(W, (n1,) * 3, 1e-8))
+        for prec in ("fp32", "fp64"):
+            ms = s.time_stencil(prec, iters=3)
+            out[f"{n1}:{prec}"] = {"avg_us": round(ms * 1e3, 2), **s.stencil_info(prec)}
+        s.close()
+    print(json.dumps({"traffic_probe": out}), flush=True)
+
+
+def stencil_traffic(nn, prec, zc):
+    """HBM bytes per dispatch of the stencil from the committed counter
+    summary (profiles/stencil_traffic.json), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "stencil_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic"].get(f"{nn}:{prec}:{zc}")
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(msg):
@@ -75,6 +114,9 @@ def fd_rows(fs, _lib, grid, vals_d, nrms_d, n):
 
 def main():
     args = parse()
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched and "
               f"single-system solvers here use one MI355X. Multi-GPU sharding is design-only.",
@@ -151,14 +193,17 @@ def main():
     avg_ms = system.time_stencil(prec, iters=50)
     bytes_alg = n * 2 * sb
     achieved = bytes_alg / (avg_ms * 1e-3) / 1e9
+    info_st = system.stencil_info(prec)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"regulariser stencil ({prec}): k_st_bulk<uniform> + "
-                          f"k_st_bulk<edge> + k_st_zb launch group",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": stencil_traffic(nn, prec, info_st["zc"]),
+                "traffic_source": "profiles/stencil_traffic.json (FETCH_SIZE x2 + WRITE_SIZE, "
+                                  "copy-kernel calibrated)",
+                "kernel": f"regulariser stencil ({prec}): k_b_tiles<{prec}, zc>",
                 "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes": bytes_alg,
                 "note": f"{nn}^3 working set is Infinity-Cache (MALL) resident; the HBM-bound "
                         f"figure is roofline_hbm_512",
-                "autotune": system.stencil_info(prec)}
+                "autotune": info_st}
     log(f"stencil {prec}: {avg_ms * 1e3:.2f} us, {achieved:.0f} GB/s")
     hbm = None if args.no_hbm_sweep else hbm_sweep(torch, _lib, fs)
 

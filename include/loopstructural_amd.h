@@ -153,8 +153,8 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, cons
                           double* c, lsamd_solve_info* info, void* stream);
 
 /* Measurement: launch the regulariser stencil kernel alone `iters` times
- * (FP64 or FP32, on the handle's work vectors) bracketed by HIP events on
- * `stream`; *avg_ms = mean kernel duration. */
+ * back to back (FP64 or FP32, on the handle's work vectors, after one warm
+ * launch) between two HIP events on `stream`; *avg_ms = elapsed / iters. */
 int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
                                  double* avg_ms, void* stream);
 
