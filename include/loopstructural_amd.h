@@ -84,7 +84,13 @@ typedef struct lsamd_solve_opts {
   int64_t max_outer;    /* refinement sweeps (<=0: 50)                     */
   int32_t jacobi;       /* 1: Jacobi preconditioner = the reference's column
                            scaling (apply_scaling_matrix=True); 0: plain CG
-                           with the 2-norm stopping test                   */
+                           with the 2-norm stopping test; 2: geometric
+                           multigrid preconditioned flexible CG (FD systems
+                           only; FP32 V-cycle, FP64 outer iteration, the same
+                           Jacobi-scaled stopping test; precision, inner_rtol
+                           and max_outer unused; check_every capped at 8).
+                           No reference counterpart: the reference solves
+                           with Jacobi-scaled CG; the minimiser is the same. */
   int32_t pad;
 } lsamd_solve_opts;
 
