@@ -431,7 +431,8 @@ def p1_c3(torch, _lib, fs, synthetic):
 
 def adjoint_c5(torch, synthetic):
     """Forward solve + adjoint gradient w.r.t. 5k values, 6k weights, 3k normal
-    components at 128^3 (mixed precision), loss at 2k held-out points."""
+    components at 128^3 (multigrid-preconditioned solves), loss at 2k held-out
+    points."""
     from loopstructural_amd.adjoint import FDProblem
 
     values, normals = synthetic.constraints(5000, 1000, 1005)
@@ -439,7 +440,7 @@ def adjoint_c5(torch, synthetic):
     held = rng.uniform(0.02, 0.98, (2000, 3))
     f_held, _ = synthetic.field1(held)
     prob = FDProblem((0, 0, 0), (127,) * 3, (1 / 127,) * 3, values[:, :3], normals[:, :3],
-                     regularisation=0.1, precision="fp32", rtol=1e-10, inner_rtol=1e-3)
+                     regularisation=0.1, precision="mg", rtol=1e-10)
     dev = prob.dev
     target = torch.tensor(f_held, device=dev)
 

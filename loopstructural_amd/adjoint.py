@@ -59,6 +59,11 @@ class FDProblem:
         return fs.concat_rows([bn, bv], self.n)
 
     def _solve(self, rhs=None):
+        """Forward and adjoint solves: precision 'fp64' (Jacobi PCG), 'fp32'
+        (FP32 operator, FP64 refinement) or 'mg' (multigrid-preconditioned
+        flexible CG; the hierarchy is built once per set of rows)."""
+        if self.precision == "mg":
+            return self.system.solve(rtol=self.rtol, rhs=rhs, multigrid=True, check_every=8)
         return self.system.solve(rtol=self.rtol, precision=self.precision, rhs=rhs,
                                  inner_rtol=self.inner_rtol, check_every=128)
 

@@ -96,13 +96,14 @@ def test_adjoint_matches_oracle_differences():
         assert abs(fd - grads[i][j]) <= 1e-4 * gmax, (i, j, fd, grads[i][j])
 
 
-def test_mixed_precision_adjoint_close_to_fp64():
+@pytest.mark.parametrize("form", ["fp32", "mg"])
+def test_mixed_precision_and_multigrid_adjoint_close_to_fp64(form):
     _, p64, loss64, *_ = _setup("fp64", 1e-13)
     loss64(p64).backward()
-    _, p32, loss32, *_ = _setup("fp32", 1e-11)
-    l32 = loss32(p32)
-    l32.backward()
-    for a, b in zip(p64, p32):
+    _, p2, loss2, *_ = _setup(form, 1e-11)
+    l2 = loss2(p2)
+    l2.backward()
+    for a, b in zip(p64, p2):
         ga, gb = a.grad.reshape(-1), b.grad.reshape(-1)
         assert float((ga - gb).abs().max()) <= 1e-4 * float(ga.abs().max())
 
