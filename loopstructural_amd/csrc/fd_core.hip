@@ -1211,6 +1211,7 @@ struct MgLevel {
   int64_t* gptr = nullptr;
   int32_t* gcol = nullptr;
   float* gval = nullptr;
+  double* gval64 = nullptr;
   GridDev g;
   int64_t n = 0;
   float *b = nullptr, *x = nullptr, *q = nullptr, *d = nullptr;
@@ -1279,9 +1280,22 @@ void mg_level_apply(const MgLevel& L, const float* x, float* y, const CgCtl* ctl
   }
   lsamd_fd_system* s = L.sys;
   launch_stencil<float>(s, x, y, ctl, st, s->zc[1]);
-  hipLaunchKernelGGL(k_gram_spmv, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_gram_spmv<float>, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
                      (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const float*)L.gval, x, y,
                      L.n, ctl);
+}
+
+// FP64 level apply (setup: power iteration and the dense coarsest operator)
+int mg_level_apply64(const MgLevel& L, const double* x, double* y, hipStream_t st) {
+  lsamd_fd_system* s = L.sys;
+  if (!L.owned) return launch_apply<double>(s, x, y, s->t64, s->ctl_apply, st);
+  launch_stencil<double>(s, x, y, s->ctl_apply, st, s->zc[0]);
+  if (L.gptr)
+    hipLaunchKernelGGL(k_gram_spmv<double>, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+                       (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const double*)L.gval64,
+                       x, y, L.n, (const CgCtl*)s->ctl_apply);
+  LSAMD_CHECK_LAUNCH("multigrid level apply");
+  return LSAMD_OK;
 }
 
 lsamd_grid mg_coarse_grid(const lsamd_grid& f) {
@@ -1303,14 +1317,15 @@ int mg_prepare_level(lsamd_fd_system* sys, hipStream_t st) {
 }
 
 // lambda_max(D^-1 N) by power iteration (FP64), on the level's work vectors
-int mg_lambda(Mg* m, lsamd_fd_system* sys, double* out, hipStream_t st) {
+int mg_lambda(Mg* m, const MgLevel& L, double* out, hipStream_t st) {
+  lsamd_fd_system* sys = L.sys;
   const int64_t n = sys->n;
   const int vb = vec_blocks(n);
   double *v = sys->p64, *y = sys->q64, *w = sys->r64;
   const dim3 nb((unsigned)ceil_div(n, 256));
   hipLaunchKernelGGL(k_mg_start, nb, dim3(256), 0, st, v, n);
   for (int it = 0; it < 20; ++it) {
-    if (int rc = launch_apply<double>(sys, v, y, sys->t64, sys->ctl_apply, st)) return rc;
+    if (int rc = mg_level_apply64(L, v, y, st)) return rc;
     hipLaunchKernelGGL(k_mg_power, dim3(vb), dim3(256), 0, st, (const double*)y,
                        (const double*)sys->minv64, (const double*)v, w, n, sys->part);
     hipLaunchKernelGGL(k_mg_norm_fin, dim3(1), dim3(256), 0, st, (const double*)sys->part, vb,
@@ -1360,8 +1375,9 @@ bool mg_invert(std::vector<double>& a, int n, std::vector<double>& inv) {
 
 // G = A^T W^2 A of a coarse level's (fixed-width) data rows: pairs, stable
 // radix sort by (row, col), reduce by key in sorted order (deterministic)
-int mg_gram(Mg* m, MgLevel& L, lsamd_fd_system* cs, int wc, hipStream_t st) {
-  const int64_t nr = cs->nrows;
+int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const double* w2,
+            int64_t nr, int wc, hipStream_t st) {
+  lsamd_fd_system* cs = L.sys;
   const int64_t np = nr * wc * wc;
   if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: data term too large");
   std::vector<void*> tmp;
@@ -1383,9 +1399,8 @@ int mg_gram(Mg* m, MgLevel& L, lsamd_fd_system* cs, int wc, hipStream_t st) {
     cleanup();
     return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
   }
-  hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
-                     (const int32_t*)cs->cols, (const double*)cs->coef64,
-                     (const double*)cs->w2_64, nr, wc, key, val);
+  hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, rcols,
+                     rcoef, w2, nr, wc, key, val);
   int bits = 1;
   while (bits < 31 && ((int64_t)1 << bits) < cs->n) ++bits;
   size_t tb1 = 0, tb2 = 0;
@@ -1419,11 +1434,19 @@ int mg_gram(Mg* m, MgLevel& L, lsamd_fd_system* cs, int wc, hipStream_t st) {
     cleanup();
     return rc;
   }
+  if (int rc = mg_alloc(m, &L.gval64, m_)) {
+    cleanup();
+    return rc;
+  }
   hipLaunchKernelGGL(k_gram_ptr, dim3((unsigned)ceil_div(L.n + 1, 256)), dim3(256), 0, st,
                      (const uint64_t*)ukey, m_, L.n, L.gptr);
   hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)ceil_div(std::max<int64_t>(m_, 1), 256)),
                      dim3(256), 0, st, (const uint64_t*)ukey, (const double*)uval, m_, L.gcol,
-                     L.gval);
+                     L.gval, L.gval64);
+  // coarse diagonal: regulariser (already in cs->diag) + Gram diagonal
+  hipLaunchKernelGGL(k_gram_diag, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+                     (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const double*)uval,
+                     cs->diag, cs->minv64, cs->minv32, L.n);
   const hipError_t e = hipStreamSynchronize(st);
   cleanup();
   if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid Gram assembly failed");
@@ -1450,7 +1473,13 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   if (int rc = mg_level_vectors(m, L0)) return rc;
   m->lv.push_back(L0);
   lsamd_grid gprev = s->grid;
-  lsamd_fd_system* prev = s;
+  GridDev gdprev = s->gd;
+  // data rows of the previous level: CSR (level 0: the system's rows)
+  int64_t nr = s->nrows;
+  const int64_t* prow = s->row_ptr;
+  const int32_t* pcols = s->cols;
+  const double* pcoef = s->coef64;
+  const double* w2 = s->w2_64;
   for (int l = 1;; ++l) {
     if (gprev.nnodes[0] <= kMgCoarse && gprev.nnodes[1] <= kMgCoarse &&
         gprev.nnodes[2] <= kMgCoarse)
@@ -1470,14 +1499,13 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
     L.n = cs->n;
     m->lv.push_back(L);
     if (int rc = mg_level_vectors(m, m->lv.back())) return rc;
-    if (prev->nrows > 0) {
-      const int64_t nr = prev->nrows;
+    if (int rc = mg_prepare_level(cs, st)) return rc;
+    if (nr > 0) {
       int32_t* cnt = nullptr;
       if (int rc = mg_alloc(m, &cnt, nr)) return rc;
-      const GridDev gf = prev->gd, gcd = cs->gd;
-      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
-                         (const int64_t*)prev->row_ptr, (const int32_t*)prev->cols,
-                         (const double*)prev->coef64, nr, gf, gcd, 1, 0, cnt, (int32_t*)nullptr,
+      const GridDev gcd = cs->gd;
+      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, prow,
+                         pcols, pcoef, nr, gdprev, gcd, 1, 0, cnt, (int32_t*)nullptr,
                          (double*)nullptr);
       std::vector<int32_t> hc(nr);
       LSAMD_HIP(hipMemcpyAsync(hc.data(), cnt, nr * 4, hipMemcpyDeviceToHost, st));
@@ -1494,19 +1522,16 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
       std::vector<int64_t> hr(nr + 1);
       for (int64_t r = 0; r <= nr; ++r) hr[r] = r * wc;
       LSAMD_HIP(hipMemcpyAsync(rptr, hr.data(), (nr + 1) * 8, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st,
-                         (const int64_t*)prev->row_ptr, (const int32_t*)prev->cols,
-                         (const double*)prev->coef64, nr, gf, gcd, 0, wc, (int32_t*)nullptr,
-                         ccols, ccoef);
+      hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, prow,
+                         pcols, pcoef, nr, gdprev, gcd, 0, wc, (int32_t*)nullptr, ccols, ccoef);
       LSAMD_CHECK_LAUNCH("multigrid rows");
-      if (int rc = lsamd_fd_system_set_rows(cs, nr, nr * wc, rptr, ccols, ccoef, prev->b, prev->w,
-                                            st))
-        return rc;
-      if (int rc = mg_gram(m, m->lv.back(), cs, wc, st)) return rc;
+      if (int rc = mg_gram(m, m->lv.back(), ccols, ccoef, w2, nr, wc, st)) return rc;
+      prow = rptr;
+      pcols = ccols;
+      pcoef = ccoef;
     }
-    if (int rc = mg_prepare_level(cs, st)) return rc;
-    prev = cs;
     gprev = gc;
+    gdprev = cs->gd;
   }
   // coarsest level: dense inverse of its FP64 operator
   MgLevel& C = m->lv.back();
@@ -1517,9 +1542,7 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   for (int j = 0; j < nc; ++j) {
     hipLaunchKernelGGL(k_mg_unit, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, st,
                        C.sys->p64, (int64_t)nc, (int64_t)j);
-    if (int rc = launch_apply<double>(C.sys, C.sys->p64, C.sys->q64, C.sys->t64,
-                                      C.sys->ctl_apply, st))
-      return rc;
+    if (int rc = mg_level_apply64(C, C.sys->p64, C.sys->q64, st)) return rc;
     LSAMD_HIP(hipMemcpyAsync(&a[(size_t)j * nc], C.sys->q64, nc * 8, hipMemcpyDeviceToHost, st));
   }
   LSAMD_HIP(hipStreamSynchronize(st));
@@ -1530,7 +1553,7 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   // smoother coefficients of the other levels
   for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
     MgLevel& L = m->lv[l];
-    if (int rc = mg_lambda(m, L.sys, &L.lambda, st)) return rc;
+    if (int rc = mg_lambda(m, L, &L.lambda, st)) return rc;
     const double lmax = 1.15 * L.lambda, lmin = L.lambda / 30.0;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;

@@ -170,23 +170,41 @@ __global__ void k_gram_ptr(const uint64_t* __restrict__ ukey, int64_t m, int64_t
 }
 
 __global__ void k_gram_cols(const uint64_t* __restrict__ ukey, const double* __restrict__ v,
-                            int64_t m, int32_t* __restrict__ col, float* __restrict__ v32) {
+                            int64_t m, int32_t* __restrict__ col, float* __restrict__ v32,
+                            double* __restrict__ v64) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
   col[e] = (int32_t)(ukey[e] & 0xffffffffu);
   v32[e] = (float)v[e];
+  v64[e] = v[e];
 }
 
-// y += G x (FP32), one node per thread
+// y += G x, one node per thread
+template <typename T>
 __global__ void k_gram_spmv(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                            const float* __restrict__ val, const float* __restrict__ x,
-                            float* __restrict__ y, int64_t n, const CgCtl* ctl) {
+                            const T* __restrict__ val, const T* __restrict__ x,
+                            T* __restrict__ y, int64_t n, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float acc = 0.f;
+  T acc = 0;
   for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) acc += val[e] * x[col[e]];
   y[i] += acc;
+}
+
+// diag += diagonal of G; then the Jacobi inverses
+__global__ void k_gram_diag(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, double* __restrict__ diag,
+                            double* __restrict__ m64, float* __restrict__ m32, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = diag[i];
+  for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e)
+    if (col[e] == (int32_t)i) d += val[e];
+  diag[i] = d;
+  const double m = d > 0.0 ? 1.0 / d : 1.0;
+  m64[i] = m;
+  m32[i] = (float)m;
 }
 
 // Coarse data rows A P: row r of width wf (columns on grid gf) mapped through
