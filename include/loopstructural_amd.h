@@ -22,6 +22,7 @@
  *                          LI/_discrete_interpolator.py:684-755,815-833 (build_matrix, scaling)
  *                          LI/_solver_pipeline.py:86-124 (ridge, column scaling)
  *                          LI/_solver_strategy.py:48-71 (cg on A.T@A)
+ *   lsamd_fd_system_lsmr   LI/_solver_strategy.py:108-136 (lsmr on A S)
  *   lsamd_fd_evaluate      LC/supports/_3d_structured_grid.py:295-391
  *                          LI/_discrete_interpolator.py:1342-1379
  *   lsamd_rows_adjoint     no reference counterpart (gradient of a held-out
@@ -157,6 +158,37 @@ int lsamd_fd_system_diag(lsamd_fd_system* s, double* diag, void* stream);
 /* Solve N c = rhs (rhs NULL: the system's own).  c (nnodes) FP64, output. */
 int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, const double* rhs,
                           double* c, lsamd_solve_info* info, void* stream);
+
+/* A13: LSMR on the rectangular least-squares system of an FD handle
+ * (replaces solve_with_lsmr, LI/_solver_strategy.py:108-136, i.e.
+ * scipy.sparse.linalg.lsmr(A S, b, atol, btol, conlim, maxiter) on the
+ * ridge-augmented, column-scaled system of LI/_solver_pipeline.py:86-124,
+ * then c = S y).  Matrix-free: data rows from set_rows, regulariser rows
+ * from the stencil definition (rows of one mask centred at one node merged,
+ * which leaves every LSMR iterate unchanged), ridge rows.  FP64 throughout.
+ * maxiter <= 0 means the number of nodes (scipy's min(m, n)); conlim <= 0
+ * disables the condition test.  FD handles only (not P1 or batches). */
+typedef struct lsamd_lsmr_opts {
+  double atol;
+  double btol;
+  double conlim;
+  int64_t maxiter;
+  int32_t check_every;  /* iterations per captured graph chunk (<=0: 64)   */
+  int32_t scaling;      /* 1: column scaling S (apply_scaling_matrix=True);
+                           0: S = I                                        */
+} lsamd_lsmr_opts;
+
+typedef struct lsamd_lsmr_info {
+  int64_t iterations;
+  int32_t istop;        /* scipy's codes: 0 b = 0, 1 |r| small, 2 LS optimal,
+                           3 cond > conlim, 4-6 the same at machine
+                           precision, 7 maxiter                            */
+  int32_t pad;
+  double normr, normar, normA, condA, normx;
+} lsamd_lsmr_info;
+
+int lsamd_fd_system_lsmr(lsamd_fd_system* s, const lsamd_lsmr_opts* opts, double* c,
+                         lsamd_lsmr_info* info, void* stream);
 
 /* Measurement: launch the regulariser stencil kernel alone `iters` times
  * back to back (FP64 or FP32, on the handle's work vectors, after one warm

@@ -53,6 +53,17 @@ class SolveInfo(C.Structure):
                 ("pad", C.c_int32), ("residual", C.c_double), ("threshold", C.c_double)]
 
 
+class LsmrOpts(C.Structure):
+    _fields_ = [("atol", C.c_double), ("btol", C.c_double), ("conlim", C.c_double),
+                ("maxiter", C.c_int64), ("check_every", C.c_int32), ("scaling", C.c_int32)]
+
+
+class LsmrInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("istop", C.c_int32), ("pad", C.c_int32),
+                ("normr", C.c_double), ("normar", C.c_double), ("normA", C.c_double),
+                ("condA", C.c_double), ("normx", C.c_double)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _I32 = C.c_int32
@@ -72,6 +83,7 @@ SIGNATURES = {
     "lsamd_fd_system_rhs": [_P, _P, _P],
     "lsamd_fd_system_diag": [_P, _P, _P],
     "lsamd_fd_system_solve": [_P, _P, _P, _P, _P, _P],
+    "lsamd_fd_system_lsmr": [_P, _P, _P, _P, _P],
     "lsamd_fd_evaluate": [_P, _P, _P, _I64, _P, _P, _P],
     "lsamd_rows_adjoint": [_P, _P, _P, _P, _P, _P],
     "lsamd_rows_normalise": [_I64, _I32, _P, _P, _P, _P],

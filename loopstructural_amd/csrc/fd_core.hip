@@ -707,6 +707,7 @@ __global__ void k_check_cols(const int32_t* __restrict__ cols, int64_t nnz, int6
 }
 
 #include "batch_kernels.hpp"
+#include "lsmr.hpp"
 #include "mg.hpp"
 
 // rows of system s may only address its own nodes; one workgroup per system
@@ -725,10 +726,13 @@ using namespace lsamd;
 namespace lsamd {
 struct Mg;
 void mg_destroy(Mg* m);
+struct Ls;
+void ls_destroy(Ls* l);
 }  // namespace lsamd
 
 struct lsamd_fd_system {
   lsamd::Mg* mg = nullptr;  // multigrid hierarchy (built on first MG solve after set_rows)
+  lsamd::Ls* ls = nullptr;  // LSMR work (built on the first LSMR solve after set_rows)
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -1696,6 +1700,190 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
   return LSAMD_OK;
 }
 
+
+// ------------------------------------------------------------------ LSMR ----
+struct Ls {
+  double *v = nullptr, *h = nullptr, *hbar = nullptr, *x = nullptr, *xs = nullptr, *y = nullptr;
+  double *ureg = nullptr, *urid = nullptr, *ud = nullptr, *tr = nullptr;
+  double *pa = nullptr, *pb = nullptr, *pv = nullptr, *px = nullptr;  // partials
+  int na = 0, nb = 0, nv = 0;
+  LsCtl* ctl = nullptr;
+  LsCtl* host = nullptr;
+  std::vector<void*> mem;
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+  const double* graph_scale = nullptr;  // column scaling the graph was captured with
+};
+
+void ls_destroy(Ls* l) {
+  if (!l) return;
+  if (l->graph) hipGraphExecDestroy(l->graph);
+  for (void* p : l->mem) hipFree(p);
+  if (l->host) hipHostFree(l->host);
+  delete l;
+}
+
+namespace {
+
+template <typename P>
+int ls_alloc(Ls* l, P** p, int64_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<int64_t>(count, 1) * sizeof(P)) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (LSMR)");
+  l->mem.push_back(q);
+  *p = (P*)q;
+  return LSAMD_OK;
+}
+
+int ls_build(lsamd_fd_system* s, Ls* l) {
+  const int64_t n = s->n, m = std::max<int64_t>(s->nrows, 1);
+  for (double** p : {&l->v, &l->h, &l->hbar, &l->x, &l->xs, &l->y, &l->urid})
+    if (int rc = ls_alloc(l, p, n)) return rc;
+  if (int rc = ls_alloc(l, &l->ureg, 6 * n)) return rc;
+  if (int rc = ls_alloc(l, &l->ud, m)) return rc;
+  if (int rc = ls_alloc(l, &l->tr, m)) return rc;
+  l->na = (int)ceil_div(n, 256);
+  l->nb = (int)ceil_div(m, 256);
+  l->nv = vec_blocks(n);
+  if (int rc = ls_alloc(l, &l->pa, l->na)) return rc;
+  if (int rc = ls_alloc(l, &l->pb, l->nb)) return rc;
+  if (int rc = ls_alloc(l, &l->pv, l->nv)) return rc;
+  if (int rc = ls_alloc(l, &l->px, l->nv)) return rc;
+  if (int rc = ls_alloc(l, &l->ctl, 1)) return rc;
+  if (hipHostMalloc((void**)&l->host, sizeof(LsCtl), hipHostMallocDefault) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed (LSMR)");
+  for (double* p : {l->v, l->h, l->hbar, l->x, l->xs, l->y, l->urid})
+    LSAMD_HIP(hipMemset(p, 0, n * 8));
+  LSAMD_HIP(hipMemset(l->ureg, 0, 6 * n * 8));
+  LSAMD_HIP(hipMemset(l->ud, 0, m * 8));
+  LSAMD_HIP(hipMemset(l->tr, 0, m * 8));
+  return LSAMD_OK;
+}
+
+// u <- A S xs - fwd_coef u ; beta ; v <- S A^T u / beta - beta v ; partials
+// of |v|^2.  init: u = b instead of the product, v = S A^T u / beta.
+// sc: squared column scaling (diag(A^T A)^-1, or ones)
+void ls_bidiag(lsamd_fd_system* s, Ls* l, const double* sc, int init, hipStream_t st) {
+  const int64_t n = s->n;
+  const RegW<double> w = make_regw<double>(s->reg);
+  RowsDev R = rows_dev(s, false);
+  const dim3 nb((unsigned)l->na), b256(256);
+  if (!init)
+    hipLaunchKernelGGL(k_ls_scale, nb, b256, 0, st, (const double*)l->v,
+                       sc, l->xs, n, (const LsCtl*)l->ctl);
+  hipLaunchKernelGGL(k_ls_reg_fwd, nb, b256, 0, st, (const double*)l->xs, s->gd, w,
+                     (const double*)s->scale2, l->ureg, l->urid, n, init, l->pa,
+                     (const LsCtl*)l->ctl);
+  int nbp = 0;
+  if (s->nrows > 0) {
+    nbp = l->nb;
+    hipLaunchKernelGGL(k_ls_data_fwd, dim3((unsigned)l->nb), b256, 0, st, (const double*)l->xs,
+                       R, (const double*)s->w, (const double*)s->b, l->ud, init, l->pb,
+                       (const LsCtl*)l->ctl);
+  }
+  hipLaunchKernelGGL(k_ls_beta, dim3(1), b256, 0, st, (const double*)l->pa, l->na,
+                     (const double*)l->pb, nbp, l->ctl, init);
+  hipLaunchKernelGGL(k_ls_reg_t, nb, b256, 0, st, (const double*)l->ureg, (const double*)l->urid,
+                     s->gd, w, (const double*)s->scale2, l->y, n, (const LsCtl*)l->ctl);
+  if (s->nrows > 0) {
+    hipLaunchKernelGGL(k_ls_data_t, dim3((unsigned)l->nb), b256, 0, st, (const double*)l->ud,
+                       (const double*)s->w, l->tr, s->nrows, (const LsCtl*)l->ctl);
+    hipLaunchKernelGGL((k_rows_scatter<double, double>),
+                       dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256))), b256, 0, st, R,
+                       (const double*)l->tr, l->y, 1, (CgCtl*)nullptr, 0, (const double*)nullptr,
+                       0, (const double*)nullptr, 0);
+  }
+  hipLaunchKernelGGL(k_ls_vupd, dim3((unsigned)l->nv), b256, 0, st, (const double*)l->y,
+                     sc, l->v, n, init, l->pv, (const LsCtl*)l->ctl);
+  hipLaunchKernelGGL(k_ls_step, dim3(1), b256, 0, st, (const double*)l->pv, l->nv, l->ctl, init);
+  hipLaunchKernelGGL(k_ls_xupd, dim3((unsigned)l->nv), b256, 0, st, l->v, l->h, l->hbar, l->x, n,
+                     init, l->px, (const LsCtl*)l->ctl);
+  if (!init)
+    hipLaunchKernelGGL(k_ls_test, dim3(1), b256, 0, st, (const double*)l->px, l->nv, l->ctl);
+}
+
+__global__ void k_ls_setup(LsCtl* c, double atol, double btol, double ctol, int maxiter) {
+  LsCtl z = {};
+  z.atol = atol;
+  z.btol = btol;
+  z.ctol = ctol;
+  z.maxiter = maxiter;
+  *c = z;
+}
+
+}  // namespace
+
+int ls_solve(lsamd_fd_system* s, const lsamd_lsmr_opts* o, double* c, lsamd_lsmr_info* inf,
+             hipStream_t st) {
+  if (s->op != 0) return fail(LSAMD_ERR_ARG, "LSMR is available for FD systems only");
+  if (!s->ls) {
+    Ls* l = new Ls();
+    if (int rc = ls_build(s, l)) {
+      ls_destroy(l);
+      return rc;
+    }
+    s->ls = l;
+  }
+  Ls* l = s->ls;
+  const int64_t n = s->n;
+  const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : n;
+  const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
+  const int chunk = o->check_every > 0 ? o->check_every : 64;
+  const double ctol = o->conlim > 0 ? 1.0 / o->conlim : 0.0;
+  hipLaunchKernelGGL(k_ls_setup, dim3(1), dim3(1), 0, st, l->ctl, o->atol, o->btol, ctol, maxiter);
+  const double* sc = s->minv64;
+  if (!o->scaling) {
+    if (!s->ones64) {
+      if (int rc = dalloc(s, &s->ones64, n)) return rc;
+      if (int rc = dalloc(s, &s->ones32, n)) return rc;
+      hipLaunchKernelGGL(k_fill_one, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                         s->ones64, s->ones32, n);
+    }
+    sc = s->ones64;
+  }
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, l->x, n);
+  ls_bidiag(s, l, sc, 1, st);
+  LSAMD_CHECK_LAUNCH("lsmr init");
+  if (l->graph && (l->graph_chunk != chunk || l->graph_scale != sc)) {
+    hipGraphExecDestroy(l->graph);
+    l->graph = nullptr;
+  }
+  if (!l->graph) {
+    hipStream_t cs;
+    LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    LSAMD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; ++i) ls_bidiag(s, l, sc, 0, cs);
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "graph capture failed (LSMR)");
+    e = hipGraphInstantiate(&l->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "graph instantiate failed (LSMR)");
+    l->graph_chunk = chunk;
+    l->graph_scale = sc;
+  }
+  for (;;) {
+    LSAMD_HIP(hipMemcpyAsync(l->host, l->ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    if (l->host->done) break;
+    LSAMD_HIP(hipGraphLaunch(l->graph, st));
+  }
+  hipLaunchKernelGGL(k_ls_out, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                     (const double*)l->x, sc, c, n);
+  LSAMD_CHECK_LAUNCH("lsmr out");
+  LSAMD_HIP(hipStreamSynchronize(st));
+  const LsCtl* h = l->host;
+  inf->iterations = h->itn;
+  inf->istop = h->istop;
+  inf->normr = h->normr;
+  inf->normar = h->normar;
+  inf->normA = h->normA;
+  inf->condA = h->condA;
+  inf->normx = h->normx;
+  return LSAMD_OK;
+}
+
 }  // namespace lsamd
 
 extern "C" {
@@ -1801,6 +1989,7 @@ int lsamd_fd_system_destroy(lsamd_fd_system* s) {
     if (s->bgraph[i]) hipGraphExecDestroy(s->bgraph[i]);
   }
   if (s->mg) mg_destroy(s->mg);
+  if (s->ls) ls_destroy(s->ls);
   for (void* p : s->owned) hipFree(p);
   if (s->ctl_host) hipHostFree(s->ctl_host);
   if (s->bctl_host) hipHostFree(s->bctl_host);
@@ -1823,6 +2012,10 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
   if (s->mg) {
     mg_destroy(s->mg);
     s->mg = nullptr;
+  }
+  if (s->ls) {
+    ls_destroy(s->ls);
+    s->ls = nullptr;
   }
   for (int i = 0; i < 2; ++i) {
     if (s->graph[i]) {
@@ -2116,6 +2309,19 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const d
   } else {
     return fail(LSAMD_ERR_ARG, "unknown precision");
   }
+  if (info) *info = inf;
+  return LSAMD_OK;
+}
+
+int lsamd_fd_system_lsmr(lsamd_fd_system* s, const lsamd_lsmr_opts* o, double* c,
+                         lsamd_lsmr_info* info, void* stream) {
+  if (int rc = injected("lsmr")) return rc;
+  if (!s || !o || !c) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_lsmr: NULL argument");
+  if (!(o->atol >= 0.0) || !(o->btol >= 0.0))
+    return fail(LSAMD_ERR_ARG, "atol/btol must be non-negative");
+  lsamd_lsmr_info inf;
+  memset(&inf, 0, sizeof(inf));
+  if (int rc = ls_solve(s, o, c, &inf, (hipStream_t)stream)) return rc;
   if (info) *info = inf;
   return LSAMD_OK;
 }

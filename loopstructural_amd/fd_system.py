@@ -178,6 +178,27 @@ class FdSystem:
                    "converged": bool(info.converged), "residual": float(info.residual),
                    "threshold": float(info.threshold)}
 
+    def lsmr(self, atol=1e-6, btol=1e-6, conlim=1e8, maxiter=None, check_every=64,
+             scaling=True):
+        """LSMR on the rectangular least-squares system (ridge rows and the
+        column scaling included), as scipy.sparse.linalg.lsmr with the same
+        atol/btol/conlim/maxiter meaning; returns c = S y and scipy's
+        (istop, itn, normr, normar, normA, condA, normx) as a dict."""
+        o = _lib.LsmrOpts()
+        o.atol, o.btol = float(atol), float(btol)
+        o.conlim = float(conlim) if conlim else 0.0
+        o.maxiter = int(maxiter) if maxiter else 0
+        o.check_every = int(check_every)
+        o.scaling = 1 if scaling else 0
+        info = _lib.LsmrInfo()
+        c = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_lsmr", self._h, C.byref(o), _lib.ptr(c), C.byref(info),
+                  _lib.stream())
+        return c, {"istop": int(info.istop), "iterations": int(info.iterations),
+                   "normr": float(info.normr), "normar": float(info.normar),
+                   "normA": float(info.normA), "condA": float(info.condA),
+                   "normx": float(info.normx)}
+
     def time_stencil(self, precision="fp64", iters=20) -> float:
         """Mean duration (ms) of the stencil kernel alone, HIP events on the
         current stream."""

@@ -560,8 +560,11 @@ class DiscreteInterpolator(GeologicalInterpolator):
         if callable(choice):
             raise NotImplementedError(
                 "a callable solver needs the explicit sparse system; use solver='cg'")
+        if choice == "lsmr" and not isinstance(system, fs.P1System):
+            return self._solve_lsmr(system, tol, kw, timing, started)
         if choice == "lsmr":
-            logger.warning("lsmr runs as GPU CG on the normal equations (same minimiser)")
+            logger.warning("lsmr on P1 supports runs as GPU CG on the normal equations "
+                           "(same minimiser)")
             kw["rtol"] = kw.pop("btol", tol if tol is not None else 1e-6)
             for k in ("atol", "conlim", "damp", "show"):
                 kw.pop(k, None)
@@ -588,6 +591,40 @@ class DiscreteInterpolator(GeologicalInterpolator):
         if not info["converged"]:
             logger.warning("CG reached iteration limit (%s) and did not converge; using last "
                            "iteration", info["iterations"])
+        self._c_dev = c
+        self.c = c.cpu().numpy()
+        self.up_to_date = True
+        timing["total_seconds"] = perf_counter() - started
+        timing["up_to_date"] = True
+        self.latest_solve_timing = timing
+        return self.up_to_date
+
+    def _solve_lsmr(self, system, tol, kw, timing, started) -> bool:
+        """solve_with_lsmr (LI/_solver_strategy.py:108-136): LSMR on the
+        ridge-augmented, column-scaled rectangular system, matrix-free on the
+        GPU; ``tol`` sets btol (and atol = 0) unless btol is given."""
+        if "btol" not in kw and tol is not None:
+            kw["btol"] = tol
+            kw["atol"] = 0.0
+        for k in ("damp", "show", "x0"):
+            if kw.pop(k, None):
+                logger.warning("solver_kwargs[%r] is not supported by the GPU LSMR; ignored", k)
+        kw.pop("preconditioner", None)
+        kw.pop("precision", None)
+        t1 = perf_counter()
+        c, info = system.lsmr(atol=kw.pop("atol", 1e-6), btol=kw.pop("btol", 1e-6),
+                              conlim=kw.pop("conlim", 1e8), maxiter=kw.pop("maxiter", None),
+                              scaling=self.apply_scaling_matrix)
+        torch.cuda.synchronize()
+        for k in kw:
+            logger.warning("solver_kwargs[%r] is not supported by the GPU LSMR; ignored", k)
+        timing["solve_seconds"] = perf_counter() - t1
+        timing.update({"iterations": info["iterations"], "istop": info["istop"],
+                       "normr": info["normr"], "normar": info["normar"]})
+        if info["istop"] == 0:
+            logger.warning("Solution to least squares problem is all zeros, check input data")
+        elif info["istop"] in (3, 6):
+            logger.warning("COND(A) seems to be greater than CONLIM, check input data")
         self._c_dev = c
         self.c = c.cpu().numpy()
         self.up_to_date = True
