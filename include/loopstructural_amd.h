@@ -87,9 +87,13 @@ typedef struct lsamd_solve_opts {
                            scaling (apply_scaling_matrix=True); 0: plain CG
                            with the 2-norm stopping test; 2: geometric
                            multigrid preconditioned flexible CG (FD systems
-                           only; FP32 V-cycle, FP64 outer iteration, the same
+                           only; FP32 V-cycle with an additive-Schwarz
+                           Chebyshev smoother, FP64 outer iteration, the same
                            Jacobi-scaled stopping test; precision, inner_rtol
-                           and max_outer unused; check_every capped at 8).
+                           and max_outer unused; check_every capped at 8.
+                           Environment: LSAMD_MG_SMOOTHER=jacobi for point
+                           Jacobi, LSAMD_MG_DEGREE=1..4 for the Chebyshev
+                           degree (2), LSAMD_MG_TIMING=1 prints setup times).
                            No reference counterpart: the reference solves
                            with Jacobi-scaled CG; the minimiser is the same. */
   int32_t pad;

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <hipcub/hipcub.hpp>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1219,12 +1220,21 @@ struct MgLevel {
   GridDev g;
   int64_t n = 0;
   float *b = nullptr, *x = nullptr, *q = nullptr, *d = nullptr;
-  float c0 = 0.f, c1[kMgDeg] = {0.f}, c2[kMgDeg] = {0.f};
+  float c0 = 0.f, c1[kMgDegMax] = {0.f}, c2[kMgDegMax] = {0.f};
   double lambda = 0.0;
+  // smoother: tags (Jacobi scale, or -1 on Schwarz rows) and the summed block
+  // inverses as CSR over all nodes (sptr == nullptr: point Jacobi only)
+  float* sm = nullptr;
+  int64_t* sptr = nullptr;
+  int32_t* scol = nullptr;
+  float* sval = nullptr;
+  double* sval64 = nullptr;
+  int64_t nblocks = 0;
 };
 
 struct Mg {
   std::vector<MgLevel> lv;
+  int deg = 2;  // Chebyshev degree
   float* cinv = nullptr;  // dense inverse of the coarsest operator
   int nc = 0;
   double* z64 = nullptr;
@@ -1331,7 +1341,9 @@ int mg_lambda(Mg* m, const MgLevel& L, double* out, hipStream_t st) {
   for (int it = 0; it < 20; ++it) {
     if (int rc = mg_level_apply64(L, v, y, st)) return rc;
     hipLaunchKernelGGL(k_mg_power, dim3(vb), dim3(256), 0, st, (const double*)y,
-                       (const double*)sys->minv64, (const double*)v, w, n, sys->part);
+                       (const double*)sys->minv64, (const float*)(L.sm ? L.sm : sys->minv32),
+                       (const int64_t*)L.sptr, (const int32_t*)L.scol, (const double*)L.sval64,
+                       (const double*)v, w, n, sys->part);
     hipLaunchKernelGGL(k_mg_norm_fin, dim3(1), dim3(256), 0, st, (const double*)sys->part, vb,
                        m->scal);
     hipLaunchKernelGGL(k_mg_scale, nb, dim3(256), 0, st, (const double*)w, v, n,
@@ -1379,11 +1391,18 @@ bool mg_invert(std::vector<double>& a, int n, std::vector<double>& inv) {
 
 // G = A^T W^2 A of a coarse level's (fixed-width) data rows: pairs, stable
 // radix sort by (row, col), reduce by key in sorted order (deterministic)
-int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const double* w2,
-            int64_t nr, int wc, hipStream_t st) {
-  lsamd_fd_system* cs = L.sys;
-  const int64_t np = nr * wc * wc;
-  if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: data term too large");
+// (row << 32 | col, value) pairs -> CSR over n rows with duplicates summed
+// (radix sort + reduce-by-key); the outputs are level memory, the pair
+// arrays are the caller's
+struct CsrOut {
+  int64_t* ptr = nullptr;
+  int32_t* col = nullptr;
+  float* v32 = nullptr;
+  double* v64 = nullptr;
+};
+int pairs_csr(Mg* m, const uint64_t* key, const double* val, int64_t np, int64_t n, CsrOut* out,
+              hipStream_t st) {
+  if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: too many matrix entries");
   std::vector<void*> tmp;
   auto talloc = [&](void** p, size_t bytes) {
     if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
@@ -1393,20 +1412,17 @@ int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const 
   auto cleanup = [&]() {
     for (void* p : tmp) hipFree(p);
   };
-  uint64_t *key = nullptr, *key_s = nullptr, *ukey = nullptr;
-  double *val = nullptr, *val_s = nullptr, *uval = nullptr;
+  uint64_t *key_s = nullptr, *ukey = nullptr;
+  double *val_s = nullptr, *uval = nullptr;
   int32_t* nrun = nullptr;
-  if (!talloc((void**)&key, np * 8) || !talloc((void**)&key_s, np * 8) ||
-      !talloc((void**)&val, np * 8) || !talloc((void**)&val_s, np * 8) ||
+  if (!talloc((void**)&key_s, np * 8) || !talloc((void**)&val_s, np * 8) ||
       !talloc((void**)&ukey, np * 8) || !talloc((void**)&uval, np * 8) ||
       !talloc((void**)&nrun, 8)) {
     cleanup();
-    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid CSR)");
   }
-  hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, rcols,
-                     rcoef, w2, nr, wc, key, val);
   int bits = 1;
-  while (bits < 31 && ((int64_t)1 << bits) < cs->n) ++bits;
+  while (bits < 31 && ((int64_t)1 << bits) < n) ++bits;
   size_t tb1 = 0, tb2 = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, key, key_s, val, val_s, (int)np, 0, 32 + bits,
                                      st);
@@ -1416,7 +1432,7 @@ int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const 
   const size_t tb = std::max(tb1, tb2);
   if (!talloc(&work, tb)) {
     cleanup();
-    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid CSR)");
   }
   size_t t1 = tb, t2 = tb;
   hipcub::DeviceRadixSort::SortPairs(work, t1, key, key_s, val, val_s, (int)np, 0, 32 + bits, st);
@@ -1426,38 +1442,237 @@ int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const 
   LSAMD_HIP(hipMemcpyAsync(&nrun_h, nrun, 4, hipMemcpyDeviceToHost, st));
   LSAMD_HIP(hipStreamSynchronize(st));
   const int64_t m_ = nrun_h;
-  if (int rc = mg_alloc(m, &L.gptr, L.n + 1)) {
+  int rc = mg_alloc(m, &out->ptr, n + 1);
+  if (!rc) rc = mg_alloc(m, &out->col, m_);
+  if (!rc) rc = mg_alloc(m, &out->v32, m_);
+  if (!rc) rc = mg_alloc(m, &out->v64, m_);
+  if (rc) {
     cleanup();
     return rc;
   }
-  if (int rc = mg_alloc(m, &L.gcol, m_)) {
-    cleanup();
-    return rc;
-  }
-  if (int rc = mg_alloc(m, &L.gval, m_)) {
-    cleanup();
-    return rc;
-  }
-  if (int rc = mg_alloc(m, &L.gval64, m_)) {
-    cleanup();
-    return rc;
-  }
-  hipLaunchKernelGGL(k_gram_ptr, dim3((unsigned)ceil_div(L.n + 1, 256)), dim3(256), 0, st,
-                     (const uint64_t*)ukey, m_, L.n, L.gptr);
+  hipLaunchKernelGGL(k_gram_ptr, dim3((unsigned)ceil_div(n + 1, 256)), dim3(256), 0, st,
+                     (const uint64_t*)ukey, m_, n, out->ptr);
   hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)ceil_div(std::max<int64_t>(m_, 1), 256)),
-                     dim3(256), 0, st, (const uint64_t*)ukey, (const double*)uval, m_, L.gcol,
-                     L.gval, L.gval64);
-  // coarse diagonal: regulariser (already in cs->diag) + Gram diagonal
-  hipLaunchKernelGGL(k_gram_diag, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
-                     (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const double*)uval,
-                     cs->diag, cs->minv64, cs->minv32, L.n);
+                     dim3(256), 0, st, (const uint64_t*)ukey, (const double*)uval, m_, out->col,
+                     out->v32, out->v64);
   const hipError_t e = hipStreamSynchronize(st);
   cleanup();
-  if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid Gram assembly failed");
-  LSAMD_CHECK_LAUNCH("multigrid Gram");
+  if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid CSR assembly failed");
+  LSAMD_CHECK_LAUNCH("multigrid CSR");
   return LSAMD_OK;
 }
 
+// Gram matrix G = A^T W^2 A of fixed-width rows (wc columns each) on n nodes
+int gram_csr(Mg* m, const int32_t* rcols, const double* rcoef, const double* w2, int64_t nr,
+             int wc, int64_t n, CsrOut* out, hipStream_t st) {
+  const int64_t np = nr * wc * wc;
+  if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: data term too large");
+  uint64_t* key = nullptr;
+  double* val = nullptr;
+  if (hipMalloc((void**)&key, std::max<int64_t>(np, 1) * 8) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+  if (hipMalloc((void**)&val, std::max<int64_t>(np, 1) * 8) != hipSuccess) {
+    hipFree(key);
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
+  }
+  hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, rcols,
+                     rcoef, w2, nr, wc, key, val);
+  const int rc = pairs_csr(m, key, val, np, n, out, st);
+  hipFree(key);
+  hipFree(val);
+  return rc;
+}
+
+int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const double* w2,
+            int64_t nr, int wc, hipStream_t st) {
+  lsamd_fd_system* cs = L.sys;
+  CsrOut G;
+  if (int rc = gram_csr(m, rcols, rcoef, w2, nr, wc, L.n, &G, st)) return rc;
+  L.gptr = G.ptr;
+  L.gcol = G.col;
+  L.gval = G.v32;
+  L.gval64 = G.v64;
+  // coarse diagonal: regulariser (already in cs->diag) + Gram diagonal
+  hipLaunchKernelGGL(k_gram_diag, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+                     (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const double*)L.gval64,
+                     cs->diag, cs->minv64, cs->minv32, L.n);
+  LSAMD_CHECK_LAUNCH("multigrid Gram diagonal");
+  return LSAMD_OK;
+}
+
+void mg_free(Mg* m, void* p) {
+  if (!p) return;
+  for (size_t i = 0; i < m->mem.size(); ++i)
+    if (m->mem[i] == p) {
+      hipFree(p);
+      m->mem.erase(m->mem.begin() + i);
+      return;
+    }
+}
+
+// Schwarz blocks of one level: boundary-layer blocks of its grid and the
+// 8-node blocks of `cells` (min-corner node ids); G is the level's data Gram
+// matrix (nullable).  Builds L.sm / L.sptr / L.scol / L.sval / L.sval64.
+int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOut& G,
+               hipStream_t st) {
+  const GridDev g = L.g;
+  const int64_t nx = g.nx, ny = g.ny, nxy = nx * ny;
+  struct Grp {
+    int start, len;
+    bool bnd;
+  };
+  auto groups = [](int n) {
+    std::vector<Grp> v;
+    if (n <= 6) {
+      v.push_back({0, n, true});
+      return v;
+    }
+    v.push_back({0, 3, true});
+    for (int i = 3; i < n - 3; ++i) v.push_back({i, 1, false});
+    v.push_back({n - 3, 3, true});
+    return v;
+  };
+  const std::vector<Grp> gx = groups(g.nx), gy = groups(g.ny), gz = groups(g.nz);
+  std::vector<Grp> gxb;
+  for (const Grp& a : gx)
+    if (a.bnd) gxb.push_back(a);
+  std::vector<int64_t> boff(1, 0), moff(1, 0);
+  std::vector<int32_t> nodes;
+  auto add_block = [&](const Grp& a, const Grp& b, const Grp& c) {
+    for (int k = c.start; k < c.start + c.len; ++k)
+      for (int j = b.start; j < b.start + b.len; ++j)
+        for (int i = a.start; i < a.start + a.len; ++i)
+          nodes.push_back((int32_t)(i + nx * j + nxy * k));
+    const int64_t sz = (int64_t)a.len * b.len * c.len;
+    boff.push_back(boff.back() + sz);
+    moff.push_back(moff.back() + sz * sz);
+  };
+  for (const Grp& c : gz)
+    for (const Grp& b : gy) {
+      const std::vector<Grp>& xs = (c.bnd || b.bnd) ? gx : gxb;
+      for (const Grp& a : xs) add_block(a, b, c);
+    }
+  for (int64_t cell : cells) {
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) nodes.push_back((int32_t)(cell + di + nx * dj + nxy * dk));
+    boff.push_back(boff.back() + 8);
+    moff.push_back(moff.back() + 64);
+  }
+  const int nb = (int)boff.size() - 1;
+  L.nblocks = nb;
+  if (nb == 0) return LSAMD_OK;
+  const int64_t ntot = moff.back();
+  std::vector<void*> tmp;
+  auto talloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    tmp.push_back(*p);
+    return true;
+  };
+  auto cleanup = [&]() {
+    for (void* p : tmp) hipFree(p);
+  };
+  int64_t *dboff = nullptr, *dmoff = nullptr;
+  int32_t* dnodes = nullptr;
+  double *dense = nullptr, *val = nullptr;
+  uint64_t* key = nullptr;
+  int* bad = nullptr;
+  if (!talloc((void**)&dboff, boff.size() * 8) || !talloc((void**)&dmoff, moff.size() * 8) ||
+      !talloc((void**)&dnodes, nodes.size() * 4) || !talloc((void**)&dense, ntot * 8) ||
+      !talloc((void**)&val, ntot * 8) || !talloc((void**)&key, ntot * 8) ||
+      !talloc((void**)&bad, 4)) {
+    cleanup();
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid smoother blocks)");
+  }
+  LSAMD_HIP(hipMemcpyAsync(dboff, boff.data(), boff.size() * 8, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipMemcpyAsync(dmoff, moff.data(), moff.size() * 8, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipMemcpyAsync(dnodes, nodes.data(), nodes.size() * 4, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipMemsetAsync(bad, 0, 4, st));
+  lsamd_fd_system* S = L.sys;
+  hipLaunchKernelGGL(k_blk_fill, dim3((unsigned)ceil_div(ntot, 256)), dim3(256), 0, st,
+                     (const int64_t*)dboff, (const int32_t*)dnodes, (const int64_t*)dmoff, nb,
+                     ntot, g, make_regw<double>(S->reg), (const double*)S->scale2,
+                     (const int64_t*)G.ptr, (const int32_t*)G.col, (const double*)G.v64, dense);
+  hipLaunchKernelGGL(k_blk_inv, dim3((unsigned)nb), dim3(64), 0, st,
+                     (const int64_t*)dboff, (const int64_t*)dmoff, nb, dense, bad);
+  hipLaunchKernelGGL(k_blk_pairs, dim3((unsigned)ceil_div(ntot, 256)), dim3(256), 0, st,
+                     (const int64_t*)dboff, (const int32_t*)dnodes, (const int64_t*)dmoff, nb,
+                     ntot, (const double*)dense, key, val);
+  LSAMD_CHECK_LAUNCH("multigrid smoother blocks");
+  int hbad = 0;
+  LSAMD_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  if (hbad) {
+    cleanup();
+    return fail(LSAMD_ERR_ARG, "multigrid: a smoother block is not positive definite");
+  }
+  CsrOut M;
+  if (int rc = pairs_csr(m, key, val, ntot, L.n, &M, st)) {
+    cleanup();
+    return rc;
+  }
+  cleanup();
+  L.sptr = M.ptr;
+  L.scol = M.col;
+  L.sval = M.v32;
+  L.sval64 = M.v64;
+  if (int rc = mg_alloc(m, &L.sm, L.n)) return rc;
+  hipLaunchKernelGGL(k_blk_tags, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
+                     (const int64_t*)L.sptr, (const float*)S->minv32, L.sm, L.n);
+  LSAMD_CHECK_LAUNCH("multigrid smoother tags");
+  return LSAMD_OK;
+}
+
+// cells (min-corner node ids) of the level-0 data rows: every run of 8
+// entries of a row whose nodes lie within one cell gives that cell
+int mg_data_cells(lsamd_fd_system* s, std::vector<int64_t>& cells, int* wmax, hipStream_t st) {
+  cells.clear();
+  *wmax = 0;
+  if (s->nrows == 0) return LSAMD_OK;
+  const GridDev g = s->gd;
+  std::vector<int64_t> rp(s->nrows + 1);
+  std::vector<int32_t> cols(s->nnz);
+  LSAMD_HIP(hipMemcpyAsync(rp.data(), s->row_ptr, rp.size() * 8, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipMemcpyAsync(cols.data(), s->cols, cols.size() * 4, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  for (int64_t r = 0; r < s->nrows; ++r) *wmax = std::max<int>(*wmax, (int)(rp[r + 1] - rp[r]));
+  if (g.nx < 2 || g.ny < 2 || g.nz < 2) return LSAMD_OK;
+  const int64_t nx = g.nx, nxy = (int64_t)g.nx * g.ny;
+  for (int64_t r = 0; r < s->nrows; ++r)
+    for (int64_t e0 = rp[r]; e0 < rp[r + 1]; e0 += 8) {
+      const int64_t e1 = std::min<int64_t>(e0 + 8, rp[r + 1]);
+      int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+      for (int64_t e = e0; e < e1; ++e) {
+        const int64_t id = cols[e];
+        const int c[3] = {(int)(id % nx), (int)((id / nx) % g.ny), (int)(id / nxy)};
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], c[a]);
+          hi[a] = std::max(hi[a], c[a]);
+        }
+      }
+      if (hi[0] - lo[0] > 1 || hi[1] - lo[1] > 1 || hi[2] - lo[2] > 1) continue;
+      const int ci = std::min(lo[0], g.nx - 2), cj = std::min(lo[1], g.ny - 2),
+                ck = std::min(lo[2], g.nz - 2);
+      cells.push_back(ci + nx * cj + nxy * ck);
+    }
+  std::sort(cells.begin(), cells.end());
+  cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+  return LSAMD_OK;
+}
+
+// cells of the next coarser level: each fine cell lies in coarse cell c/2
+void mg_coarse_cells(const GridDev& gf, const GridDev& gc, std::vector<int64_t>& cells) {
+  const int64_t fnx = gf.nx, fnxy = (int64_t)gf.nx * gf.ny;
+  const int64_t cnx = gc.nx, cnxy = (int64_t)gc.nx * gc.ny;
+  for (int64_t& c : cells) {
+    const int i = (int)(c % fnx), j = (int)((c / fnx) % gf.ny), k = (int)(c / fnxy);
+    const int ci = std::min(i >> 1, gc.nx - 2), cj = std::min(j >> 1, gc.ny - 2),
+              ck = std::min(k >> 1, gc.nz - 2);
+    c = ci + cnx * cj + cnxy * ck;
+  }
+  std::sort(cells.begin(), cells.end());
+  cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+}
 int mg_level_vectors(Mg* m, MgLevel& L) {
   if (int rc = mg_alloc(m, &L.b, L.n)) return rc;
   if (int rc = mg_alloc(m, &L.x, L.n)) return rc;
@@ -1467,15 +1682,64 @@ int mg_level_vectors(Mg* m, MgLevel& L) {
   return LSAMD_OK;
 }
 
+// LSAMD_MG_TIMING=1: per-phase wall time of the multigrid setup on stderr
+struct MgTimer {
+  hipStream_t st;
+  bool on;
+  double t0 = 0.0;
+  MgTimer(hipStream_t s) : st(s), on(getenv("LSAMD_MG_TIMING") != nullptr) { tick(nullptr); }
+  void tick(const char* what) {
+    if (!on) return;
+    hipStreamSynchronize(st);
+    const double t =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+            .count();
+    if (what) fprintf(stderr, "[multigrid setup] %-24s %8.2f ms\n", what, t - t0);
+    t0 = t;
+  }
+};
+
 int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
+  MgTimer tm(st);
   if (int rc = mg_prepare_level(s, st)) return rc;
+  tm.tick("prepare level 0");
   if (int rc = mg_alloc(m, &m->scal, 2)) return rc;
+  const char* sm_env = getenv("LSAMD_MG_SMOOTHER");
+  const bool schwarz = !(sm_env && strcmp(sm_env, "jacobi") == 0);
+  if (const char* dg = getenv("LSAMD_MG_DEGREE")) m->deg = std::max(1, std::min(kMgDegMax, atoi(dg)));
   MgLevel L0;
   L0.sys = s;
   L0.g = s->gd;
   L0.n = s->n;
   if (int rc = mg_level_vectors(m, L0)) return rc;
   m->lv.push_back(L0);
+  // Schwarz blocks of level 0: data cells from the rows, entries of the data
+  // term from a Gram matrix of the rows built for this purpose only
+  std::vector<std::vector<int64_t>> cells(1);
+  if (schwarz) {
+    int wmax = 0;
+    if (int rc = mg_data_cells(s, cells[0], &wmax, st)) return rc;
+    CsrOut G0;
+    if (s->nrows > 0 && wmax > 0) {
+      int32_t* fc = nullptr;
+      double* fcoef = nullptr;
+      if (int rc = mg_alloc(m, &fc, s->nrows * wmax)) return rc;
+      if (int rc = mg_alloc(m, &fcoef, s->nrows * wmax)) return rc;
+      hipLaunchKernelGGL(k_rows_fixed, dim3((unsigned)ceil_div(s->nrows, 128)), dim3(128), 0, st,
+                         (const int64_t*)s->row_ptr, (const int32_t*)s->cols,
+                         (const double*)s->coef64, s->nrows, wmax, fc, fcoef);
+      LSAMD_CHECK_LAUNCH("multigrid fixed-width rows");
+      const int rc = gram_csr(m, fc, fcoef, s->w2_64, s->nrows, wmax, s->n, &G0, st);
+      mg_free(m, fc);
+      mg_free(m, fcoef);
+      if (rc) return rc;
+    }
+    tm.tick("level-0 cells + gram");
+    const int rc = mg_schwarz(m, m->lv[0], cells[0], G0, st);
+    tm.tick("level-0 schwarz");
+    for (void* p : {(void*)G0.ptr, (void*)G0.col, (void*)G0.v32, (void*)G0.v64}) mg_free(m, p);
+    if (rc) return rc;
+  }
   lsamd_grid gprev = s->grid;
   GridDev gdprev = s->gd;
   // data rows of the previous level: CSR (level 0: the system's rows)
@@ -1529,14 +1793,32 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
       hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, prow,
                          pcols, pcoef, nr, gdprev, gcd, 0, wc, (int32_t*)nullptr, ccols, ccoef);
       LSAMD_CHECK_LAUNCH("multigrid rows");
+      tm.tick("coarse level + rows");
       if (int rc = mg_gram(m, m->lv.back(), ccols, ccoef, w2, nr, wc, st)) return rc;
+      tm.tick("coarse gram");
       prow = rptr;
       pcols = ccols;
       pcoef = ccoef;
     }
+    if (schwarz) {
+      cells.push_back(cells.back());
+      mg_coarse_cells(gdprev, cs->gd, cells.back());
+    }
     gprev = gc;
     gdprev = cs->gd;
   }
+  // Schwarz blocks of the intermediate levels (the coarsest is solved densely)
+  if (schwarz)
+    for (size_t l = 1; l + 1 < m->lv.size(); ++l) {
+      MgLevel& L = m->lv[l];
+      CsrOut G;
+      G.ptr = L.gptr;
+      G.col = L.gcol;
+      G.v32 = L.gval;
+      G.v64 = L.gval64;
+      if (int rc = mg_schwarz(m, L, cells[l], G, st)) return rc;
+      tm.tick("coarse schwarz");
+    }
   // coarsest level: dense inverse of its FP64 operator
   MgLevel& C = m->lv.back();
   const int nc = (int)C.n;
@@ -1550,6 +1832,7 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
     LSAMD_HIP(hipMemcpyAsync(&a[(size_t)j * nc], C.sys->q64, nc * 8, hipMemcpyDeviceToHost, st));
   }
   LSAMD_HIP(hipStreamSynchronize(st));
+  tm.tick("coarsest matrix");
   if (!mg_invert(a, nc, inv)) return fail(LSAMD_ERR_ARG, "multigrid: singular coarsest operator");
   std::vector<float> invf(inv.begin(), inv.end());
   if (int rc = mg_alloc(m, &m->cinv, (int64_t)nc * nc)) return rc;
@@ -1558,12 +1841,13 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
     MgLevel& L = m->lv[l];
     if (int rc = mg_lambda(m, L, &L.lambda, st)) return rc;
+    tm.tick("lambda");
     const double lmax = 1.15 * L.lambda, lmin = L.lambda / 30.0;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
     L.c0 = (float)(1.0 / theta);
-    for (int k = 1; k < kMgDeg; ++k) {
+    for (int k = 1; k < m->deg; ++k) {
       const double rn = 1.0 / (2.0 * sigma - rho);
       L.c1[k] = (float)(rn * rho);
       L.c2[k] = (float)(2.0 * rn / delta);
@@ -1586,14 +1870,20 @@ void mg_vcycle(Mg* m, int l, const CgCtl* ctl, hipStream_t st) {
   }
   lsamd_fd_system* S = L.sys;
   const dim3 nb((unsigned)ceil_div(L.n, 256));
-  hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b, (const float*)L.q,
-                     (const float*)S->minv32, L.x, L.d, L.n, L.c0, 1, ctl);
-  for (int k = 1; k < kMgDeg; ++k) {
-    mg_level_apply(L, L.x, L.q, ctl, st);
-    hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
-                       (const float*)L.q, (const float*)S->minv32, L.x, L.d, L.n, L.c1[k],
-                       L.c2[k], ctl);
-  }
+  const float* sm = L.sm ? L.sm : S->minv32;
+  const int64_t* sp = L.sptr;
+  const int32_t* sc = L.scol;
+  const float* sv = L.sval;
+  auto smooth = [&](int from_zero) {
+    hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b,
+                       (const float*)L.q, sm, sp, sc, sv, L.x, L.d, L.n, L.c0, from_zero, ctl);
+    for (int k = 1; k < m->deg; ++k) {
+      mg_level_apply(L, L.x, L.q, ctl, st);
+      hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
+                         (const float*)L.q, sm, sp, sc, sv, L.x, L.d, L.n, L.c1[k], L.c2[k], ctl);
+    }
+  };
+  smooth(1);
   mg_level_apply(L, L.x, L.q, ctl, st);
   MgLevel& C = m->lv[l + 1];
   hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)ceil_div(C.n, 256)), dim3(256), 0, st,
@@ -1601,14 +1891,7 @@ void mg_vcycle(Mg* m, int l, const CgCtl* ctl, hipStream_t st) {
   mg_vcycle(m, l + 1, ctl, st);
   hipLaunchKernelGGL(k_mg_prolong, nb, dim3(256), 0, st, (const float*)C.x, C.g, L.x, L.g, ctl);
   mg_level_apply(L, L.x, L.q, ctl, st);
-  hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b, (const float*)L.q,
-                     (const float*)S->minv32, L.x, L.d, L.n, L.c0, 0, ctl);
-  for (int k = 1; k < kMgDeg; ++k) {
-    mg_level_apply(L, L.x, L.q, ctl, st);
-    hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
-                       (const float*)L.q, (const float*)S->minv32, L.x, L.d, L.n, L.c1[k],
-                       L.c2[k], ctl);
-  }
+  smooth(0);
 }
 
 int launch_fcg_iteration(lsamd_fd_system* s, hipStream_t st) {

@@ -41,10 +41,10 @@ inline RegW<T> make_regw(const lsamd_fd_reg& r) {
   return w;
 }
 
-__device__ inline bool in_grid(const GridDev& g, int i, int j, int k) {
+__host__ __device__ inline bool in_grid(const GridDev& g, int i, int j, int k) {
   return i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
 }
-__device__ inline int64_t nid(const GridDev& g, int i, int j, int k) {
+__host__ __device__ inline int64_t nid(const GridDev& g, int i, int j, int k) {
   return (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
 }
 
@@ -53,7 +53,7 @@ enum Fam { F_XX = 0, F_YY, F_ZZ, F_XY, F_XZ, F_YZ };
 // Squared weight of the rows of family f centred at node (i,j,k); 0 when
 // no such row exists (including nodes outside the grid).
 template <typename T>
-__device__ inline T row_weight(const GridDev& g, const RegW<T>& w, const double* scale2, int f,
+__host__ __device__ inline T row_weight(const GridDev& g, const RegW<T>& w, const double* scale2, int f,
                                int i, int j, int k) {
   if (!in_grid(g, i, j, k)) return (T)0;
   const bool I = i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;

@@ -9,15 +9,25 @@
 // re-discretised on the coarse grid with every squared weight scaled by
 // 2^-l (a smooth field's second-difference energy halves per coarsening).
 // The coarsest level (<= 5 nodes per axis) is solved with a dense inverse.
-// Smoother: Chebyshev polynomial of degree kMgDeg in D^-1 N on
-// [lambda_max/30, 1.1 lambda_max], lambda_max by power iteration.  The
-// V-cycle runs in FP32; the outer flexible CG (Polak-Ribiere beta) runs in
-// FP64 with the same FP64 stopping test as the Jacobi form.
+// Smoother: Chebyshev polynomial (degree 2) in M N on
+// [lambda_max/30, 1.15 lambda_max], lambda_max by power iteration, with M
+// additive Schwarz: the exact inverse of N on each block, summed over
+// blocks, point Jacobi on the nodes no block covers.  Blocks are the 8 nodes
+// of every cell that holds a data row, and the boundary layers: per axis the
+// first three and the last three nodes form one group (a short axis of <= 6
+// nodes is one group), every other index is its own group, and each product
+// of groups with at least one boundary group is a block.  Point Jacobi alone
+// leaves the modes that those strong local couplings (data rows of weight
+// ~1 against regulariser rows of ~0.1; border rows of weight dx) do not see:
+// they are oscillatory and of tiny Rayleigh quotient, so neither the
+// smoother nor the coarse grid reduces them.  The V-cycle runs in FP32; the
+// outer flexible CG (Polak-Ribiere beta) runs in FP64 with the same FP64
+// stopping test as the Jacobi form.
 #pragma once
 
 namespace lsamd {
 
-constexpr int kMgDeg = 2;
+constexpr int kMgDegMax = 4;  // Chebyshev degree: 2 unless LSAMD_MG_DEGREE says 1..4
 constexpr int kMgCoarse = 5;  // stop coarsening at <= 5 nodes per axis
 
 // --------------------------------------------------------------- kernels ----
@@ -34,34 +44,68 @@ __global__ void k_mg_to_f64(const float* __restrict__ a, double* __restrict__ b,
   if (i < n) b[i] = (double)a[i];
 }
 
-// first Chebyshev step: d = c0 D^-1 (b - q) (q unused when from_zero), x (+)= d
+// Smoother preconditioner M applied to r = b - q (r = b when from_zero) at
+// node i: sm[i] >= 0 is the Jacobi scale of an uncovered node; sm[i] < 0
+// marks a node covered by a Schwarz block, whose row of M is
+// sval[sptr[i] .. sptr[i+1]) over the columns scol.
+__device__ inline float mg_precond(int64_t i, const float* __restrict__ sm,
+                                   const int64_t* __restrict__ sptr,
+                                   const int32_t* __restrict__ scol,
+                                   const float* __restrict__ sval, const float* __restrict__ b,
+                                   const float* __restrict__ q, int from_zero) {
+  const float t = sm[i];
+  if (t >= 0.f) return t * (from_zero ? b[i] : b[i] - q[i]);
+  // four entries per step: independent loads in flight hide the latency of
+  // the column-indexed gathers
+  float acc = 0.f;
+  int64_t e = sptr[i];
+  const int64_t e1 = sptr[i + 1];
+  for (; e + 4 <= e1; e += 4) {
+    const int32_t j0 = scol[e], j1 = scol[e + 1], j2 = scol[e + 2], j3 = scol[e + 3];
+    const float v0 = sval[e], v1 = sval[e + 1], v2 = sval[e + 2], v3 = sval[e + 3];
+    if (from_zero) {
+      acc += v0 * b[j0] + v1 * b[j1] + v2 * b[j2] + v3 * b[j3];
+    } else {
+      acc += v0 * (b[j0] - q[j0]) + v1 * (b[j1] - q[j1]) + v2 * (b[j2] - q[j2]) +
+             v3 * (b[j3] - q[j3]);
+    }
+  }
+  for (; e < e1; ++e) {
+    const int32_t j = scol[e];
+    acc += sval[e] * (from_zero ? b[j] : b[j] - q[j]);
+  }
+  return acc;
+}
+
+// first Chebyshev step: d = c0 M (b - q) (q unused when from_zero), x (+)= d
 __global__ void k_mg_cheb_first(const float* __restrict__ b, const float* __restrict__ q,
-                                const float* __restrict__ minv, float* __restrict__ x,
-                                float* __restrict__ d, int64_t n, float c0, int from_zero,
-                                const CgCtl* ctl) {
+                                const float* __restrict__ sm, const int64_t* __restrict__ sptr,
+                                const int32_t* __restrict__ scol, const float* __restrict__ sval,
+                                float* __restrict__ x, float* __restrict__ d, int64_t n, float c0,
+                                int from_zero, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float r = from_zero ? b[i] : b[i] - q[i];
-  const float dv = c0 * minv[i] * r;
+  const float dv = c0 * mg_precond(i, sm, sptr, scol, sval, b, q, from_zero);
   d[i] = dv;
   x[i] = from_zero ? dv : x[i] + dv;
 }
 
-// later steps: d = c1 d + c2 D^-1 (b - q), x += d
+// later steps: d = c1 d + c2 M (b - q), x += d
 __global__ void k_mg_cheb_step(const float* __restrict__ b, const float* __restrict__ q,
-                               const float* __restrict__ minv, float* __restrict__ x,
-                               float* __restrict__ d, int64_t n, float c1, float c2,
-                               const CgCtl* ctl) {
+                               const float* __restrict__ sm, const int64_t* __restrict__ sptr,
+                               const int32_t* __restrict__ scol, const float* __restrict__ sval,
+                               float* __restrict__ x, float* __restrict__ d, int64_t n, float c1,
+                               float c2, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float dv = c1 * d[i] + c2 * minv[i] * (b[i] - q[i]);
+  const float dv = c1 * d[i] + c2 * mg_precond(i, sm, sptr, scol, sval, b, q, 0);
   d[i] = dv;
   x[i] += dv;
 }
 
-__device__ inline void mg_ijk(int64_t id, int nx, int ny, int& i, int& j, int& k) {
+__host__ __device__ inline void mg_ijk(int64_t id, int nx, int ny, int& i, int& j, int& k) {
   const int64_t nxy = (int64_t)nx * ny;
   k = (int)(id / nxy);
   const int64_t r = id - (int64_t)k * nxy;
@@ -188,7 +232,13 @@ __global__ void k_gram_spmv(const int64_t* __restrict__ ptr, const int32_t* __re
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T acc = 0;
-  for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) acc += val[e] * x[col[e]];
+  int64_t e = ptr[i];
+  const int64_t e1 = ptr[i + 1];
+  for (; e + 4 <= e1; e += 4) {
+    const int32_t c0 = col[e], c1 = col[e + 1], c2 = col[e + 2], c3 = col[e + 3];
+    acc += val[e] * x[c0] + val[e + 1] * x[c1] + val[e + 2] * x[c2] + val[e + 3] * x[c3];
+  }
+  for (; e < e1; ++e) acc += val[e] * x[col[e]];
   y[i] += acc;
 }
 
@@ -256,16 +306,26 @@ __global__ void k_mg_rows(const int64_t* __restrict__ row_ptr, const int32_t* __
   }
 }
 
-// power iteration step: w = D^-1 y (y = N v), partial |w|^2 and v.w
+// power iteration step: w = M y (y = N v), partial |w|^2 and v.w
 __global__ __launch_bounds__(256) void k_mg_power(const double* __restrict__ y,
                                                   const double* __restrict__ minv,
+                                                  const float* __restrict__ sm,
+                                                  const int64_t* __restrict__ sptr,
+                                                  const int32_t* __restrict__ scol,
+                                                  const double* __restrict__ sval,
                                                   const double* __restrict__ v,
                                                   double* __restrict__ w, int64_t n,
                                                   double* __restrict__ part) {
   __shared__ double sh[4];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const double wv = minv[i] * y[i];
+    double wv;
+    if (sm[i] >= 0.f) {
+      wv = minv[i] * y[i];
+    } else {
+      wv = 0.0;
+      for (int64_t e = sptr[i]; e < sptr[i + 1]; ++e) wv += sval[e] * y[scol[e]];
+    }
     w[i] = wv;
     a += wv * wv;
     b += v[i] * wv;
@@ -401,6 +461,197 @@ __global__ __launch_bounds__(256) void k_fcg_upd_fin(const double* __restrict__ 
     const int conv = sqrt(tot) < ctl->thr;
     ctl->converged = conv;
     ctl->done = conv || it >= ctl->maxiter;
+  }
+}
+
+
+// ------------------------------------------------ Schwarz smoother blocks ----
+// Entry N(a, b) of a level's operator: regulariser rows (from their
+// definition, ridge included) plus the data Gram matrix G (CSR, columns
+// sorted per row; may be absent).
+__host__ __device__ inline double reg_entry(const GridDev& g, const RegW<double>& w, const double* scale2,
+                                   int64_t ia, int64_t ib) {
+  int ai, aj, ak, bi, bj, bk;
+  mg_ijk(ia, g.nx, g.ny, ai, aj, ak);
+  mg_ijk(ib, g.nx, g.ny, bi, bj, bk);
+  double v = ia == ib ? w.ridge : 0.0;
+  // axial masks [1, -2, 1] / sqrt6: product of two taps / 6
+  auto m = [](int o) { return o == 0 ? -2.0 : 1.0; };
+  if (aj == bj && ak == bk)
+    for (int c = (ai > bi ? ai : bi) - 1; c <= (ai < bi ? ai : bi) + 1; ++c)
+      v += row_weight(g, w, scale2, F_XX, c, aj, ak) * m(ai - c) * m(bi - c) / 6.0;
+  if (ai == bi && ak == bk)
+    for (int c = (aj > bj ? aj : bj) - 1; c <= (aj < bj ? aj : bj) + 1; ++c)
+      v += row_weight(g, w, scale2, F_YY, ai, c, ak) * m(aj - c) * m(bj - c) / 6.0;
+  if (ai == bi && aj == bj)
+    for (int c = (ak > bk ? ak : bk) - 1; c <= (ak < bk ? ak : bk) + 1; ++c)
+      v += row_weight(g, w, scale2, F_ZZ, ai, aj, c) * m(ak - c) * m(bk - c) / 6.0;
+  // cross masks: tap -1/2 * du * dv at (du, dv) in {-1, 1}^2
+  for (int du = -1; du <= 1; du += 2)
+    for (int dv = -1; dv <= 1; dv += 2) {
+      // a sits at centre + (du, dv); b at centre + (eu, ev)
+      if (ak == bk) {
+        const int cx = ai - du, cy = aj - dv, eu = bi - cx, ev = bj - cy;
+        if ((eu == 1 || eu == -1) && (ev == 1 || ev == -1))
+          v += row_weight(g, w, scale2, F_XY, cx, cy, ak) * 0.25 * du * dv * eu * ev;
+      }
+      if (aj == bj) {
+        const int cx = ai - du, cz = ak - dv, eu = bi - cx, ev = bk - cz;
+        if ((eu == 1 || eu == -1) && (ev == 1 || ev == -1))
+          v += row_weight(g, w, scale2, F_XZ, cx, aj, cz) * 0.25 * du * dv * eu * ev;
+      }
+      if (ai == bi) {
+        const int cy = aj - du, cz = ak - dv, eu = bj - cy, ev = bk - cz;
+        if ((eu == 1 || eu == -1) && (ev == 1 || ev == -1))
+          v += row_weight(g, w, scale2, F_YZ, ai, cy, cz) * 0.25 * du * dv * eu * ev;
+      }
+    }
+  return v;
+}
+
+__device__ inline double gram_entry(const int64_t* __restrict__ gptr,
+                                    const int32_t* __restrict__ gcol,
+                                    const double* __restrict__ gval, int64_t a, int64_t b) {
+  if (!gptr) return 0.0;
+  int64_t lo = gptr[a], hi = gptr[a + 1];
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (gcol[mid] < (int32_t)b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < gptr[a + 1] && gcol[lo] == (int32_t)b) ? gval[lo] : 0.0;
+}
+
+// dense N restricted to every block: entry e of block q is (a, c) of the
+// block's s x s matrix, nodes bnodes[boff[q] + a], bnodes[boff[q] + c]
+__global__ void k_blk_fill(const int64_t* __restrict__ boff, const int32_t* __restrict__ bnodes,
+                           const int64_t* __restrict__ moff, int nb, int64_t ntot, GridDev g,
+                           RegW<double> w, const double* scale2, const int64_t* gptr,
+                           const int32_t* gcol, const double* gval, double* __restrict__ dense) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntot) return;
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {  // last block with moff <= e
+    const int mid = (lo + hi + 1) >> 1;
+    if (moff[mid] <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t s = boff[lo + 1] - boff[lo];
+  const int64_t r = e - moff[lo];
+  const int64_t ia = bnodes[boff[lo] + r / s], ib = bnodes[boff[lo] + r % s];
+  dense[e] = reg_entry(g, w, scale2, ia, ib) + gram_entry(gptr, gcol, gval, ia, ib);
+}
+
+// in-place Gauss-Jordan inverse of every (symmetric positive definite)
+// block, one workgroup of 64 per block (the matrix in LDS up to kBlkLds
+// rows, in global memory beyond); *bad set when a pivot is not positive
+constexpr int kBlkLds = 40;
+__global__ __launch_bounds__(64) void k_blk_inv(const int64_t* __restrict__ boff,
+                                                const int64_t* __restrict__ moff, int nb,
+                                                double* __restrict__ dense,
+                                                int* __restrict__ bad) {
+  __shared__ double lds[kBlkLds * kBlkLds];
+  __shared__ double fcol[kBlkLds];
+  const int q = blockIdx.x;
+  if (q >= nb) return;
+  const int s = (int)(boff[q + 1] - boff[q]);
+  double* G = dense + moff[q];
+  const bool in_lds = s <= kBlkLds;
+  double* A = in_lds ? lds : G;
+  const int t = threadIdx.x;
+  if (in_lds)
+    for (int e = t; e < s * s; e += 64) lds[e] = G[e];
+  __syncthreads();
+  for (int k = 0; k < s; ++k) {
+    const double p = A[k * s + k];
+    if (!(p > 0.0)) {
+      if (t == 0) *bad = 1;
+      return;  // uniform: every thread read the same pivot
+    }
+    const double ip = 1.0 / p;
+    __syncthreads();
+    for (int j = t; j < s; j += 64) A[k * s + j] = (j == k) ? ip : A[k * s + j] * ip;
+    for (int i = t; i < s; i += 64)
+      if (i != k && in_lds) fcol[i] = A[i * s + k];
+    __syncthreads();
+    if (in_lds) {
+      for (int e = t; e < s * s; e += 64) {
+        const int i = e / s, j = e - (e / s) * s;
+        if (i == k) continue;
+        const double f = fcol[i];
+        A[e] = (j == k) ? -f * ip : A[e] - f * A[k * s + j];
+      }
+    } else {
+      // global path: one row per thread, the factor read before the row
+      for (int i = t; i < s; i += 64) {
+        if (i == k) continue;
+        const double f = A[i * s + k];
+        for (int j = 0; j < s; ++j)
+          A[i * s + j] = (j == k) ? -f * ip : A[i * s + j] - f * A[k * s + j];
+      }
+    }
+    __syncthreads();
+  }
+  if (in_lds)
+    for (int e = t; e < s * s; e += 64) G[e] = lds[e];
+}
+
+// (row << 32 | col, value) pairs of every block inverse
+__global__ void k_blk_pairs(const int64_t* __restrict__ boff, const int32_t* __restrict__ bnodes,
+                            const int64_t* __restrict__ moff, int nb, int64_t ntot,
+                            const double* __restrict__ dense, uint64_t* __restrict__ key,
+                            double* __restrict__ val) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntot) return;
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (moff[mid] <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t s = boff[lo + 1] - boff[lo];
+  const int64_t r = e - moff[lo];
+  const int32_t ia = bnodes[boff[lo] + r / s], ib = bnodes[boff[lo] + r % s];
+  key[e] = ((uint64_t)(uint32_t)ia << 32) | (uint32_t)ib;
+  val[e] = dense[e];
+}
+
+// smoother tags: -1 on nodes with a Schwarz row, the Jacobi scale elsewhere
+__global__ void k_blk_tags(const int64_t* __restrict__ sptr, const float* __restrict__ minv,
+                           float* __restrict__ sm, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sm[i] = sptr[i + 1] > sptr[i] ? -1.f : minv[i];
+}
+
+// data rows of level 0 in fixed width wc (repeated columns merged, padded
+// with zero coefficients on the first column), for the level's Gram matrix
+__global__ void k_rows_fixed(const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
+                             const double* __restrict__ coef, int64_t nrows, int wc,
+                             int32_t* __restrict__ ccols, double* __restrict__ ccoef) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int nu = 0;
+  const int64_t o = r * wc;
+  for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    int u = 0;
+    while (u < nu && ccols[o + u] != cols[e]) ++u;
+    if (u == nu) {
+      if (nu == wc) continue;  // cannot happen: wc >= row width
+      ccols[o + nu] = cols[e];
+      ccoef[o + nu++] = 0.0;
+    }
+    ccoef[o + u] += coef[e];
+  }
+  for (int u = nu; u < wc; ++u) {
+    ccols[o + u] = nu > 0 ? ccols[o] : 0;
+    ccoef[o + u] = 0.0;
   }
 }
 
