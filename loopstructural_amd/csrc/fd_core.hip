@@ -727,6 +727,7 @@ using namespace lsamd;
 namespace lsamd {
 struct Mg;
 void mg_destroy(Mg* m);
+void mg_invalidate(Mg* m);
 struct Ls;
 void ls_destroy(Ls* l);
 }  // namespace lsamd
@@ -1234,13 +1235,19 @@ struct MgLevel {
 
 struct Mg {
   std::vector<MgLevel> lv;
-  int deg = 2;  // Chebyshev degree
+  int deg = 2;            // Chebyshev degree
+  bool schwarz = true;    // Schwarz blocks in the smoother (else point Jacobi)
+  bool stale = true;      // data rows changed: row-dependent parts to rebuild
   float* cinv = nullptr;  // dense inverse of the coarsest operator
   int nc = 0;
   double* z64 = nullptr;
   double* rp64 = nullptr;
   double* scal = nullptr;  // power-iteration scalars
+  // memory: `mem` lives as long as the hierarchy (grids, level vectors);
+  // `dmem` depends on the data rows and returns to `pool` when they change,
+  // so that a rebuild with rows of the same sizes allocates nothing
   std::vector<void*> mem;
+  std::vector<std::pair<void*, size_t>> dmem, pool;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
 };
@@ -1251,7 +1258,33 @@ void mg_destroy(Mg* m) {
   for (auto& L : m->lv)
     if (L.owned && L.sys) lsamd_fd_system_destroy(L.sys);
   for (void* p : m->mem) hipFree(p);
+  for (auto& b : m->dmem) hipFree(b.first);
+  for (auto& b : m->pool) hipFree(b.first);
   delete m;
+}
+
+// The data rows changed: return their memory to the pool and drop the
+// captured iteration (it holds their addresses).
+void mg_invalidate(Mg* m) {
+  if (m->graph) {
+    hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+  }
+  for (auto& b : m->dmem) m->pool.push_back(b);
+  m->dmem.clear();
+  for (MgLevel& L : m->lv) {
+    L.gptr = nullptr;
+    L.gcol = nullptr;
+    L.gval = nullptr;
+    L.gval64 = nullptr;
+    L.sm = nullptr;
+    L.sptr = nullptr;
+    L.scol = nullptr;
+    L.sval = nullptr;
+    L.sval64 = nullptr;
+    L.nblocks = 0;
+  }
+  m->stale = true;
 }
 
 namespace {
@@ -1264,6 +1297,45 @@ int mg_alloc(Mg* m, P** p, int64_t count) {
   m->mem.push_back(q);
   *p = (P*)q;
   return LSAMD_OK;
+}
+
+// row-dependent memory: the smallest pooled block that fits (and wastes at
+// most half of itself), else a new one
+void* mg_take(Mg* m, size_t bytes) {
+  bytes = std::max<size_t>((bytes + 255) & ~(size_t)255, 256);
+  int best = -1;
+  for (size_t i = 0; i < m->pool.size(); ++i) {
+    const size_t b = m->pool[i].second;
+    if (b >= bytes && b <= 2 * bytes && (best < 0 || b < m->pool[best].second)) best = (int)i;
+  }
+  void* q = nullptr;
+  size_t got = bytes;
+  if (best >= 0) {
+    q = m->pool[best].first;
+    got = m->pool[best].second;
+    m->pool.erase(m->pool.begin() + best);
+  } else if (hipMalloc(&q, bytes) != hipSuccess) {
+    return nullptr;
+  }
+  m->dmem.push_back({q, got});
+  return q;
+}
+template <typename P>
+int mg_dalloc(Mg* m, P** p, int64_t count) {
+  void* q = mg_take(m, (size_t)std::max<int64_t>(count, 1) * sizeof(P));
+  if (!q) return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid)");
+  *p = (P*)q;
+  return LSAMD_OK;
+}
+// return one row-dependent block to the pool (no-op for nullptr)
+void mg_give(Mg* m, const void* p) {
+  if (!p) return;
+  for (size_t i = 0; i < m->dmem.size(); ++i)
+    if (m->dmem[i].first == p) {
+      m->pool.push_back(m->dmem[i]);
+      m->dmem.erase(m->dmem.begin() + i);
+      return;
+    }
 }
 
 // y = N x of a level, without the alpha reduction; `ctl` only gates on done
@@ -1405,12 +1477,13 @@ int pairs_csr(Mg* m, const uint64_t* key, const double* val, int64_t np, int64_t
   if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: too many matrix entries");
   std::vector<void*> tmp;
   auto talloc = [&](void** p, size_t bytes) {
-    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    *p = mg_take(m, bytes);
+    if (!*p) return false;
     tmp.push_back(*p);
     return true;
   };
   auto cleanup = [&]() {
-    for (void* p : tmp) hipFree(p);
+    for (void* p : tmp) mg_give(m, p);
   };
   uint64_t *key_s = nullptr, *ukey = nullptr;
   double *val_s = nullptr, *uval = nullptr;
@@ -1442,10 +1515,10 @@ int pairs_csr(Mg* m, const uint64_t* key, const double* val, int64_t np, int64_t
   LSAMD_HIP(hipMemcpyAsync(&nrun_h, nrun, 4, hipMemcpyDeviceToHost, st));
   LSAMD_HIP(hipStreamSynchronize(st));
   const int64_t m_ = nrun_h;
-  int rc = mg_alloc(m, &out->ptr, n + 1);
-  if (!rc) rc = mg_alloc(m, &out->col, m_);
-  if (!rc) rc = mg_alloc(m, &out->v32, m_);
-  if (!rc) rc = mg_alloc(m, &out->v64, m_);
+  int rc = mg_dalloc(m, &out->ptr, n + 1);
+  if (!rc) rc = mg_dalloc(m, &out->col, m_);
+  if (!rc) rc = mg_dalloc(m, &out->v32, m_);
+  if (!rc) rc = mg_dalloc(m, &out->v64, m_);
   if (rc) {
     cleanup();
     return rc;
@@ -1455,10 +1528,8 @@ int pairs_csr(Mg* m, const uint64_t* key, const double* val, int64_t np, int64_t
   hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)ceil_div(std::max<int64_t>(m_, 1), 256)),
                      dim3(256), 0, st, (const uint64_t*)ukey, (const double*)uval, m_, out->col,
                      out->v32, out->v64);
-  const hipError_t e = hipStreamSynchronize(st);
-  cleanup();
-  if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid CSR assembly failed");
   LSAMD_CHECK_LAUNCH("multigrid CSR");
+  cleanup();  // stream-ordered: the pooled blocks are only reused by later work of st
   return LSAMD_OK;
 }
 
@@ -1469,17 +1540,13 @@ int gram_csr(Mg* m, const int32_t* rcols, const double* rcoef, const double* w2,
   if (np > 2000000000LL) return fail(LSAMD_ERR_ARG, "multigrid: data term too large");
   uint64_t* key = nullptr;
   double* val = nullptr;
-  if (hipMalloc((void**)&key, std::max<int64_t>(np, 1) * 8) != hipSuccess)
-    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
-  if (hipMalloc((void**)&val, std::max<int64_t>(np, 1) * 8) != hipSuccess) {
-    hipFree(key);
-    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram)");
-  }
+  if (int rc = mg_dalloc(m, &key, np)) return rc;
+  if (int rc = mg_dalloc(m, &val, np)) return rc;
   hipLaunchKernelGGL(k_gram_pairs, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, rcols,
                      rcoef, w2, nr, wc, key, val);
   const int rc = pairs_csr(m, key, val, np, n, out, st);
-  hipFree(key);
-  hipFree(val);
+  mg_give(m, key);
+  mg_give(m, val);
   return rc;
 }
 
@@ -1498,16 +1565,6 @@ int mg_gram(Mg* m, MgLevel& L, const int32_t* rcols, const double* rcoef, const 
                      cs->diag, cs->minv64, cs->minv32, L.n);
   LSAMD_CHECK_LAUNCH("multigrid Gram diagonal");
   return LSAMD_OK;
-}
-
-void mg_free(Mg* m, void* p) {
-  if (!p) return;
-  for (size_t i = 0; i < m->mem.size(); ++i)
-    if (m->mem[i] == p) {
-      hipFree(p);
-      m->mem.erase(m->mem.begin() + i);
-      return;
-    }
 }
 
 // Schwarz blocks of one level: boundary-layer blocks of its grid and the
@@ -1565,12 +1622,13 @@ int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOu
   const int64_t ntot = moff.back();
   std::vector<void*> tmp;
   auto talloc = [&](void** p, size_t bytes) {
-    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    *p = mg_take(m, bytes);
+    if (!*p) return false;
     tmp.push_back(*p);
     return true;
   };
   auto cleanup = [&]() {
-    for (void* p : tmp) hipFree(p);
+    for (void* p : tmp) mg_give(m, p);
   };
   int64_t *dboff = nullptr, *dmoff = nullptr;
   int32_t* dnodes = nullptr;
@@ -1616,7 +1674,7 @@ int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOu
   L.scol = M.col;
   L.sval = M.v32;
   L.sval64 = M.v64;
-  if (int rc = mg_alloc(m, &L.sm, L.n)) return rc;
+  if (int rc = mg_dalloc(m, &L.sm, L.n)) return rc;
   hipLaunchKernelGGL(k_blk_tags, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st,
                      (const int64_t*)L.sptr, (const float*)S->minv32, L.sm, L.n);
   LSAMD_CHECK_LAUNCH("multigrid smoother tags");
@@ -1699,13 +1757,14 @@ struct MgTimer {
   }
 };
 
-int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
-  MgTimer tm(st);
+// Row-independent part of the hierarchy: the coarse grids as FD systems
+// (re-discretised regulariser, tuned stencils, work vectors) and the level
+// vectors.  Built once per system.
+int mg_build_static(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   if (int rc = mg_prepare_level(s, st)) return rc;
-  tm.tick("prepare level 0");
   if (int rc = mg_alloc(m, &m->scal, 2)) return rc;
   const char* sm_env = getenv("LSAMD_MG_SMOOTHER");
-  const bool schwarz = !(sm_env && strcmp(sm_env, "jacobi") == 0);
+  m->schwarz = !(sm_env && strcmp(sm_env, "jacobi") == 0);
   if (const char* dg = getenv("LSAMD_MG_DEGREE")) m->deg = std::max(1, std::min(kMgDegMax, atoi(dg)));
   MgLevel L0;
   L0.sys = s;
@@ -1713,41 +1772,7 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   L0.n = s->n;
   if (int rc = mg_level_vectors(m, L0)) return rc;
   m->lv.push_back(L0);
-  // Schwarz blocks of level 0: data cells from the rows, entries of the data
-  // term from a Gram matrix of the rows built for this purpose only
-  std::vector<std::vector<int64_t>> cells(1);
-  if (schwarz) {
-    int wmax = 0;
-    if (int rc = mg_data_cells(s, cells[0], &wmax, st)) return rc;
-    CsrOut G0;
-    if (s->nrows > 0 && wmax > 0) {
-      int32_t* fc = nullptr;
-      double* fcoef = nullptr;
-      if (int rc = mg_alloc(m, &fc, s->nrows * wmax)) return rc;
-      if (int rc = mg_alloc(m, &fcoef, s->nrows * wmax)) return rc;
-      hipLaunchKernelGGL(k_rows_fixed, dim3((unsigned)ceil_div(s->nrows, 128)), dim3(128), 0, st,
-                         (const int64_t*)s->row_ptr, (const int32_t*)s->cols,
-                         (const double*)s->coef64, s->nrows, wmax, fc, fcoef);
-      LSAMD_CHECK_LAUNCH("multigrid fixed-width rows");
-      const int rc = gram_csr(m, fc, fcoef, s->w2_64, s->nrows, wmax, s->n, &G0, st);
-      mg_free(m, fc);
-      mg_free(m, fcoef);
-      if (rc) return rc;
-    }
-    tm.tick("level-0 cells + gram");
-    const int rc = mg_schwarz(m, m->lv[0], cells[0], G0, st);
-    tm.tick("level-0 schwarz");
-    for (void* p : {(void*)G0.ptr, (void*)G0.col, (void*)G0.v32, (void*)G0.v64}) mg_free(m, p);
-    if (rc) return rc;
-  }
   lsamd_grid gprev = s->grid;
-  GridDev gdprev = s->gd;
-  // data rows of the previous level: CSR (level 0: the system's rows)
-  int64_t nr = s->nrows;
-  const int64_t* prow = s->row_ptr;
-  const int32_t* pcols = s->cols;
-  const double* pcoef = s->coef64;
-  const double* w2 = s->w2_64;
   for (int l = 1;; ++l) {
     if (gprev.nnodes[0] <= kMgCoarse && gprev.nnodes[1] <= kMgCoarse &&
         gprev.nnodes[2] <= kMgCoarse)
@@ -1768,45 +1793,158 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
     m->lv.push_back(L);
     if (int rc = mg_level_vectors(m, m->lv.back())) return rc;
     if (int rc = mg_prepare_level(cs, st)) return rc;
+    gprev = gc;
+  }
+  const int64_t nc = m->lv.back().n;
+  if (nc > 4096) return fail(LSAMD_ERR_ARG, "multigrid: coarsest level too large");
+  m->nc = (int)nc;
+  if (int rc = mg_alloc(m, &m->cinv, nc * nc)) return rc;
+  if (int rc = mg_alloc(m, &m->z64, s->n)) return rc;
+  if (int rc = mg_alloc(m, &m->rp64, s->n)) return rc;
+  LSAMD_HIP(hipMemset(m->z64, 0, s->n * 8));
+  LSAMD_HIP(hipMemset(m->rp64, 0, s->n * 8));
+  return LSAMD_OK;
+}
+
+// Dense FP64 operator of the coarsest level from its entries (regulariser
+// rows and Gram matrix), inverted on the host, stored in FP32.
+int mg_coarsest(Mg* m, const CsrOut& G, hipStream_t st) {
+  MgLevel& C = m->lv.back();
+  const int nc = m->nc;
+  std::vector<int64_t> boff = {0, nc}, moff = {0, (int64_t)nc * nc};
+  std::vector<int32_t> nodes(nc);
+  for (int i = 0; i < nc; ++i) nodes[i] = i;
+  int64_t *dboff = nullptr, *dmoff = nullptr;
+  int32_t* dnodes = nullptr;
+  double* dense = nullptr;
+  if (int rc = mg_dalloc(m, &dboff, 2)) return rc;
+  if (int rc = mg_dalloc(m, &dmoff, 2)) return rc;
+  if (int rc = mg_dalloc(m, &dnodes, nc)) return rc;
+  if (int rc = mg_dalloc(m, &dense, (int64_t)nc * nc)) return rc;
+  LSAMD_HIP(hipMemcpyAsync(dboff, boff.data(), 16, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipMemcpyAsync(dmoff, moff.data(), 16, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipMemcpyAsync(dnodes, nodes.data(), nc * 4, hipMemcpyHostToDevice, st));
+  lsamd_fd_system* S = C.sys;
+  hipLaunchKernelGGL(k_blk_fill, dim3((unsigned)ceil_div((int64_t)nc * nc, 256)), dim3(256), 0,
+                     st, (const int64_t*)dboff, (const int32_t*)dnodes, (const int64_t*)dmoff, 1,
+                     (int64_t)nc * nc, C.g, make_regw<double>(S->reg), (const double*)S->scale2,
+                     (const int64_t*)G.ptr, (const int32_t*)G.col, (const double*)G.v64, dense);
+  LSAMD_CHECK_LAUNCH("multigrid coarsest operator");
+  std::vector<double> a((size_t)nc * nc), inv;
+  LSAMD_HIP(hipMemcpyAsync(a.data(), dense, a.size() * 8, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  for (void* p : {(void*)dboff, (void*)dmoff, (void*)dnodes, (void*)dense}) mg_give(m, p);
+  if (!mg_invert(a, nc, inv)) return fail(LSAMD_ERR_ARG, "multigrid: singular coarsest operator");
+  std::vector<float> invf(inv.begin(), inv.end());
+  LSAMD_HIP(hipMemcpyAsync(m->cinv, invf.data(), invf.size() * 4, hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  return LSAMD_OK;
+}
+
+// Row-dependent part: coarse data rows (A P per level) and their Gram
+// matrices, the coarse diagonals, the Schwarz blocks, the coarsest inverse
+// and the smoother polynomials.  Rebuilt after every set_rows.
+int mg_build_rows(lsamd_fd_system* s, Mg* m, hipStream_t st) {
+  MgTimer tm(st);
+  const bool schwarz = m->schwarz;
+  const bool single = m->lv.size() == 1;  // level 0 is the coarsest
+  // level 0 keeps its data term as rows; a Gram matrix of them is built here
+  // for the entries of its Schwarz blocks (or of its dense inverse)
+  std::vector<std::vector<int64_t>> cells(1);
+  CsrOut G0;
+  if (schwarz || single) {
+    int wmax = 0;
+    if (int rc = mg_data_cells(s, cells[0], &wmax, st)) return rc;
+    if (s->nrows > 0 && wmax > 0) {
+      int32_t* fc = nullptr;
+      double* fcoef = nullptr;
+      if (int rc = mg_dalloc(m, &fc, s->nrows * wmax)) return rc;
+      if (int rc = mg_dalloc(m, &fcoef, s->nrows * wmax)) return rc;
+      hipLaunchKernelGGL(k_rows_fixed, dim3((unsigned)ceil_div(s->nrows, 128)), dim3(128), 0, st,
+                         (const int64_t*)s->row_ptr, (const int32_t*)s->cols,
+                         (const double*)s->coef64, s->nrows, wmax, fc, fcoef);
+      LSAMD_CHECK_LAUNCH("multigrid fixed-width rows");
+      const int rc = gram_csr(m, fc, fcoef, s->w2_64, s->nrows, wmax, s->n, &G0, st);
+      mg_give(m, fc);
+      mg_give(m, fcoef);
+      if (rc) return rc;
+    }
+    tm.tick("level-0 cells + gram");
+  }
+  if (schwarz && !single) {
+    if (int rc = mg_schwarz(m, m->lv[0], cells[0], G0, st)) return rc;
+    tm.tick("level-0 schwarz");
+  }
+  if (single) {
+    if (int rc = mg_coarsest(m, G0, st)) return rc;
+  }
+  for (const void* p : {(const void*)G0.ptr, (const void*)G0.col, (const void*)G0.v32,
+                        (const void*)G0.v64})
+    mg_give(m, p);
+  // data rows of the previous level: CSR (level 0: the system's rows)
+  const int64_t nr = s->nrows;
+  const int64_t* prow = s->row_ptr;
+  const int32_t* pcols = s->cols;
+  const double* pcoef = s->coef64;
+  const double* w2 = s->w2_64;
+  int64_t* own_row = nullptr;  // restricted rows of the previous level (pooled)
+  int32_t* own_cols = nullptr;
+  double* own_coef = nullptr;
+  int32_t* cnt = nullptr;
+  if (nr > 0)
+    if (int rc = mg_dalloc(m, &cnt, nr)) return rc;
+  std::vector<int32_t> hc(std::max<int64_t>(nr, 1));
+  for (size_t l = 1; l < m->lv.size(); ++l) {
+    MgLevel& L = m->lv[l];
+    lsamd_fd_system* cs = L.sys;
+    const GridDev gf = m->lv[l - 1].g, gcd = L.g;
+    // regulariser-only diagonal of the coarse system, then its Gram diagonal
+    launch_reg_diag(cs, st);
     if (nr > 0) {
-      int32_t* cnt = nullptr;
-      if (int rc = mg_alloc(m, &cnt, nr)) return rc;
-      const GridDev gcd = cs->gd;
       hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, prow,
-                         pcols, pcoef, nr, gdprev, gcd, 1, 0, cnt, (int32_t*)nullptr,
+                         pcols, pcoef, nr, gf, gcd, 1, 0, cnt, (int32_t*)nullptr,
                          (double*)nullptr);
-      std::vector<int32_t> hc(nr);
       LSAMD_HIP(hipMemcpyAsync(hc.data(), cnt, nr * 4, hipMemcpyDeviceToHost, st));
       LSAMD_HIP(hipStreamSynchronize(st));
       int wc = 1;
-      for (int32_t c : hc) wc = std::max(wc, (int)c);
+      for (int64_t r = 0; r < nr; ++r) wc = std::max(wc, (int)hc[r]);
       if (wc > 64) return fail(LSAMD_ERR_ARG, "multigrid: a data row spans too many cells");
       int32_t* ccols = nullptr;
       double* ccoef = nullptr;
       int64_t* rptr = nullptr;
-      if (int rc = mg_alloc(m, &ccols, nr * wc)) return rc;
-      if (int rc = mg_alloc(m, &ccoef, nr * wc)) return rc;
-      if (int rc = mg_alloc(m, &rptr, nr + 1)) return rc;
-      std::vector<int64_t> hr(nr + 1);
-      for (int64_t r = 0; r <= nr; ++r) hr[r] = r * wc;
-      LSAMD_HIP(hipMemcpyAsync(rptr, hr.data(), (nr + 1) * 8, hipMemcpyHostToDevice, st));
+      if (int rc = mg_dalloc(m, &ccols, nr * wc)) return rc;
+      if (int rc = mg_dalloc(m, &ccoef, nr * wc)) return rc;
+      if (int rc = mg_dalloc(m, &rptr, nr + 1)) return rc;
+      hipLaunchKernelGGL(k_iota_rows, dim3((unsigned)ceil_div(nr + 1, 256)), dim3(256), 0, st,
+                         rptr, nr + 1, (int64_t)wc);
       hipLaunchKernelGGL(k_mg_rows, dim3((unsigned)ceil_div(nr, 128)), dim3(128), 0, st, prow,
-                         pcols, pcoef, nr, gdprev, gcd, 0, wc, (int32_t*)nullptr, ccols, ccoef);
+                         pcols, pcoef, nr, gf, gcd, 0, wc, (int32_t*)nullptr, ccols, ccoef);
       LSAMD_CHECK_LAUNCH("multigrid rows");
-      tm.tick("coarse level + rows");
-      if (int rc = mg_gram(m, m->lv.back(), ccols, ccoef, w2, nr, wc, st)) return rc;
+      mg_give(m, own_row);
+      mg_give(m, own_cols);
+      mg_give(m, own_coef);
+      own_row = rptr;
+      own_cols = ccols;
+      own_coef = ccoef;
+      tm.tick("coarse rows");
+      if (int rc = mg_gram(m, L, ccols, ccoef, w2, nr, wc, st)) return rc;
       tm.tick("coarse gram");
       prow = rptr;
       pcols = ccols;
       pcoef = ccoef;
+    } else {
+      hipLaunchKernelGGL(k_minv, dim3((unsigned)ceil_div(L.n, 256)), dim3(256), 0, st, cs->diag,
+                         cs->minv64, cs->minv32, L.n);
     }
     if (schwarz) {
       cells.push_back(cells.back());
-      mg_coarse_cells(gdprev, cs->gd, cells.back());
+      mg_coarse_cells(gf, gcd, cells.back());
     }
-    gprev = gc;
-    gdprev = cs->gd;
   }
+  mg_give(m, own_row);
+  mg_give(m, own_cols);
+  mg_give(m, own_coef);
+  mg_give(m, cnt);
   // Schwarz blocks of the intermediate levels (the coarsest is solved densely)
   if (schwarz)
     for (size_t l = 1; l + 1 < m->lv.size(); ++l) {
@@ -1819,24 +1957,16 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
       if (int rc = mg_schwarz(m, L, cells[l], G, st)) return rc;
       tm.tick("coarse schwarz");
     }
-  // coarsest level: dense inverse of its FP64 operator
-  MgLevel& C = m->lv.back();
-  const int nc = (int)C.n;
-  if (nc > 4096) return fail(LSAMD_ERR_ARG, "multigrid: coarsest level too large");
-  m->nc = nc;
-  std::vector<double> a((size_t)nc * nc), inv;
-  for (int j = 0; j < nc; ++j) {
-    hipLaunchKernelGGL(k_mg_unit, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, st,
-                       C.sys->p64, (int64_t)nc, (int64_t)j);
-    if (int rc = mg_level_apply64(C, C.sys->p64, C.sys->q64, st)) return rc;
-    LSAMD_HIP(hipMemcpyAsync(&a[(size_t)j * nc], C.sys->q64, nc * 8, hipMemcpyDeviceToHost, st));
+  if (!single) {
+    MgLevel& C = m->lv.back();
+    CsrOut G;
+    G.ptr = C.gptr;
+    G.col = C.gcol;
+    G.v32 = C.gval;
+    G.v64 = C.gval64;
+    if (int rc = mg_coarsest(m, G, st)) return rc;
   }
-  LSAMD_HIP(hipStreamSynchronize(st));
-  tm.tick("coarsest matrix");
-  if (!mg_invert(a, nc, inv)) return fail(LSAMD_ERR_ARG, "multigrid: singular coarsest operator");
-  std::vector<float> invf(inv.begin(), inv.end());
-  if (int rc = mg_alloc(m, &m->cinv, (int64_t)nc * nc)) return rc;
-  LSAMD_HIP(hipMemcpy(m->cinv, invf.data(), invf.size() * 4, hipMemcpyHostToDevice));
+  tm.tick("coarsest inverse");
   // smoother coefficients of the other levels
   for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
     MgLevel& L = m->lv[l];
@@ -1854,10 +1984,7 @@ int mg_build(lsamd_fd_system* s, Mg* m, hipStream_t st) {
       rho = rn;
     }
   }
-  if (int rc = mg_alloc(m, &m->z64, s->n)) return rc;
-  if (int rc = mg_alloc(m, &m->rp64, s->n)) return rc;
-  LSAMD_HIP(hipMemset(m->z64, 0, s->n * 8));
-  LSAMD_HIP(hipMemset(m->rp64, 0, s->n * 8));
+  m->stale = false;
   return LSAMD_OK;
 }
 
@@ -1960,11 +2087,18 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
   if (s->op != 0) return fail(LSAMD_ERR_ARG, "multigrid is available for FD systems only");
   if (!s->mg) {
     Mg* m = new Mg();
-    if (int rc = mg_build(s, m, st)) {
+    if (int rc = mg_build_static(s, m, st)) {
       mg_destroy(m);
       return rc;
     }
     s->mg = m;
+  }
+  if (s->mg->stale) {
+    if (int rc = mg_build_rows(s, s->mg, st)) {
+      mg_destroy(s->mg);
+      s->mg = nullptr;
+      return rc;
+    }
   }
   const int64_t n = s->n;
   const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * n;
@@ -2292,10 +2426,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     return fail(LSAMD_ERR_ARG, "lsamd_fd_system_set_rows: NULL row array");
   hipStream_t st = (hipStream_t)stream;
   LSAMD_HIP(hipStreamSynchronize(st));
-  if (s->mg) {
-    mg_destroy(s->mg);
-    s->mg = nullptr;
-  }
+  if (s->mg) mg_invalidate(s->mg);
   if (s->ls) {
     ls_destroy(s->ls);
     s->ls = nullptr;

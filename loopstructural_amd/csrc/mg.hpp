@@ -369,9 +369,10 @@ __global__ void k_mg_start(double* __restrict__ v, int64_t n) {
   v[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-__global__ void k_mg_unit(double* v, int64_t n, int64_t j) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = i == j ? 1.0 : 0.0;
+// rptr[r] = r * wc: row pointers of fixed-width rows
+__global__ void k_iota_rows(int64_t* __restrict__ rptr, int64_t n, int64_t wc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) rptr[r] = r * wc;
 }
 
 // ------------------------------------------------------ flexible PCG ----

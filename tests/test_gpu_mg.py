@@ -18,15 +18,10 @@ W = {"dxx": 0.1, "dyy": 0.1, "dzz": 0.1, "dxy": 0.1, "dxz": 0.1, "dyz": 0.1,
      "dx": 1.0, "dy": 1.0, "dz": 1.0}
 
 
-def _system(cells, seed, nv=300, nn=60):
+def _rows(g, n, seed, nv, nn):
     from loopstructural_amd import _lib, synthetic
     from loopstructural_amd import fd_system as fs
 
-    cells = np.array(cells)
-    step = 1.0 / cells
-    nn_ = cells + 1
-    g = _lib.make_grid((0, 0, 0), step, nn_)
-    n = int(np.prod(nn_))
     values, normals = synthetic.constraints(nv, nn, seed, value_noise=0.02)
     blocks = []
     for pts, kind, wcol, ws in ((normals, _lib.ROWS_NORMAL, 6, 10.0 / 3),
@@ -34,8 +29,21 @@ def _system(cells, seed, nv=300, nn=60):
         p = torch.from_numpy(pts).cuda()
         _, ins, _, _ = fs.locate(g, p[:, :3].contiguous(), want_w=False)
         blocks.append(fs.build_rows(g, kind, p[ins].contiguous(), wcol, ws, n))
+    return fs.concat_rows(blocks, n), values, normals
+
+
+def _system(cells, seed, nv=300, nn=60):
+    from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
+
+    cells = np.array(cells)
+    step = 1.0 / cells
+    nn_ = cells + 1
+    g = _lib.make_grid((0, 0, 0), step, nn_)
+    n = int(np.prod(nn_))
+    rows, values, normals = _rows(g, n, seed, nv, nn)
     s = fs.FdSystem(g, fs.reg_struct(W, nn_, 1e-8))
-    s.set_rows(fs.concat_rows(blocks, n))
+    s.set_rows(rows)
     og = ofd.Grid((0, 0, 0), cells, step)
     sysm = ofd.assemble_fd(og, {"value": values.copy(), "normal": normals.copy()},
                            regularisation=0.1)
@@ -56,6 +64,9 @@ def test_multigrid_solve_matches_minimiser(cells):
 
 
 def test_multigrid_rebuilds_after_new_rows_and_rhs():
+    """The hierarchy outlives set_rows: its row-dependent parts are rebuilt
+    (twice here, with rows of another size in between) and the solution is
+    the minimiser of the new rows, equal to a fresh system's."""
     s, sysm = _system((20, 20, 20), 6)
     c1, _ = s.solve(rtol=1e-12, multigrid=True)
     rhs = torch.from_numpy(np.random.default_rng(3).standard_normal(s.n)).cuda()
@@ -67,6 +78,13 @@ def test_multigrid_rebuilds_after_new_rows_and_rhs():
     c3, _ = s2.solve(rtol=1e-12, multigrid=True)
     ref = ofd.solve_exact(sysm2)
     assert np.linalg.norm(c3.cpu().numpy() - ref) <= 1e-8 * np.linalg.norm(ref)
+    # new rows on the first system: seed 8 (other sizes), then seed 7 again
+    g, n = s.grid, s.n
+    s.set_rows(_rows(g, n, 8, 500, 20)[0])
+    s.solve(rtol=1e-12, multigrid=True)
+    s.set_rows(_rows(g, n, 7, 300, 60)[0])
+    c4, _ = s.solve(rtol=1e-12, multigrid=True)
+    assert np.linalg.norm(c4.cpu().numpy() - ref) <= 1e-8 * np.linalg.norm(ref)
 
 
 def test_interpolator_multigrid_option():
