@@ -37,72 +37,81 @@ __global__ void k_mg_to_f32(const double* __restrict__ a, float* __restrict__ b,
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
 }
-__global__ void k_mg_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n,
-                            const CgCtl* ctl) {
-  if (ctl->done) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (double)a[i];
+// Smoother preconditioner M applied to r = b - q (r = b when from_zero) at
+// node i.  Tag sm[i]: >= 0, the Jacobi scale of an uncovered node; -1, a
+// node covered by Schwarz blocks, whose row of summed block inverses is
+// sval[sptr[i] .. sptr[i+1]) over the columns scol.
+struct MgSm {
+  const float* sm;
+  const int64_t* sptr;
+  const int32_t* scol;
+  const float* sval;
+};
+
+__device__ inline float mg_r(const float* __restrict__ b, const float* __restrict__ q,
+                             int64_t j, int from_zero) {
+  return from_zero ? b[j] : b[j] - q[j];
 }
 
-// Smoother preconditioner M applied to r = b - q (r = b when from_zero) at
-// node i: sm[i] >= 0 is the Jacobi scale of an uncovered node; sm[i] < 0
-// marks a node covered by a Schwarz block, whose row of M is
-// sval[sptr[i] .. sptr[i+1]) over the columns scol.
-__device__ inline float mg_precond(int64_t i, const float* __restrict__ sm,
-                                   const int64_t* __restrict__ sptr,
-                                   const int32_t* __restrict__ scol,
-                                   const float* __restrict__ sval, const float* __restrict__ b,
+__device__ inline float mg_precond(int64_t i, const MgSm& S, const float* __restrict__ b,
                                    const float* __restrict__ q, int from_zero) {
-  const float t = sm[i];
-  if (t >= 0.f) return t * (from_zero ? b[i] : b[i] - q[i]);
+  const float t = S.sm[i];
+  if (t >= 0.f) return t * mg_r(b, q, i, from_zero);
   // four entries per step: independent loads in flight hide the latency of
   // the column-indexed gathers
   float acc = 0.f;
-  int64_t e = sptr[i];
-  const int64_t e1 = sptr[i + 1];
+  int64_t e = S.sptr[i];
+  const int64_t e1 = S.sptr[i + 1];
   for (; e + 4 <= e1; e += 4) {
-    const int32_t j0 = scol[e], j1 = scol[e + 1], j2 = scol[e + 2], j3 = scol[e + 3];
-    const float v0 = sval[e], v1 = sval[e + 1], v2 = sval[e + 2], v3 = sval[e + 3];
-    if (from_zero) {
-      acc += v0 * b[j0] + v1 * b[j1] + v2 * b[j2] + v3 * b[j3];
-    } else {
-      acc += v0 * (b[j0] - q[j0]) + v1 * (b[j1] - q[j1]) + v2 * (b[j2] - q[j2]) +
-             v3 * (b[j3] - q[j3]);
-    }
+    const int32_t j0 = S.scol[e], j1 = S.scol[e + 1], j2 = S.scol[e + 2], j3 = S.scol[e + 3];
+    acc += S.sval[e] * mg_r(b, q, j0, from_zero) + S.sval[e + 1] * mg_r(b, q, j1, from_zero) +
+           S.sval[e + 2] * mg_r(b, q, j2, from_zero) + S.sval[e + 3] * mg_r(b, q, j3, from_zero);
   }
-  for (; e < e1; ++e) {
-    const int32_t j = scol[e];
-    acc += sval[e] * (from_zero ? b[j] : b[j] - q[j]);
-  }
+  for (; e < e1; ++e) acc += S.sval[e] * mg_r(b, q, S.scol[e], from_zero);
   return acc;
 }
 
-// first Chebyshev step: d = c0 M (b - q) (q unused when from_zero), x (+)= d
+// skip_cov: leave the covered nodes to k_mg_cheb_cov (large levels, where
+// their gathers would stall every wavefront that holds one)
 __global__ void k_mg_cheb_first(const float* __restrict__ b, const float* __restrict__ q,
-                                const float* __restrict__ sm, const int64_t* __restrict__ sptr,
-                                const int32_t* __restrict__ scol, const float* __restrict__ sval,
-                                float* __restrict__ x, float* __restrict__ d, int64_t n, float c0,
-                                int from_zero, const CgCtl* ctl) {
+                                MgSm S, float* __restrict__ x, float* __restrict__ d, int64_t n,
+                                float c0, int from_zero, int skip_cov, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float dv = c0 * mg_precond(i, sm, sptr, scol, sval, b, q, from_zero);
+  if (skip_cov && S.sm[i] < 0.f) return;
+  const float dv = c0 * mg_precond(i, S, b, q, from_zero);
   d[i] = dv;
   x[i] = from_zero ? dv : x[i] + dv;
 }
 
 // later steps: d = c1 d + c2 M (b - q), x += d
 __global__ void k_mg_cheb_step(const float* __restrict__ b, const float* __restrict__ q,
-                               const float* __restrict__ sm, const int64_t* __restrict__ sptr,
-                               const int32_t* __restrict__ scol, const float* __restrict__ sval,
-                               float* __restrict__ x, float* __restrict__ d, int64_t n, float c1,
-                               float c2, const CgCtl* ctl) {
+                               MgSm S, float* __restrict__ x, float* __restrict__ d, int64_t n,
+                               float c1, float c2, int skip_cov, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float dv = c1 * d[i] + c2 * mg_precond(i, sm, sptr, scol, sval, b, q, 0);
+  if (skip_cov && S.sm[i] < 0.f) return;
+  const float dv = c1 * d[i] + c2 * mg_precond(i, S, b, q, 0);
   d[i] = dv;
   x[i] += dv;
+}
+
+// the covered nodes of a level (compact list): mode 0 first step from zero,
+// 1 first step, 2 later step (d = c1 d + c2 M r)
+__global__ void k_mg_cheb_cov(const int32_t* __restrict__ cov, int64_t ncov,
+                              const float* __restrict__ b, const float* __restrict__ q, MgSm S,
+                              float* __restrict__ x, float* __restrict__ d, int mode, float c0,
+                              float c1, float c2, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncov) return;
+  const int64_t i = cov[t];
+  const float z = mg_precond(i, S, b, q, mode == 0);
+  const float dv = mode == 2 ? c1 * d[i] + c2 * z : c0 * z;
+  d[i] = dv;
+  x[i] = mode == 0 ? dv : x[i] + dv;
 }
 
 __host__ __device__ inline void mg_ijk(int64_t id, int nx, int ny, int& i, int& j, int& k) {
@@ -115,12 +124,11 @@ __host__ __device__ inline void mg_ijk(int64_t id, int nx, int ny, int& i, int& 
 
 // bc = P^T (bf - qf): per coarse node, the fine nodes 2J-1, 2J, 2J+1 per axis
 // with weights 1/2, 1, 1/2 (those inside the fine grid)
-__global__ void k_mg_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
-                              GridDev gf, float* __restrict__ bc, GridDev gc, const CgCtl* ctl) {
-  if (ctl->done) return;
-  const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nc) return;
+// P^T (bf - qf) at coarse node id: the fine nodes 2J-1, 2J, 2J+1 per axis
+// with weights 1/2, 1, 1/2 (those inside the fine grid)
+__device__ inline float mg_restrict_node(const float* __restrict__ bf,
+                                         const float* __restrict__ qf, const GridDev& gf,
+                                         const GridDev& gc, int64_t id) {
   int I, J, K;
   mg_ijk(id, gc.nx, gc.ny, I, J, K);
   const int64_t fnxy = (int64_t)gf.nx * gf.ny;
@@ -142,16 +150,12 @@ __global__ void k_mg_restrict(const float* __restrict__ bf, const float* __restr
       }
     }
   }
-  bc[id] = acc;
+  return acc;
 }
 
-// xf += P xc
-__global__ void k_mg_prolong(const float* __restrict__ xc, GridDev gc, float* __restrict__ xf,
-                             GridDev gf, const CgCtl* ctl) {
-  if (ctl->done) return;
-  const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nf) return;
+// (P xc) at fine node id: trilinear, coarse node c = fine node 2c
+__device__ inline float mg_prolong_node(const float* __restrict__ xc, const GridDev& gc,
+                                        const GridDev& gf, int64_t id) {
   int i, j, k;
   mg_ijk(id, gf.nx, gf.ny, i, j, k);
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
@@ -163,10 +167,26 @@ __global__ void k_mg_prolong(const float* __restrict__ xc, GridDev gc, float* __
     for (int b = 0; b < nj; ++b)
       for (int a = 0; a < ni; ++a)
         acc += xc[(int64_t)(i0 + a) + (int64_t)(j0 + b) * gc.nx + (int64_t)(k0 + c) * cnxy];
-  xf[id] += wi * wj * wk * acc;
+  return wi * wj * wk * acc;
 }
 
-// x = Cinv b, dense (coarsest level): one wave per row
+__global__ void k_mg_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
+                              GridDev gf, float* __restrict__ bc, GridDev gc, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  bc[id] = mg_restrict_node(bf, qf, gf, gc, id);
+}
+__global__ void k_mg_prolong(const float* __restrict__ xc, GridDev gc, float* __restrict__ xf,
+                             GridDev gf, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nf) return;
+  xf[id] += mg_prolong_node(xc, gc, gf, id);
+}
+
 __global__ __launch_bounds__(64) void k_mg_dense(const float* __restrict__ cinv,
                                                  const float* __restrict__ b,
                                                  float* __restrict__ x, int n, const CgCtl* ctl) {
@@ -376,8 +396,8 @@ __global__ void k_iota_rows(int64_t* __restrict__ rptr, int64_t n, int64_t wc) {
 }
 
 // ------------------------------------------------------ flexible PCG ----
-// partials of z.r, z.r_prev, r.(minv r)
-__global__ __launch_bounds__(256) void k_fcg_dots(const double* __restrict__ z,
+// partials of z.r, z.r_prev (z: the FP32 V-cycle output, widened)
+__global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const double* __restrict__ r,
                                                   const double* __restrict__ rp, int64_t n,
                                                   double* __restrict__ part, const CgCtl* ctl) {
@@ -385,8 +405,9 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const double* __restrict__ z,
   __shared__ double sh[4];
   double a = 0.0, b = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    a += z[i] * r[i];
-    b += z[i] * rp[i];
+    const double zi = (double)z[i];
+    a += zi * r[i];
+    b += zi * rp[i];
   }
   const double sa = block_sum(a, sh);
   const double sb = block_sum(b, sh);
@@ -416,21 +437,23 @@ __global__ __launch_bounds__(256) void k_fcg_dots_fin(const double* __restrict__
 }
 
 // p = z + beta p
-__global__ void k_fcg_dir(const double* __restrict__ z, double* __restrict__ p, int64_t n,
+__global__ void k_fcg_dir(const float* __restrict__ z, double* __restrict__ p, int64_t n,
                           const CgCtl* ctl) {
   if (ctl->done) return;
   const double beta = ctl->acc;
   const int first = ctl->iter == 0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-    p[i] = first ? z[i] : z[i] + beta * p[i];
+    p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
 }
 
-// x += alpha p ; r_prev = r ; r -= alpha q ; partials of r.(minv r)
+// x += alpha p ; r_prev = r ; r -= alpha q ; rf = (float) r, the next
+// V-cycle's input ; partials of r.(minv r)
 __global__ __launch_bounds__(256) void k_fcg_upd(double* __restrict__ x, double* __restrict__ r,
                                                  double* __restrict__ rp,
                                                  const double* __restrict__ p,
                                                  const double* __restrict__ q,
-                                                 const double* __restrict__ minv, int64_t n,
+                                                 const double* __restrict__ minv,
+                                                 float* __restrict__ rf, int64_t n,
                                                  double* __restrict__ part, const CgCtl* ctl) {
   if (ctl->done) return;
   __shared__ double sh[4];
@@ -442,6 +465,7 @@ __global__ __launch_bounds__(256) void k_fcg_upd(double* __restrict__ x, double*
     rp[i] = rv;
     const double rn = rv - alpha * q[i];
     r[i] = rn;
+    rf[i] = (float)rn;
     acc += rn * (minv[i] * rn);
   }
   const double bs = block_sum(acc, sh);
@@ -465,6 +489,63 @@ __global__ __launch_bounds__(256) void k_fcg_upd_fin(const double* __restrict__ 
   }
 }
 
+
+// ----------------------------------------------- coarse levels, fused ----
+// Device view of one coarse level's operator: regulariser (class table, or
+// the exact per-node formula when the table does not apply) + data Gram
+// matrix.
+struct MgDev {
+  GridDev g;
+  int64_t n;
+  const float* tab;  // FP32 class table, nullptr: per-node formula with w
+  RegW<float> w;
+  const int64_t* gptr;
+  const int32_t* gcol;
+  const float* gval;
+};
+
+__device__ inline float mg_apply_node(const MgDev& L, const float* __restrict__ x, int64_t id) {
+  int i, j, k;
+  mg_ijk(id, L.g.nx, L.g.ny, i, j, k);
+  float acc;
+  if (L.tab) {
+    const int nx = L.g.nx, ny = L.g.ny, nz = L.g.nz;
+    const int64_t nxy = (int64_t)nx * ny;
+    const float* c = L.tab + ((cls7(k, nz) * 7 + cls7(j, ny)) * 7 + cls7(i, nx)) * kTaps;
+    acc = c[0] * x[id];
+#pragma unroll
+    for (int t = 1; t < kTaps; ++t) {
+      const int dx = kBTap[t][0], dy = kBTap[t][1], dz = kBTap[t][2];
+      const bool ok = (unsigned)(i + dx) < (unsigned)nx && (unsigned)(j + dy) < (unsigned)ny &&
+                      (unsigned)(k + dz) < (unsigned)nz;
+      const float xv = x[ok ? id + dx + (int64_t)nx * dy + nxy * dz : id];
+      acc += c[t] * (ok ? xv : 0.f);
+    }
+  } else {
+    acc = general_node<float>(x, L.g, L.w, nullptr, i, j, k);
+  }
+  if (L.gptr) {
+    int64_t e = L.gptr[id];
+    const int64_t e1 = L.gptr[id + 1];
+    for (; e + 4 <= e1; e += 4) {
+      const int32_t c0 = L.gcol[e], c1 = L.gcol[e + 1], c2 = L.gcol[e + 2], c3 = L.gcol[e + 3];
+      acc += L.gval[e] * x[c0] + L.gval[e + 1] * x[c1] + L.gval[e + 2] * x[c2] +
+             L.gval[e + 3] * x[c3];
+    }
+    for (; e < e1; ++e) acc += L.gval[e] * x[L.gcol[e]];
+  }
+  return acc;
+}
+
+// y = N x of a coarse level in one launch (stencil and Gram row per node)
+__global__ __launch_bounds__(256) void k_mg_apply(MgDev L, const float* __restrict__ x,
+                                                  float* __restrict__ y, const CgCtl* ctl) {
+  if (ctl->done) return;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < L.n) y[id] = mg_apply_node(L, x, id);
+}
+
+constexpr int64_t kSplitMin = 1 << 20;  // levels this large smooth covered nodes apart
 
 // ------------------------------------------------ Schwarz smoother blocks ----
 // Entry N(a, b) of a level's operator: regulariser rows (from their
@@ -621,6 +702,12 @@ __global__ void k_blk_pairs(const int64_t* __restrict__ boff, const int32_t* __r
   const int32_t ia = bnodes[boff[lo] + r / s], ib = bnodes[boff[lo] + r % s];
   key[e] = ((uint64_t)(uint32_t)ia << 32) | (uint32_t)ib;
   val[e] = dense[e];
+}
+
+// flags of the covered nodes (negative tags)
+__global__ void k_blk_flags(const float* __restrict__ sm, uint8_t* __restrict__ flag, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = sm[i] < 0.f ? 1 : 0;
 }
 
 // smoother tags: -1 on nodes with a Schwarz row, the Jacobi scale elsewhere
