@@ -163,6 +163,20 @@ int lsamd_fd_system_diag(lsamd_fd_system* s, double* diag, void* stream);
 int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, const double* rhs,
                           double* c, lsamd_solve_info* info, void* stream);
 
+/* Several right-hand sides of one FD system, multigrid (opts->jacobi must be
+ * 2): rhs and c are (nrhs, nnodes) FP64, one vector after the other; infos
+ * (nrhs) host memory.  Realisations that share the constraint locations and
+ * differ in observed values have one matrix and nrhs right-hand sides; up to
+ * 8 of them are solved per launch sequence (their vectors interleaved per
+ * node), with the single-system hierarchy and stopping test per rhs.  No
+ * reference counterpart (the reference solves realisations one by one). */
+int lsamd_fd_system_solve_multi(lsamd_fd_system* s, const lsamd_solve_opts* opts, int32_t nrhs,
+                                const double* rhs, double* c, lsamd_solve_info* infos,
+                                void* stream);
+/* rhs (nnodes) = A^T W^2 b for per-row values b (nrows) of the handle's
+ * rows (their coefficients and weights). */
+int lsamd_fd_system_rhs_rows(lsamd_fd_system* s, const double* b, double* rhs, void* stream);
+
 /* A13: LSMR on the rectangular least-squares system of an FD handle
  * (replaces solve_with_lsmr, LI/_solver_strategy.py:108-136, i.e.
  * scipy.sparse.linalg.lsmr(A S, b, atol, btol, conlim, maxiter) on the

@@ -84,6 +84,8 @@ SIGNATURES = {
     "lsamd_fd_system_diag": [_P, _P, _P],
     "lsamd_fd_system_solve": [_P, _P, _P, _P, _P, _P],
     "lsamd_fd_system_lsmr": [_P, _P, _P, _P, _P],
+    "lsamd_fd_system_solve_multi": [_P, _P, _I32, _P, _P, _P, _P],
+    "lsamd_fd_system_rhs_rows": [_P, _P, _P, _P],
     "lsamd_fd_evaluate": [_P, _P, _P, _I64, _P, _P, _P],
     "lsamd_rows_adjoint": [_P, _P, _P, _P, _P, _P],
     "lsamd_rows_normalise": [_I64, _I32, _P, _P, _P, _P],

@@ -710,6 +710,7 @@ __global__ void k_check_cols(const int32_t* __restrict__ cols, int64_t nnz, int6
 #include "batch_kernels.hpp"
 #include "lsmr.hpp"
 #include "mg.hpp"
+#include "mrhs.hpp"
 
 // rows of system s may only address its own nodes; one workgroup per system
 namespace lsamd {
@@ -730,11 +731,15 @@ void mg_destroy(Mg* m);
 void mg_invalidate(Mg* m);
 struct Ls;
 void ls_destroy(Ls* l);
+struct MrWork;
+void mr_destroy(MrWork* w);
+void mr_invalidate(MrWork* w);
 }  // namespace lsamd
 
 struct lsamd_fd_system {
   lsamd::Mg* mg = nullptr;  // multigrid hierarchy (built on first MG solve after set_rows)
   lsamd::Ls* ls = nullptr;  // LSMR work (built on the first LSMR solve after set_rows)
+  lsamd::MrWork* mr = nullptr;  // multi-rhs multigrid work (kept across set_rows)
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -2136,9 +2141,8 @@ int run_fcg(lsamd_fd_system* s, int chunk, hipStream_t st) {
 
 }  // namespace
 
-int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, double* c,
-             lsamd_solve_info* inf, hipStream_t st) {
-  if (s->op != 0) return fail(LSAMD_ERR_ARG, "multigrid is available for FD systems only");
+// hierarchy of the system, built or brought up to date with its rows
+int mg_ensure(lsamd_fd_system* s, hipStream_t st) {
   if (!s->mg) {
     Mg* m = new Mg();
     if (int rc = mg_build_static(s, m, st)) {
@@ -2154,6 +2158,13 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
       return rc;
     }
   }
+  return LSAMD_OK;
+}
+
+int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, double* c,
+             lsamd_solve_info* inf, hipStream_t st) {
+  if (s->op != 0) return fail(LSAMD_ERR_ARG, "multigrid is available for FD systems only");
+  if (int rc = mg_ensure(s, st)) return rc;
   const int64_t n = s->n;
   const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * n;
   const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
@@ -2170,6 +2181,346 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
   inf->converged = s->ctl_host->converged;
   inf->residual = s->ctl_host->iter > 0 ? sqrt(s->ctl_host->res2) : sqrt(s->ctl_host->rho);
   inf->threshold = s->ctl_host->thr;
+  return LSAMD_OK;
+}
+
+// ------------------------------------------------- multi-rhs multigrid ----
+struct MrWork {
+  int R = 0;
+  std::vector<float*> b, x, q, d;  // per level, R lanes interleaved
+  double *rhs = nullptr, *x64 = nullptr, *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  float* t32 = nullptr;
+  double* t64 = nullptr;
+  int64_t t_rows = -1;
+  double* part = nullptr;
+  MrCtl* ctl = nullptr;
+  int* all_done = nullptr;
+  MrCtl* host = nullptr;  // pinned: R controls and the all-done flag
+  // level-0 stencil over the R lanes as R systems of the plane-streaming
+  // kernel: system table, work items per precision (FP64, FP32), zero
+  // controls (no lane skipped), dot partials (unused)
+  BSys* sysR = nullptr;
+  int4* tilesR[2] = {nullptr, nullptr};
+  int ntR[2] = {0, 0};
+  int zcR[2] = {0, 0};
+  CgCtl* ctlR = nullptr;
+  double* partR = nullptr;
+  std::vector<void*> mem;
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+};
+
+void mr_invalidate(MrWork* w) {
+  if (w && w->graph) {
+    hipGraphExecDestroy(w->graph);
+    w->graph = nullptr;
+  }
+}
+
+void mr_destroy(MrWork* w) {
+  if (!w) return;
+  mr_invalidate(w);
+  for (void* p : w->mem) hipFree(p);
+  if (w->host) hipHostFree(w->host);
+  delete w;
+}
+
+namespace {
+
+template <typename P>
+int mr_alloc(MrWork* w, P** p, int64_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, (size_t)std::max<int64_t>(count, 1) * sizeof(P)) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multi-rhs multigrid)");
+  w->mem.push_back(q);
+  *p = (P*)q;
+  return LSAMD_OK;
+}
+
+// work of R lanes for the system's hierarchy; row buffers follow the rows
+int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
+  MrWork* w = s->mr;
+  if (w && w->R != R) {
+    mr_destroy(w);
+    w = s->mr = nullptr;
+  }
+  const Mg* m = s->mg;
+  const int64_t n = s->n;
+  if (!w) {
+    w = new MrWork();
+    w->R = R;
+    s->mr = w;
+    for (const MgLevel& L : m->lv) {
+      float *b, *x, *q, *d;
+      for (float** v : {&b, &x, &q, &d}) {
+        if (int rc = mr_alloc(w, v, L.n * R)) return rc;
+        LSAMD_HIP(hipMemset(*v, 0, L.n * R * 4));
+      }
+      w->b.push_back(b);
+      w->x.push_back(x);
+      w->q.push_back(q);
+      w->d.push_back(d);
+    }
+    for (double** v : {&w->rhs, &w->x64, &w->r64, &w->p64, &w->q64}) {
+      if (int rc = mr_alloc(w, v, n * R)) return rc;
+      LSAMD_HIP(hipMemset(*v, 0, n * R * 8));
+    }
+    if (int rc = mr_alloc(w, &w->part, (int64_t)vec_blocks(n) * 2 * R)) return rc;
+    if (int rc = mr_alloc(w, &w->ctl, R)) return rc;
+    if (int rc = mr_alloc(w, &w->all_done, 1)) return rc;
+    if (hipHostMalloc((void**)&w->host, (R + 1) * sizeof(MrCtl), hipHostMallocDefault) !=
+        hipSuccess)
+      return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed (multi-rhs multigrid)");
+    // R-system plan of the level-0 stencil
+    std::vector<BSys> hs(R);
+    memset(hs.data(), 0, R * sizeof(BSys));
+    for (int r = 0; r < R; ++r) {
+      hs[r].g = s->gd;
+      hs[r].n = n;
+      hs[r].off = (int64_t)r * n;
+    }
+    if (int rc = mr_alloc(w, &w->sysR, R)) return rc;
+    LSAMD_HIP(hipMemcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
+    int64_t maxt = 1;
+    for (int p = 0; p < 2; ++p) {
+      const int zc = s->zc[p] == 8 || s->zc[p] == 16 || s->zc[p] == 32 ? s->zc[p] : 16;
+      std::vector<int4> t;
+      for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, zc, t);
+      if (int rc = mr_alloc(w, &w->tilesR[p], (int64_t)t.size())) return rc;
+      LSAMD_HIP(hipMemcpy(w->tilesR[p], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+      w->ntR[p] = (int)t.size();
+      w->zcR[p] = zc;
+      maxt = std::max<int64_t>(maxt, (int64_t)t.size());
+    }
+    if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
+    LSAMD_HIP(hipMemset(w->ctlR, 0, R * sizeof(CgCtl)));
+    if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
+  }
+  if (w->t_rows != s->nrows) {
+    mr_invalidate(w);
+    if (w->t32) {
+      for (void* p : {(void*)w->t32, (void*)w->t64})
+        for (size_t i = 0; i < w->mem.size(); ++i)
+          if (w->mem[i] == p) {
+            hipFree(p);
+            w->mem.erase(w->mem.begin() + i);
+            break;
+          }
+    }
+    if (int rc = mr_alloc(w, &w->t32, std::max<int64_t>(s->nrows, 1) * R)) return rc;
+    if (int rc = mr_alloc(w, &w->t64, std::max<int64_t>(s->nrows, 1) * R)) return rc;
+    w->t_rows = s->nrows;
+  }
+  *out = w;
+  return LSAMD_OK;
+}
+
+// level-0 regulariser over the R lanes with the plane-streaming kernel
+template <typename T>
+void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st) {
+  const int p = sizeof(T) == 4 ? 1 : 0;
+  const T* tab = p ? (const T*)s->tab32 : (const T*)s->tab64;
+  Coef25<T> ku;
+  for (int q = 0; q < kTaps; ++q) ku.k[q] = p ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
+  const dim3 g((unsigned)w->ntR[p]), b(NT);
+  switch (w->zcR[p]) {
+    case 8:
+      hipLaunchKernelGGL((k_b_tiles<T, 8>), g, b, 0, st, x, y, (const BSys*)w->sysR,
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_b_tiles<T, 32>), g, b, 0, st, x, y, (const BSys*)w->sysR,
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+      break;
+    default:
+      hipLaunchKernelGGL((k_b_tiles<T, 16>), g, b, 0, st, x, y, (const BSys*)w->sysR,
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+      break;
+  }
+}
+
+template <int R>
+void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hipStream_t st) {
+  const Mg* m = s->mg;
+  const MgLevel& L = m->lv[l];
+  const int* ad = w->all_done;
+  const dim3 nb((unsigned)ceil_div(L.n, 256), R), b256(256);
+  if (l == 0) {
+    RowsDev Rw = rows_dev(s, true);
+    if (s->nrows > 0)
+      hipLaunchKernelGGL(k_mr_rows_t<float>, dim3((unsigned)ceil_div(s->nrows, 256), R), b256, 0,
+                         st, x, L.n, Rw, w->t32, R, ad);
+    mr_tiles<float>(s, w, x, y, st);
+    if (s->nrows > 0)
+      hipLaunchKernelGGL(k_mr_rows_scatter<float>,
+                         dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0,
+                         st, Rw, (const float*)w->t32, y, L.n, R, ad);
+    return;
+  }
+  if (L.n >= kMrNodeThreads)
+    hipLaunchKernelGGL(k_mr_apply_coarse_all<R>, dim3((unsigned)ceil_div(L.n, 256)), b256, 0, st,
+                       mg_dev(L), x, y, ad);
+  else
+    hipLaunchKernelGGL(k_mr_apply_coarse, nb, b256, 0, st, mg_dev(L), x, y, R, ad);
+}
+
+template <int R>
+void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
+  const Mg* m = s->mg;
+  const MgLevel& L = m->lv[l];
+  const int* ad = w->all_done;
+  const dim3 b256(256);
+  if (l == (int)m->lv.size() - 1) {
+    hipLaunchKernelGGL(k_mr_dense, dim3((unsigned)ceil_div((int64_t)m->nc * R, 256)), b256, 0, st,
+                       (const float*)m->cinv, (const float*)w->b[l], w->x[l], m->nc, R, ad);
+    return;
+  }
+  const dim3 nb((unsigned)ceil_div(L.n, 256), R);
+  const MgSm S = mg_sm(L);
+  const int split = L.cov ? 1 : 0;
+  const dim3 ncb((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)), R);
+  auto step = [&](int mode, float c1, float c2) {
+    hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
+                       w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
+    if (split && L.ncov > 0)
+      hipLaunchKernelGGL(k_mr_cheb_cov, ncb, b256, 0, st, (const int32_t*)L.cov, L.ncov,
+                         (const float*)w->b[l], (const float*)w->q[l], S, w->x[l], w->d[l], L.n,
+                         mode, L.c0, c1, c2, ad);
+  };
+  auto smooth = [&](int from_zero) {
+    step(from_zero ? 0 : 1, 0.f, 0.f);
+    for (int k = 1; k < m->deg; ++k) {
+      mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
+      step(2, L.c1[k], L.c2[k]);
+    }
+  };
+  smooth(1);
+  mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
+  const MgLevel& C = m->lv[l + 1];
+  hipLaunchKernelGGL(k_mr_restrict, dim3((unsigned)ceil_div(C.n, 256), R), b256, 0, st,
+                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, R, ad);
+  mr_vcycle<R>(s, w, l + 1, st);
+  hipLaunchKernelGGL(k_mr_prolong, nb, b256, 0, st, (const float*)w->x[l + 1], C.g, w->x[l], L.g,
+                     R, ad);
+  mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
+  smooth(0);
+}
+
+template <int R>
+void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
+  const int64_t n = s->n;
+  const int vb = vec_blocks(n);
+  const int* ad = w->all_done;
+  const dim3 b256(256), fin(64 * R);
+  mr_vcycle<R>(s, w, 0, st);  // from w->b[0] = (float) r
+  hipLaunchKernelGGL(k_mr_dots<R>, dim3(vb), b256, 0, st, (const float*)w->x[0],
+                     (const double*)w->r64, (const double*)w->q64, n, w->part, ad);
+  hipLaunchKernelGGL(k_mr_dots_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
+                     ad);
+  hipLaunchKernelGGL(k_mr_dir<R>, dim3(vb), b256, 0, st, (const float*)w->x[0], w->p64, n,
+                     (const MrCtl*)w->ctl, ad);
+  RowsDev Rw = rows_dev(s, false);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL(k_mr_rows_t<double>, dim3((unsigned)ceil_div(s->nrows, 256), R), b256, 0,
+                       st, (const double*)w->p64, n, Rw, w->t64, R, ad);
+  mr_tiles<double>(s, w, (const double*)w->p64, w->q64, st);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL(k_mr_rows_scatter<double>,
+                       dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0, st,
+                       Rw, (const double*)w->t64, w->q64, n, R, ad);
+  hipLaunchKernelGGL(k_mr_pq<R>, dim3(vb), b256, 0, st, (const double*)w->p64,
+                     (const double*)w->q64, n, w->part, ad);
+  hipLaunchKernelGGL(k_mr_alpha<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl, ad);
+  hipLaunchKernelGGL(k_mr_upd<R>, dim3(vb), b256, 0, st, w->x64, w->r64, (const double*)w->p64, (const double*)w->q64, (const double*)s->minv64,
+                     w->b[0], n, w->part, (const MrCtl*)w->ctl, ad);
+  hipLaunchKernelGGL(k_mr_upd_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
+                     w->all_done);
+}
+
+template <int R>
+int mr_solve_group(lsamd_fd_system* s, MrWork* w, const lsamd_solve_opts* o, int nr,
+                   const double* rhs, double* c, lsamd_solve_info* infos, hipStream_t st) {
+  const int64_t n = s->n;
+  const int vb = vec_blocks(n);
+  const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * n;
+  const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
+  const int chunk = std::max(1, std::min(o->check_every > 0 ? o->check_every : 8, 8));
+  hipLaunchKernelGGL(k_mr_pack, dim3((unsigned)ceil_div(n * R, 256)), dim3(256), 0, st, rhs, nr,
+                     R, n, w->rhs);
+  hipLaunchKernelGGL(k_mr_init<R>, dim3(vb), dim3(256), 0, st, (const double*)w->rhs,
+                     (const double*)s->minv64, w->x64, w->r64, w->b[0], n, w->part);
+  hipLaunchKernelGGL(k_mr_init_fin<R>, dim3(1), dim3(64 * R), 0, st, (const double*)w->part, vb,
+                     nr, o->rtol, o->atol, maxiter, w->ctl, w->all_done);
+  LSAMD_CHECK_LAUNCH("multi-rhs init");
+  if (w->graph && w->graph_chunk != chunk) mr_invalidate(w);
+  if (!w->graph) {
+    hipStream_t cs;
+    LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    LSAMD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk; ++i) mr_iteration<R>(s, w, cs);
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multi-rhs graph capture failed");
+    e = hipGraphInstantiate(&w->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multi-rhs graph instantiate failed");
+    w->graph_chunk = chunk;
+  }
+  int* hdone = (int*)(w->host + R);
+  for (;;) {
+    LSAMD_HIP(hipMemcpyAsync(hdone, w->all_done, sizeof(int), hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    if (*hdone) break;
+    LSAMD_HIP(hipGraphLaunch(w->graph, st));
+  }
+  LSAMD_HIP(hipMemcpyAsync(c, w->x64, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+  LSAMD_HIP(hipMemcpyAsync(w->host, w->ctl, R * sizeof(MrCtl), hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  for (int r = 0; r < nr; ++r) {
+    const MrCtl& h = w->host[r];
+    infos[r].iterations = h.iter;
+    infos[r].outer = 0;
+    infos[r].converged = h.converged;
+    infos[r].pad = 0;
+    infos[r].residual = sqrt(h.res2);
+    infos[r].threshold = h.thr;
+  }
+  return LSAMD_OK;
+}
+
+}  // namespace
+
+// Several right-hand sides of the system's normal equations; multigrid with
+// all lanes per launch when every level has a class table, else one
+// multigrid solve per right-hand side.
+int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const double* rhs,
+             double* c, lsamd_solve_info* infos, hipStream_t st) {
+  if (s->op != 0) return fail(LSAMD_ERR_ARG, "multi-rhs multigrid: FD systems only");
+  if (int rc = mg_ensure(s, st)) return rc;
+  const Mg* m = s->mg;
+  bool lanes = s->regular && s->tab32 && s->tab64;
+  for (size_t l = 1; l + 1 < m->lv.size(); ++l) lanes = lanes && m->lv[l].sys->regular;
+  const int64_t n = s->n;
+  if (!lanes) {
+    for (int r = 0; r < nrhs; ++r)
+      if (int rc = mg_solve(s, o, rhs + (int64_t)r * n, c + (int64_t)r * n, &infos[r], st))
+        return rc;
+    return LSAMD_OK;
+  }
+  for (int r0 = 0; r0 < nrhs;) {
+    const int left = nrhs - r0;
+    const int R = left <= 4 ? 4 : 8;
+    const int nr = std::min(left, R);
+    MrWork* w = nullptr;
+    if (int rc = mr_ensure(s, R, &w)) return rc;
+    const double* rr = rhs + (int64_t)r0 * n;
+    double* cc = c + (int64_t)r0 * n;
+    const int rc = R == 4 ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
+                          : mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st);
+    if (rc) return rc;
+    r0 += nr;
+  }
   return LSAMD_OK;
 }
 
@@ -2463,6 +2814,7 @@ int lsamd_fd_system_destroy(lsamd_fd_system* s) {
   }
   if (s->mg) mg_destroy(s->mg);
   if (s->ls) ls_destroy(s->ls);
+  if (s->mr) mr_destroy(s->mr);
   for (void* p : s->owned) hipFree(p);
   if (s->ctl_host) hipHostFree(s->ctl_host);
   if (s->bctl_host) hipHostFree(s->bctl_host);
@@ -2483,6 +2835,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
   hipStream_t st = (hipStream_t)stream;
   LSAMD_HIP(hipStreamSynchronize(st));
   if (s->mg) mg_invalidate(s->mg);
+  if (s->mr) mr_invalidate(s->mr);
   if (s->ls) {
     ls_destroy(s->ls);
     s->ls = nullptr;
@@ -2793,6 +3146,39 @@ int lsamd_fd_system_lsmr(lsamd_fd_system* s, const lsamd_lsmr_opts* o, double* c
   memset(&inf, 0, sizeof(inf));
   if (int rc = ls_solve(s, o, c, &inf, (hipStream_t)stream)) return rc;
   if (info) *info = inf;
+  return LSAMD_OK;
+}
+
+int lsamd_fd_system_solve_multi(lsamd_fd_system* s, const lsamd_solve_opts* o, int32_t nrhs,
+                                const double* rhs, double* c, lsamd_solve_info* infos,
+                                void* stream) {
+  if (int rc = injected("solve")) return rc;
+  if (!s || !o || !rhs || !c || !infos || nrhs < 0)
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_system_solve_multi: bad argument");
+  if (!(o->rtol >= 0.0) || !(o->atol >= 0.0))
+    return fail(LSAMD_ERR_ARG, "rtol/atol must be non-negative");
+  if (o->jacobi != 2)
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_system_solve_multi: multigrid only (jacobi = 2)");
+  if (nrhs == 0) return LSAMD_OK;
+  return mr_solve(s, o, nrhs, rhs, c, infos, (hipStream_t)stream);
+}
+
+int lsamd_fd_system_rhs_rows(lsamd_fd_system* s, const double* b, double* rhs, void* stream) {
+  if (!s || !b || !rhs) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_rhs_rows: NULL argument");
+  if (s->op == 2) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_rhs_rows: not for batches");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = s->n;
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, rhs, n);
+  if (s->nrows > 0) {
+    RowsDev R = rows_dev(s, false);
+    hipLaunchKernelGGL(k_w2b, dim3((unsigned)ceil_div(s->nrows, 256)), dim3(256), 0, st,
+                       (const double*)s->w2_64, b, s->t64, s->nrows);
+    hipLaunchKernelGGL((k_rows_scatter<double, double>),
+                       dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256))), dim3(256), 0,
+                       st, R, (const double*)s->t64, rhs, 1, (CgCtl*)nullptr, 0,
+                       (const double*)nullptr, 0, (const double*)nullptr, 0);
+  }
+  LSAMD_CHECK_LAUNCH("rhs_rows");
   return LSAMD_OK;
 }
 

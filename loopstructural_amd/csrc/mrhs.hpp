@@ -1,0 +1,455 @@
+// Multigrid-preconditioned flexible CG for several right-hand sides of one
+// FD system at once (realisations that share the constraint locations and
+// differ in observed values or normal directions: same matrix, other rhs).
+// The R vectors of a level lie one after the other, v[r * n + i] (every
+// load of a lane coalesced across the wavefront), and every launch of the
+// V-cycle and of the outer iteration serves all R lanes: the coefficients,
+// indices and Schwarz rows are read once for R lanes and the launch count
+// per solve drops R-fold.  The hierarchy (coarse grids, Gram
+// matrices, Schwarz blocks, coarsest inverse, Chebyshev polynomials) is the
+// single-system one.  Lanes converge independently: a converged lane gets
+// alpha = 0 and keeps its iterate; every kernel returns at once when all
+// lanes are done.
+#pragma once
+
+namespace lsamd {
+
+// levels with at least this many nodes apply their coarse operator one
+// thread per node for all lanes (fewer, per-lane threads below)
+constexpr int64_t kMrNodeThreads = 1 << 17;
+
+// per right-hand side
+struct MrCtl {
+  double rho, rho_prev, beta, alpha, res2, thr, bnorm;
+  int iter, done, converged, maxiter;
+};
+
+// ------------------------------------------------------------ packing ----
+// out[r n + i] = in[r n + i] for r < nr, 0 for the padding lanes nr <= r < R
+__global__ void k_mr_pack(const double* __restrict__ in, int nr, int R, int64_t n,
+                          double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * R) return;
+  out[e] = e < n * nr ? in[e] : 0.0;
+}
+
+// ----------------------------------------------------------- operator ----
+// Every kernel of the V-cycle runs one thread per (lane, node), the lane in
+// blockIdx.y: the loads of one lane are coalesced and the small levels still
+// get R times the threads of a single solve.
+
+// y = R x (class-table regulariser, ridge included)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mr_stencil(const T* __restrict__ x, T* __restrict__ y,
+                                                    GridDev g, const T* __restrict__ tab, int R,
+                                                    const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int64_t r = blockIdx.y;
+  const T* xs = x + r * n;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int64_t nxy = (int64_t)nx * ny;
+  const int k = (int)(id / nxy);
+  const int64_t rr = id - (int64_t)k * nxy;
+  const int j = (int)(rr / nx);
+  const int i = (int)(rr - (int64_t)j * nx);
+  const T* c = tab + ((cls7(k, nz) * 7 + cls7(j, ny)) * 7 + cls7(i, nx)) * kTaps;
+  T acc = c[0] * xs[id];
+#pragma unroll
+  for (int t = 1; t < kTaps; ++t) {
+    const int dx = kBTap[t][0], dy = kBTap[t][1], dz = kBTap[t][2];
+    const bool ok = (unsigned)(i + dx) < (unsigned)nx && (unsigned)(j + dy) < (unsigned)ny &&
+                    (unsigned)(k + dz) < (unsigned)nz;
+    const T xv = xs[ok ? id + dx + (int64_t)nx * dy + nxy * dz : id];
+    acc += c[t] * (ok ? xv : (T)0);
+  }
+  y[r * n + id] = acc;
+}
+
+// t[r nrows + row] = w_r^2 (a_r . x_r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mr_rows_t(const T* __restrict__ x, int64_t n, RowsDev Rw,
+                                                   T* __restrict__ t, int R,
+                                                   const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const T* coef = (const T*)Rw.coef;
+  const T* w2 = (const T*)Rw.w2;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= Rw.nrows) return;
+  const int64_t r = blockIdx.y;
+  const T* xs = x + r * n;
+  T acc = 0;
+  for (int64_t q = Rw.row_ptr[row]; q < Rw.row_ptr[row + 1]; ++q) acc += coef[q] * xs[Rw.cols[q]];
+  t[r * Rw.nrows + row] = w2[row] * acc;
+}
+
+// y[node] += sum over the node's entries of coef * t[row]
+template <typename T>
+__global__ __launch_bounds__(256) void k_mr_rows_scatter(RowsDev Rw, const T* __restrict__ t,
+                                                         T* __restrict__ y, int64_t n, int R,
+                                                         const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const T* coef = (const T*)Rw.coef;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= Rw.ndn) return;
+  const int64_t r = blockIdx.y;
+  const T* ts = t + r * Rw.nrows;
+  T acc = 0;
+  for (int64_t q = Rw.seg_ptr[d]; q < Rw.seg_ptr[d + 1]; ++q)
+    acc += coef[Rw.seg_entry[q]] * ts[Rw.seg_row[q]];
+  y[r * n + Rw.dnode[d]] += acc;
+}
+
+// coarse level: y = (class-table regulariser + Gram) x
+__global__ __launch_bounds__(256) void k_mr_apply_coarse(MgDev L, const float* __restrict__ x,
+                                                         float* __restrict__ y, int R,
+                                                         const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= L.n) return;
+  const int64_t r = blockIdx.y;
+  y[r * L.n + id] = mg_apply_node(L, x + r * L.n, id);
+}
+
+// coarse level, one thread per node and all lanes (large coarse levels: the
+// class row and the Gram row are read once for the R lanes)
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_apply_coarse_all(MgDev L,
+                                                             const float* __restrict__ x,
+                                                             float* __restrict__ y,
+                                                             const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= L.n) return;
+  const int64_t n = L.n;
+  int i, j, k;
+  mg_ijk(id, L.g.nx, L.g.ny, i, j, k);
+  const int nx = L.g.nx, ny = L.g.ny, nz = L.g.nz;
+  const int64_t nxy = (int64_t)nx * ny;
+  const float* c = L.tab + ((cls7(k, nz) * 7 + cls7(j, ny)) * 7 + cls7(i, nx)) * kTaps;
+  float acc[R];
+  const float c0 = c[0];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = c0 * x[r * n + id];
+#pragma unroll
+  for (int t = 1; t < kTaps; ++t) {
+    const int dx = kBTap[t][0], dy = kBTap[t][1], dz = kBTap[t][2];
+    const bool ok = (unsigned)(i + dx) < (unsigned)nx && (unsigned)(j + dy) < (unsigned)ny &&
+                    (unsigned)(k + dz) < (unsigned)nz;
+    if (!ok) continue;
+    const int64_t o = id + dx + (int64_t)nx * dy + nxy * dz;
+    const float ct = c[t];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] += ct * x[r * n + o];
+  }
+  if (L.gptr)
+    for (int64_t e = L.gptr[id]; e < L.gptr[id + 1]; ++e) {
+      const float a = L.gval[e];
+      const int64_t o = L.gcol[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += a * x[r * n + o];
+    }
+#pragma unroll
+  for (int r = 0; r < R; ++r) y[r * n + id] = acc[r];
+}
+
+// ----------------------------------------------------------- smoother ----
+// Chebyshev step with the Schwarz preconditioner M: mode 0 first step from
+// zero (d = c0 M b, x = d), 1 first step (d = c0 M (b - q), x += d), 2 later
+// step (d = c1 d + c2 M (b - q), x += d).  skip_cov (large levels): the
+// covered nodes are left to k_mr_cheb_cov.
+__global__ __launch_bounds__(256) void k_mr_cheb(const float* __restrict__ b,
+                                                 const float* __restrict__ q, MgSm S,
+                                                 float* __restrict__ x, float* __restrict__ d,
+                                                 int64_t n, int skip_cov, int mode, float c0,
+                                                 float c1, float c2,
+                                                 const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (skip_cov && S.sm[i] < 0.f) return;
+  const int64_t r = blockIdx.y, e = r * n + i;
+  const float z = mg_precond(i, S, b + r * n, q + r * n, mode == 0);
+  const float dv = mode == 2 ? c1 * d[e] + c2 * z : c0 * z;
+  d[e] = dv;
+  x[e] = mode == 0 ? dv : x[e] + dv;
+}
+
+// the same step for the covered nodes of a large level (compact list), one
+// thread per (lane, listed node)
+__global__ __launch_bounds__(256) void k_mr_cheb_cov(const int32_t* __restrict__ cov,
+                                                     int64_t ncov, const float* __restrict__ b,
+                                                     const float* __restrict__ q, MgSm S,
+                                                     float* __restrict__ x, float* __restrict__ d,
+                                                     int64_t n, int mode, float c0, float c1,
+                                                     float c2, const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncov) return;
+  const int64_t i = cov[t];
+  const int64_t r = blockIdx.y, e = r * n + i;
+  const float z = mg_precond(i, S, b + r * n, q + r * n, mode == 0);
+  const float dv = mode == 2 ? c1 * d[e] + c2 * z : c0 * z;
+  d[e] = dv;
+  x[e] = mode == 0 ? dv : x[e] + dv;
+}
+
+// ------------------------------------------------- transfer, coarsest ----
+__global__ void k_mr_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
+                              GridDev gf, float* __restrict__ bc, GridDev gc, int R,
+                              const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
+  const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int64_t r = blockIdx.y;
+  bc[r * nc + id] = mg_restrict_node(bf + r * nf, qf + r * nf, gf, gc, id);
+}
+
+__global__ void k_mr_prolong(const float* __restrict__ xc, GridDev gc, float* __restrict__ xf,
+                             GridDev gf, int R, const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
+  const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nf) return;
+  const int64_t r = blockIdx.y;
+  xf[r * nf + id] += mg_prolong_node(xc + r * nc, gc, gf, id);
+}
+
+// x = Cinv b on the coarsest level, one thread per (lane, row)
+__global__ void k_mr_dense(const float* __restrict__ cinv, const float* __restrict__ b,
+                           float* __restrict__ x, int nc, int R,
+                           const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc * R) return;
+  const int r = e / nc, row = e - (e / nc) * nc;
+  float acc = 0.f;
+  for (int c = 0; c < nc; ++c) acc += cinv[(int64_t)row * nc + c] * b[r * nc + c];
+  x[e] = acc;
+}
+
+// --------------------------------------------------------- outer loop ----
+// Per-block partial sums of `np` products per lane, written part[blk * np *
+// R + k * R + r]; reduced in fixed order by a one-block kernel.
+template <int R>
+__device__ inline void mr_block_partials(const double* v, int np, double* part) {
+  __shared__ double sh[4];
+  for (int k = 0; k < np; ++k)
+    for (int r = 0; r < R; ++r) {
+      const double s = block_sum(v[k * R + r], sh);
+      if (threadIdx.x == 0) part[((int64_t)blockIdx.x * np + k) * R + r] = s;
+    }
+}
+// Sum of the nblk partials of product k of lane r, by one wavefront (fixed
+// order: strided lane sums, then the butterfly); result in every lane.
+template <int R>
+__device__ inline double mr_wave_reduce(const double* part, int nblk, int np, int k, int r) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int b = lane; b < nblk; b += 64) v += part[((int64_t)b * np + k) * R + r];
+  return wave_sum(v);
+}
+
+// x = 0 ; r = rhs ; rf = (float) r ; partials of r.(minv r)
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
+                                                 const double* __restrict__ minv,
+                                                 double* __restrict__ x, double* __restrict__ rr,
+                                                 float* __restrict__ rf, int64_t n,
+                                                 double* __restrict__ part) {
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double m = minv[i];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double v = rhs[r * n + i];
+      x[r * n + i] = 0.0;
+      rr[r * n + i] = v;
+      rf[r * n + i] = (float)v;
+      acc[r] += v * (m * v);
+    }
+  }
+  mr_block_partials<R>(acc, 1, part);
+}
+
+// one block of R wavefronts, wavefront r for lane r
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_mr_init_fin(const double* __restrict__ part, int nblk,
+                                                        int nr, double rtol, double atol,
+                                                        int maxiter, MrCtl* __restrict__ ctl,
+                                                        int* __restrict__ all_done) {
+  __shared__ int done[R];
+  const int r = threadIdx.x >> 6;
+  const double s = mr_wave_reduce<R>(part, nblk, 1, 0, r);
+  if ((threadIdx.x & 63) == 0) {
+    MrCtl c = {};
+    c.bnorm = sqrt(s);
+    c.thr = fmax(atol, rtol * c.bnorm);
+    c.res2 = s;
+    c.maxiter = maxiter;
+    c.converged = r < nr ? (sqrt(s) < c.thr || s == 0.0) : 1;
+    c.done = c.converged || maxiter <= 0;
+    ctl[r] = c;
+    done[r] = c.done;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int k = 0; k < R; ++k) all = all && done[k];
+    *all_done = all;
+  }
+}
+
+// partials of z.r and z.q per lane (z: the FP32 V-cycle output; q still
+// holds the previous N p, so z.r_prev = z.r + alpha_prev z.q)
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
+                                                 const double* __restrict__ rr,
+                                                 const double* __restrict__ q, int64_t n,
+                                                 double* __restrict__ part,
+                                                 const int* __restrict__ all_done) {
+  if (*all_done) return;
+  double acc[2 * R];
+#pragma unroll
+  for (int r = 0; r < 2 * R; ++r) acc[r] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double zv = (double)z[r * n + i];
+      acc[r] += zv * rr[r * n + i];
+      acc[R + r] += zv * q[r * n + i];
+    }
+  mr_block_partials<R>(acc, 2, part);
+}
+
+// rho = z.r ; beta = (z.r - z.r_prev) / rho_prev = -alpha_prev z.q / rho_prev
+// (0 on the first step)
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_mr_dots_fin(const double* __restrict__ part, int nblk,
+                                                        MrCtl* __restrict__ ctl,
+                                                        const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int r = threadIdx.x >> 6;
+  const double a = mr_wave_reduce<R>(part, nblk, 2, 0, r);
+  const double b = mr_wave_reduce<R>(part, nblk, 2, 1, r);
+  if ((threadIdx.x & 63) == 0 && !ctl[r].done) {
+    ctl[r].beta = ctl[r].iter == 0 ? 0.0 : -ctl[r].alpha * b / ctl[r].rho_prev;
+    ctl[r].rho = a;
+  }
+}
+
+// p = z + beta p
+template <int R>
+__global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, int64_t n,
+                         const MrCtl* __restrict__ ctl, const int* __restrict__ all_done) {
+  if (*all_done) return;
+  double beta[R];
+  bool first[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    beta[r] = ctl[r].beta;
+    first[r] = ctl[r].iter == 0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t e = r * n + i;
+      p[e] = first[r] ? (double)z[e] : (double)z[e] + beta[r] * p[e];
+    }
+}
+
+// partials of p.q per lane
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_pq(const double* __restrict__ p,
+                                               const double* __restrict__ q, int64_t n,
+                                               double* __restrict__ part,
+                                               const int* __restrict__ all_done) {
+  if (*all_done) return;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] += p[r * n + i] * q[r * n + i];
+  mr_block_partials<R>(acc, 1, part);
+}
+
+// alpha = rho / p.q (0 on converged lanes)
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_mr_alpha(const double* __restrict__ part, int nblk,
+                                                     MrCtl* __restrict__ ctl,
+                                                     const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int r = threadIdx.x >> 6;
+  const double pq = mr_wave_reduce<R>(part, nblk, 1, 0, r);
+  if ((threadIdx.x & 63) == 0) ctl[r].alpha = ctl[r].done ? 0.0 : ctl[r].rho / pq;
+}
+
+// x += alpha p ; r -= alpha q ; rf = (float) r ; partials of r.(minv r)
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* __restrict__ rr,
+                                                const double* __restrict__ p,
+                                                const double* __restrict__ q,
+                                                const double* __restrict__ minv,
+                                                float* __restrict__ rf, int64_t n,
+                                                double* __restrict__ part,
+                                                const MrCtl* __restrict__ ctl,
+                                                const int* __restrict__ all_done) {
+  if (*all_done) return;
+  double alpha[R], acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    alpha[r] = ctl[r].alpha;
+    acc[r] = 0.0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double m = minv[i];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t e = r * n + i;
+      x[e] += alpha[r] * p[e];
+      const double rn = rr[e] - alpha[r] * q[e];
+      rr[e] = rn;
+      rf[e] = (float)rn;
+      acc[r] += rn * (m * rn);
+    }
+  }
+  mr_block_partials<R>(acc, 1, part);
+}
+
+// rho_prev = rho ; iter++ ; stopping test on |S r| per lane
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_mr_upd_fin(const double* __restrict__ part, int nblk,
+                                                       MrCtl* __restrict__ ctl,
+                                                       int* __restrict__ all_done) {
+  if (*all_done) return;
+  __shared__ int done[R];
+  const int r = threadIdx.x >> 6;
+  const double s = mr_wave_reduce<R>(part, nblk, 1, 0, r);
+  if ((threadIdx.x & 63) == 0) {
+    MrCtl& c = ctl[r];
+    if (!c.done) {
+      c.rho_prev = c.rho;
+      c.iter += 1;
+      c.res2 = s;
+      c.converged = sqrt(s) < c.thr;
+      c.done = c.converged || c.iter >= c.maxiter;
+    }
+    done[r] = c.done;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int k = 0; k < R; ++k) all = all && done[k];
+    *all_done = all;
+  }
+}
+
+}  // namespace lsamd

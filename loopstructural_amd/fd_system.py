@@ -152,6 +152,39 @@ class FdSystem:
         _lib.call("lsamd_fd_system_diag", self._h, _lib.ptr(out), _lib.stream())
         return out
 
+    def rhs_for(self, b: torch.Tensor) -> torch.Tensor:
+        """A^T W^2 b for other observed values b of the system's rows (one
+        value per row, the rows' own coefficients and weights)."""
+        nrows = self.rows.nrows if self.rows is not None else 0
+        if b.numel() != nrows:
+            raise ValueError(f"rhs_for: {b.numel()} values for {nrows} rows")
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        b = b.to(self.dev, F64).contiguous()
+        _lib.call("lsamd_fd_system_rhs_rows", self._h, _lib.ptr(b), _lib.ptr(out), _lib.stream())
+        return out
+
+    def solve_multi(self, rhs: torch.Tensor, rtol=1e-5, atol=0.0, maxiter=None, check_every=8):
+        """Multigrid solves of N c_r = rhs[r] for every row of rhs (R, n) at
+        once: the V-cycle and outer-iteration launches serve up to 8 right-hand
+        sides each.  Same stopping test per right-hand side as solve()."""
+        rhs = rhs.to(self.dev, F64).contiguous()
+        if rhs.dim() != 2 or rhs.shape[1] != self.n:
+            raise ValueError(f"solve_multi: rhs must be (R, {self.n})")
+        nr = rhs.shape[0]
+        o = _lib.SolveOpts()
+        o.rtol, o.atol = float(rtol), float(atol)
+        o.maxiter = int(maxiter) if maxiter else 0
+        o.precision = _lib.FP64
+        o.check_every = int(check_every)
+        o.jacobi = 2
+        infos = (_lib.SolveInfo * max(nr, 1))()
+        c = torch.empty((nr, self.n), dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_solve_multi", self._h, C.byref(o), int(nr), _lib.ptr(rhs),
+                  _lib.ptr(c), infos, _lib.stream())
+        return c, [{"iterations": int(i.iterations), "converged": bool(i.converged),
+                    "residual": float(i.residual), "threshold": float(i.threshold)}
+                   for i in list(infos)[:nr]]
+
     def solve(self, rtol=1e-5, atol=0.0, maxiter=None, precision="fp64", rhs=None,
               check_every=64, inner_rtol=1e-4, max_outer=50, jacobi=True, multigrid=False):
         """PCG on N c = rhs.  jacobi: the reference's column scaling as the

@@ -1,0 +1,92 @@
+"""Multigrid solves of several right-hand sides per launch sequence
+(realisations that share the constraint locations: one matrix, perturbed
+observed values and normal directions).
+
+Each lane is checked against the single-system multigrid solve of the same
+right-hand side (both to rtol 1e-12: agreement 1e-9 relative) and against the
+exact minimiser of its realisation (spsolve on the oracle's normal equations,
+1e-8); lane counts 3, 8 and 11 cover the 4- and 8-lane kernels and a split
+into two groups.  The right-hand side of a realisation from its row values is
+checked against the system's own rhs after set_rows with those rows.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd as ofd
+
+pytestmark = pytest.mark.gpu
+
+W = {"dxx": 0.1, "dyy": 0.1, "dzz": 0.1, "dxy": 0.1, "dxz": 0.1, "dyz": 0.1,
+     "dx": 1.0, "dy": 1.0, "dz": 1.0}
+
+
+def _rows(fs, _lib, g, n, values, normals):
+    blocks = []
+    for pts, kind, wcol, ws in ((normals, _lib.ROWS_NORMAL, 6, 10.0 / 3),
+                                (values, _lib.ROWS_VALUE, 4, 1.0)):
+        p = torch.from_numpy(pts).cuda()
+        _, ins, _, _ = fs.locate(g, p[:, :3].contiguous(), want_w=False)
+        blocks.append(fs.build_rows(g, kind, p[ins].contiguous(), wcol, ws, n))
+    return fs.concat_rows(blocks, n)
+
+
+def _setup(cells, nreal, seed=31):
+    from loopstructural_amd import _lib, synthetic
+    from loopstructural_amd import fd_system as fs
+
+    nn = cells + 1
+    g = _lib.make_grid((0, 0, 0), (1.0 / cells,) * 3, (nn,) * 3)
+    n = nn ** 3
+    values, normals = synthetic.constraints(400, 80, seed)
+    reals = [synthetic.perturb(values, normals, 100 + r) for r in range(nreal)]
+    s = fs.FdSystem(g, fs.reg_struct(W, (nn,) * 3, 1e-8))
+    s.set_rows(_rows(fs, _lib, g, n, *reals[0]))
+    rhs = torch.stack([s.rhs_for(_rows(fs, _lib, g, n, v, m).b) for v, m in reals])
+    return s, rhs, reals, g, n
+
+
+def test_rhs_for_matches_set_rows():
+    from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
+
+    s, rhs, reals, g, n = _setup(15, 2)
+    s2 = fs.FdSystem(g, fs.reg_struct(W, (16,) * 3, 1e-8))
+    s2.set_rows(_rows(fs, _lib, g, n, *reals[1]))
+    ref = s2.rhs()
+    assert float(torch.linalg.vector_norm(rhs[1] - ref) / torch.linalg.vector_norm(ref)) <= 1e-14
+
+
+@pytest.mark.parametrize("nreal", [3, 8, 11])
+def test_multi_rhs_lanes_match_single_solves(nreal):
+    s, rhs, reals, g, n = _setup(23, nreal)
+    c, infos = s.solve_multi(rhs, rtol=1e-12)
+    assert all(i["converged"] for i in infos), infos
+    for r in range(nreal):
+        c1, i1 = s.solve(rtol=1e-12, rhs=rhs[r], multigrid=True, check_every=8)
+        d = float(torch.linalg.vector_norm(c[r] - c1) / torch.linalg.vector_norm(c1))
+        assert d <= 1e-9, (r, d)
+        assert abs(infos[r]["iterations"] - i1["iterations"]) <= 2, (infos[r], i1)
+    og = ofd.Grid((0, 0, 0), (23,) * 3, (1 / 23,) * 3)
+    for r in (0, nreal - 1):
+        v, m = reals[r]
+        ref = ofd.solve_exact(ofd.assemble_fd(og, {"value": v.copy(), "normal": m.copy()},
+                                              regularisation=0.1))
+        assert np.linalg.norm(c[r].cpu().numpy() - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def test_multi_rhs_after_new_rows_and_zero_rhs():
+    s, rhs, reals, g, n = _setup(19, 4)
+    rhs[2] = 0.0
+    c, infos = s.solve_multi(rhs, rtol=1e-12)
+    assert infos[2]["iterations"] == 0 and float(c[2].abs().max()) == 0.0
+    # new rows (other realisation set) on the same handle: graph rebuilt
+    from loopstructural_amd import _lib, synthetic
+    from loopstructural_amd import fd_system as fs
+
+    values, normals = synthetic.constraints(300, 50, 77)
+    s.set_rows(_rows(fs, _lib, g, n, values, normals))
+    c2, infos2 = s.solve_multi(s.rhs()[None, :].repeat(2, 1), rtol=1e-12)
+    c1, _ = s.solve(rtol=1e-12, multigrid=True)
+    for r in range(2):
+        assert float(torch.linalg.vector_norm(c2[r] - c1) / torch.linalg.vector_norm(c1)) <= 1e-9
