@@ -92,8 +92,11 @@ typedef struct lsamd_solve_opts {
                            Jacobi-scaled stopping test; precision, inner_rtol
                            and max_outer unused; check_every capped at 8.
                            Environment: LSAMD_MG_SMOOTHER=jacobi for point
-                           Jacobi, LSAMD_MG_DEGREE=1..4 for the Chebyshev
-                           degree (2), LSAMD_MG_TIMING=1 prints setup times).
+                           Jacobi, LSAMD_MG_DEGREE=1..8 for the Chebyshev
+                           degree of the coarse levels (4),
+                           LSAMD_MG_DEGREE0 for the fine level's (2, at most
+                           the coarse one), LSAMD_MG_TIMING=1 prints set-up
+                           times).
                            No reference counterpart: the reference solves
                            with Jacobi-scaled CG; the minimiser is the same. */
   int32_t pad;

@@ -9,7 +9,8 @@
 // re-discretised on the coarse grid with every squared weight scaled by
 // 2^-l (a smooth field's second-difference energy halves per coarsening).
 // The coarsest level (<= 5 nodes per axis) is solved with a dense inverse.
-// Smoother: Chebyshev polynomial (degree 2) in M N on
+// Smoother: Chebyshev polynomial (degree 2 on the fine level, 4 on the
+// coarse ones) in M N on
 // [lambda_max/30, 1.15 lambda_max], lambda_max by power iteration, with M
 // additive Schwarz: the exact inverse of N on each block, summed over
 // blocks, point Jacobi on the nodes no block covers.  Blocks are the 8 nodes
@@ -27,7 +28,7 @@
 
 namespace lsamd {
 
-constexpr int kMgDegMax = 4;  // Chebyshev degree: 2 unless LSAMD_MG_DEGREE says 1..4
+constexpr int kMgDegMax = 8;  // largest Chebyshev degree (LSAMD_MG_DEGREE)
 constexpr int kMgCoarse = 5;  // stop coarsening at <= 5 nodes per axis
 
 // --------------------------------------------------------------- kernels ----
