@@ -560,15 +560,9 @@ class DiscreteInterpolator(GeologicalInterpolator):
         if callable(choice):
             raise NotImplementedError(
                 "a callable solver needs the explicit sparse system; use solver='cg'")
-        if choice == "lsmr" and not isinstance(system, fs.P1System):
-            return self._solve_lsmr(system, tol, kw, timing, started)
         if choice == "lsmr":
-            logger.warning("lsmr on P1 supports runs as GPU CG on the normal equations "
-                           "(same minimiser)")
-            kw["rtol"] = kw.pop("btol", tol if tol is not None else 1e-6)
-            for k in ("atol", "conlim", "damp", "show"):
-                kw.pop(k, None)
-        elif ("atol" not in kw or "rtol" not in kw) and tol is not None:
+            return self._solve_lsmr(self._lsmr_system(system), tol, kw, timing, started)
+        if ("atol" not in kw or "rtol" not in kw) and tol is not None:
             kw["atol"] = tol
         for k in ("x0", "M", "callback"):
             if kw.pop(k, None) is not None:
@@ -598,6 +592,11 @@ class DiscreteInterpolator(GeologicalInterpolator):
         timing["up_to_date"] = True
         self.latest_solve_timing = timing
         return self.up_to_date
+
+    def _lsmr_system(self, system):
+        """The system LSMR iterates on; FD systems hold their regulariser
+        as rows already."""
+        return system
 
     def _solve_lsmr(self, system, tol, kw, timing, started) -> bool:
         """solve_with_lsmr (LI/_solver_strategy.py:108-136): LSMR on the
@@ -1214,6 +1213,25 @@ class P1Interpolator(DiscreteInterpolator):
         ndata = rows.nrows if rows is not None else 0
         nnz = (rows.nnz if rows is not None else 0) + 8 * nreg
         return system, ndata + nreg, nnz
+
+    def _lsmr_system(self, system):
+        """LSMR needs the edge-jump operator as rows (LI/_p1interpolator.py:115-166
+        adds them to the explicit matrix that solve_with_lsmr iterates on,
+        LI/_solver_strategy.py:108-136).  The families share one set of rows
+        and differ only in weight, so they are merged into one block of
+        weight sqrt(sum w^2): A^T A, A^T b and |b| are unchanged, and so are
+        the LSMR iterates.  Data rows, those edge rows and the ridge go to a
+        grid system whose finite-difference regulariser is zero."""
+        grid = self.support._lsgrid()
+        blocks = list(self._rows.values())
+        if self._edge_families:
+            w2 = sum(f.weight ** 2 for f in self._edge_families.values())
+            blocks.append(fs.p1_edge_rows(grid, float(np.sqrt(w2)), self.dof))
+        ridge = float(self.ridge_factor) if self.add_ridge_regulatisation else 0.0
+        zero = {k: 0.0 for k in ("dxx", "dyy", "dzz", "dxy", "dxz", "dyz")}
+        ls = fs.FdSystem(grid, fs.reg_struct(zero, grid.nnodes, ridge))
+        ls.set_rows(fs.concat_rows(blocks, self.dof))
+        return ls
 
     def _evaluate(self, pts, values, gradients):
         v, g = fs.p1_evaluate(self.support._lsgrid(), self._c_device(), torch.from_numpy(pts),
