@@ -96,7 +96,9 @@ typedef struct lsamd_solve_opts {
                            degree of the coarse levels (4),
                            LSAMD_MG_DEGREE0 for the fine level's (2, at most
                            the coarse one), LSAMD_MG_TIMING=1 prints set-up
-                           times).
+                           times, LSAMD_MR_COV_LANES=1|2|4|8 sets the
+                           right-hand sides per thread of the multi-rhs
+                           covered-node smoothing (2)).
                            No reference counterpart: the reference solves
                            with Jacobi-scaled CG; the minimiser is the same. */
   int32_t pad;
@@ -170,8 +172,8 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, cons
  * 2): rhs and c are (nrhs, nnodes) FP64, one vector after the other; infos
  * (nrhs) host memory.  Realisations that share the constraint locations and
  * differ in observed values have one matrix and nrhs right-hand sides; up to
- * 8 of them are solved per launch sequence (their vectors interleaved per
- * node), with the single-system hierarchy and stopping test per rhs.  No
+ * 16 of them are solved per launch sequence (their vectors one after the
+ * other), with the single-system hierarchy and stopping test per rhs.  No
  * reference counterpart (the reference solves realisations one by one). */
 int lsamd_fd_system_solve_multi(lsamd_fd_system* s, const lsamd_solve_opts* opts, int32_t nrhs,
                                 const double* rhs, double* c, lsamd_solve_info* infos,

@@ -1714,7 +1714,7 @@ int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOu
                      (const int64_t*)L.sptr, (const float*)S->minv32, L.sm, L.n);
   cleanup();
   LSAMD_CHECK_LAUNCH("multigrid smoother tags");
-  if (L.n >= kSplitMin) {
+  if (L.n >= kCovListMin) {
     // compact list of the covered nodes (their tags are negative)
     uint8_t* flag = nullptr;
     int32_t *cov = nullptr, *nsel = nullptr;
@@ -2060,7 +2060,7 @@ void mg_vcycle(Mg* m, int l, const CgCtl* ctl, hipStream_t st) {
   }
   const dim3 nb((unsigned)ceil_div(L.n, 256));
   const MgSm S = mg_sm(L);
-  const int split = L.cov ? 1 : 0;
+  const int split = L.cov && L.n >= kSplitMin ? 1 : 0;
   const dim3 ncb((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)));
   auto smooth = [&](int from_zero) {
     hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b,
@@ -2193,10 +2193,11 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
 // ------------------------------------------------- multi-rhs multigrid ----
 struct MrWork {
   int R = 0;
-  std::vector<float*> b, x, q, d;  // per level, R lanes interleaved
+  std::vector<float*> b, x, q, d;  // per level, R lane vectors one after the other
   double *rhs = nullptr, *x64 = nullptr, *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
   float* t32 = nullptr;
   double* t64 = nullptr;
+  double* upart = nullptr;  // per-block partials of the row part of p.Np, per lane
   int64_t t_rows = -1;
   double* part = nullptr;
   MrCtl* ctl = nullptr;
@@ -2204,7 +2205,8 @@ struct MrWork {
   MrCtl* host = nullptr;  // pinned: R controls and the all-done flag
   // level-0 stencil over the R lanes as R systems of the plane-streaming
   // kernel: system table, work items per precision (FP64, FP32), zero
-  // controls (no lane skipped), dot partials (unused)
+  // controls (no lane skipped), per-work-item partials of x.Rx (the FP64
+  // ones give the stencil part of p.Np)
   BSys* sysR = nullptr;
   int4* tilesR[2] = {nullptr, nullptr};
   int ntR[2] = {0, 0};
@@ -2305,7 +2307,7 @@ int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
   if (w->t_rows != s->nrows) {
     mr_invalidate(w);
     if (w->t32) {
-      for (void* p : {(void*)w->t32, (void*)w->t64})
+      for (void* p : {(void*)w->t32, (void*)w->t64, (void*)w->upart})
         for (size_t i = 0; i < w->mem.size(); ++i)
           if (w->mem[i] == p) {
             hipFree(p);
@@ -2315,6 +2317,8 @@ int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
     }
     if (int rc = mr_alloc(w, &w->t32, std::max<int64_t>(s->nrows, 1) * R)) return rc;
     if (int rc = mr_alloc(w, &w->t64, std::max<int64_t>(s->nrows, 1) * R)) return rc;
+    if (int rc = mr_alloc(w, &w->upart, std::max<int64_t>(ceil_div(s->nrows, 256), 1) * R))
+      return rc;
     w->t_rows = s->nrows;
   }
   *out = w;
@@ -2370,6 +2374,17 @@ void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hi
     hipLaunchKernelGGL(k_mr_apply_coarse, nb, b256, 0, st, mg_dev(L), x, y, R, ad);
 }
 
+// lanes per thread of the covered-node smoothing step (LSAMD_MR_COV_LANES:
+// 1, 2, 4 or 8, at most R; default 2, the fastest at 128^3 with 16 lanes)
+int mr_cov_lanes(int R) {
+  static const int v = [] {
+    const char* e = getenv("LSAMD_MR_COV_LANES");
+    const int k = e ? atoi(e) : 2;
+    return k == 1 || k == 2 || k == 4 || k == 8 ? k : 2;
+  }();
+  return std::min(v, R);
+}
+
 template <int R>
 void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   const Mg* m = s->mg;
@@ -2388,10 +2403,30 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   auto step = [&](int mode, float c1, float c2) {
     hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
                        w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
-    if (split && L.ncov > 0)
-      hipLaunchKernelGGL(k_mr_cheb_cov, ncb, b256, 0, st, (const int32_t*)L.cov, L.ncov,
-                         (const float*)w->b[l], (const float*)w->q[l], S, w->x[l], w->d[l], L.n,
-                         mode, L.c0, c1, c2, ad);
+    if (split && L.ncov > 0) {
+      const int lg = mr_cov_lanes(R);
+      const dim3 ncg((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)), R / lg);
+      const int32_t* cv = (const int32_t*)L.cov;
+      const float *bl = w->b[l], *ql = w->q[l];
+      switch (lg) {
+        case 1:
+          hipLaunchKernelGGL(k_mr_cheb_cov, ncb, b256, 0, st, cv, L.ncov, bl, ql, S, w->x[l],
+                             w->d[l], L.n, mode, L.c0, c1, c2, ad);
+          break;
+        case 2:
+          hipLaunchKernelGGL(k_mr_cheb_cov_lg<2>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+          break;
+        case 8:
+          hipLaunchKernelGGL(k_mr_cheb_cov_lg<8>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+          break;
+        default:
+          hipLaunchKernelGGL(k_mr_cheb_cov_lg<4>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+          break;
+      }
+    }
   };
   auto smooth = [&](int from_zero) {
     step(from_zero ? 0 : 1, 0.f, 0.f);
@@ -2425,18 +2460,20 @@ void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
                      ad);
   hipLaunchKernelGGL(k_mr_dir<R>, dim3(vb), b256, 0, st, (const float*)w->x[0], w->p64, n,
                      (const MrCtl*)w->ctl, ad);
+  // q = N p; p.q from the stencil's per-work-item partials and the rows'
+  // per-block partials
   RowsDev Rw = rows_dev(s, false);
+  const int nbu = s->nrows > 0 ? (int)ceil_div(s->nrows, 256) : 0;
   if (s->nrows > 0)
-    hipLaunchKernelGGL(k_mr_rows_t<double>, dim3((unsigned)ceil_div(s->nrows, 256), R), b256, 0,
-                       st, (const double*)w->p64, n, Rw, w->t64, R, ad);
+    hipLaunchKernelGGL(k_mr_rows_tu, dim3((unsigned)nbu, R), b256, 0, st, (const double*)w->p64,
+                       n, Rw, w->t64, w->upart, ad);
   mr_tiles<double>(s, w, (const double*)w->p64, w->q64, st);
   if (s->nrows > 0)
     hipLaunchKernelGGL(k_mr_rows_scatter<double>,
                        dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0, st,
                        Rw, (const double*)w->t64, w->q64, n, R, ad);
-  hipLaunchKernelGGL(k_mr_pq<R>, dim3(vb), b256, 0, st, (const double*)w->p64,
-                     (const double*)w->q64, n, w->part, ad);
-  hipLaunchKernelGGL(k_mr_alpha<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl, ad);
+  hipLaunchKernelGGL(k_mr_alpha_st<R>, dim3(1), fin, 0, st, (const double*)w->partR,
+                     w->ntR[0] / R, (const double*)w->upart, nbu, w->ctl, ad);
   hipLaunchKernelGGL(k_mr_upd<R>, dim3(vb), b256, 0, st, w->x64, w->r64, (const double*)w->p64, (const double*)w->q64, (const double*)s->minv64,
                      w->b[0], n, w->part, (const MrCtl*)w->ctl, ad);
   hipLaunchKernelGGL(k_mr_upd_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
@@ -2516,14 +2553,15 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
   }
   for (int r0 = 0; r0 < nrhs;) {
     const int left = nrhs - r0;
-    const int R = left <= 4 ? 4 : 8;
+    const int R = left <= 4 ? 4 : (left <= 8 ? 8 : 16);
     const int nr = std::min(left, R);
     MrWork* w = nullptr;
     if (int rc = mr_ensure(s, R, &w)) return rc;
     const double* rr = rhs + (int64_t)r0 * n;
     double* cc = c + (int64_t)r0 * n;
-    const int rc = R == 4 ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
-                          : mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st);
+    const int rc = R == 4   ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
+                   : R == 8 ? mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st)
+                            : mr_solve_group<16>(s, w, o, nr, rr, cc, infos + r0, st);
     if (rc) return rc;
     r0 += nr;
   }

@@ -547,6 +547,9 @@ __global__ __launch_bounds__(256) void k_mg_apply(MgDev L, const float* __restri
 }
 
 constexpr int64_t kSplitMin = 1 << 20;  // levels this large smooth covered nodes apart
+// levels this large keep the compact list of their covered nodes (the
+// multi-rhs smoother splits them, the single-system one from kSplitMin)
+constexpr int64_t kCovListMin = 1 << 17;
 
 // ------------------------------------------------ Schwarz smoother blocks ----
 // Entry N(a, b) of a level's operator: regulariser rows (from their

@@ -85,6 +85,32 @@ __global__ __launch_bounds__(256) void k_mr_rows_t(const T* __restrict__ x, int6
   t[r * Rw.nrows + row] = w2[row] * acc;
 }
 
+// FP64 form with the row part of x.Nx: t = w^2 (a.x) per lane and per-block
+// partials of sum_rows t (a.x), written upart[lane * gridDim.x + blk]
+__global__ __launch_bounds__(256) void k_mr_rows_tu(const double* __restrict__ x, int64_t n,
+                                                    RowsDev Rw, double* __restrict__ t,
+                                                    double* __restrict__ upart,
+                                                    const int* __restrict__ all_done) {
+  __shared__ double sh[4];
+  if (*all_done) return;
+  const double* coef = (const double*)Rw.coef;
+  const double* w2 = (const double*)Rw.w2;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y;
+  double u = 0.0;
+  if (row < Rw.nrows) {
+    const double* xs = x + r * n;
+    double acc = 0;
+    for (int64_t q = Rw.row_ptr[row]; q < Rw.row_ptr[row + 1]; ++q)
+      acc += coef[q] * xs[Rw.cols[q]];
+    const double tv = w2[row] * acc;
+    t[r * Rw.nrows + row] = tv;
+    u = tv * acc;
+  }
+  const double bs = block_sum(u, sh);
+  if (threadIdx.x == 0) upart[r * gridDim.x + blockIdx.x] = bs;
+}
+
 // y[node] += sum over the node's entries of coef * t[row]
 template <typename T>
 __global__ __launch_bounds__(256) void k_mr_rows_scatter(RowsDev Rw, const T* __restrict__ t,
@@ -196,6 +222,56 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov(const int32_t* __restrict__
   x[e] = mode == 0 ? dv : x[e] + dv;
 }
 
+// the same step for LG lanes per thread: the covered node's row of summed
+// block inverses is read once for the LG lanes (lane group in blockIdx.y);
+// entries summed in the association of mg_precond
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restrict__ cov,
+                                                        int64_t ncov, const float* __restrict__ b,
+                                                        const float* __restrict__ q, MgSm S,
+                                                        float* __restrict__ x,
+                                                        float* __restrict__ d, int64_t n, int mode,
+                                                        float c0, float c1, float c2,
+                                                        const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncov) return;
+  const int64_t i = cov[t];
+  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  const float* bl = b + r0 * n;
+  const float* ql = q + r0 * n;
+  const int fz = mode == 0;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  int64_t e = S.sptr[i];
+  const int64_t e1 = S.sptr[i + 1];
+  for (; e + 4 <= e1; e += 4) {
+    const int64_t j0 = S.scol[e], j1 = S.scol[e + 1], j2 = S.scol[e + 2], j3 = S.scol[e + 3];
+    const float a0 = S.sval[e], a1 = S.sval[e + 1], a2 = S.sval[e + 2], a3 = S.sval[e + 3];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const float* bb = bl + l * n;
+      const float* qq = ql + l * n;
+      acc[l] += a0 * mg_r(bb, qq, j0, fz) + a1 * mg_r(bb, qq, j1, fz) +
+                a2 * mg_r(bb, qq, j2, fz) + a3 * mg_r(bb, qq, j3, fz);
+    }
+  }
+  for (; e < e1; ++e) {
+    const int64_t j = S.scol[e];
+    const float a = S.sval[e];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) acc[l] += a * mg_r(bl + l * n, ql + l * n, j, fz);
+  }
+#pragma unroll
+  for (int l = 0; l < LG; ++l) {
+    const int64_t el = (r0 + l) * n + i;
+    const float dv = mode == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
+    d[el] = dv;
+    x[el] = mode == 0 ? dv : x[el] + dv;
+  }
+}
+
 // ------------------------------------------------- transfer, coarsest ----
 __global__ void k_mr_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
                               GridDev gf, float* __restrict__ bc, GridDev gc, int R,
@@ -234,25 +310,41 @@ __global__ void k_mr_dense(const float* __restrict__ cinv, const float* __restri
 }
 
 // --------------------------------------------------------- outer loop ----
-// Per-block partial sums of `np` products per lane, written part[blk * np *
-// R + k * R + r]; reduced in fixed order by a one-block kernel.
-template <int R>
-__device__ inline void mr_block_partials(const double* v, int np, double* part) {
-  __shared__ double sh[4];
-  for (int k = 0; k < np; ++k)
-    for (int r = 0; r < R; ++r) {
-      const double s = block_sum(v[k * R + r], sh);
-      if (threadIdx.x == 0) part[((int64_t)blockIdx.x * np + k) * R + r] = s;
-    }
+// Per-block partial sums of NP products per lane (256-thread blocks),
+// written part[(k * R + r) * gridDim.x + blk]: every value is reduced per
+// wavefront first, then the four wavefronts' sums are added in index order
+// after one barrier.
+template <int R, int NP>
+__device__ inline void mr_block_partials(const double* v, double* part) {
+  __shared__ double sh[4][NP * R];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NP * R; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) sh[wid][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP * R) {
+    const int k = threadIdx.x;
+    part[(int64_t)k * gridDim.x + blockIdx.x] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+  }
 }
 // Sum of the nblk partials of product k of lane r, by one wavefront (fixed
-// order: strided lane sums, then the butterfly); result in every lane.
+// order: four strided lane sums, then the butterfly); result in every lane.
 template <int R>
-__device__ inline double mr_wave_reduce(const double* part, int nblk, int np, int k, int r) {
+__device__ inline double mr_wave_reduce(const double* part, int nblk, int k, int r) {
   const int lane = threadIdx.x & 63;
-  double v = 0.0;
-  for (int b = lane; b < nblk; b += 64) v += part[((int64_t)b * np + k) * R + r];
-  return wave_sum(v);
+  const double* p = part + (int64_t)(k * R + r) * nblk;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int b = lane;
+  for (; b + 192 < nblk; b += 256) {
+    v0 += p[b];
+    v1 += p[b + 64];
+    v2 += p[b + 128];
+    v3 += p[b + 192];
+  }
+  for (; b < nblk; b += 64) v0 += p[b];
+  return wave_sum((v0 + v1) + (v2 + v3));
 }
 
 // x = 0 ; r = rhs ; rf = (float) r ; partials of r.(minv r)
@@ -276,7 +368,7 @@ __global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
       acc[r] += v * (m * v);
     }
   }
-  mr_block_partials<R>(acc, 1, part);
+  mr_block_partials<R, 1>(acc, part);
 }
 
 // one block of R wavefronts, wavefront r for lane r
@@ -287,7 +379,7 @@ __global__ __launch_bounds__(64 * R) void k_mr_init_fin(const double* __restrict
                                                         int* __restrict__ all_done) {
   __shared__ int done[R];
   const int r = threadIdx.x >> 6;
-  const double s = mr_wave_reduce<R>(part, nblk, 1, 0, r);
+  const double s = mr_wave_reduce<R>(part, nblk, 0, r);
   if ((threadIdx.x & 63) == 0) {
     MrCtl c = {};
     c.bnorm = sqrt(s);
@@ -326,7 +418,7 @@ __global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
       acc[r] += zv * rr[r * n + i];
       acc[R + r] += zv * q[r * n + i];
     }
-  mr_block_partials<R>(acc, 2, part);
+  mr_block_partials<R, 2>(acc, part);
 }
 
 // rho = z.r ; beta = (z.r - z.r_prev) / rho_prev = -alpha_prev z.q / rho_prev
@@ -337,8 +429,8 @@ __global__ __launch_bounds__(64 * R) void k_mr_dots_fin(const double* __restrict
                                                         const int* __restrict__ all_done) {
   if (*all_done) return;
   const int r = threadIdx.x >> 6;
-  const double a = mr_wave_reduce<R>(part, nblk, 2, 0, r);
-  const double b = mr_wave_reduce<R>(part, nblk, 2, 1, r);
+  const double a = mr_wave_reduce<R>(part, nblk, 0, r);
+  const double b = mr_wave_reduce<R>(part, nblk, 1, r);
   if ((threadIdx.x & 63) == 0 && !ctl[r].done) {
     ctl[r].beta = ctl[r].iter == 0 ? 0.0 : -ctl[r].alpha * b / ctl[r].rho_prev;
     ctl[r].rho = a;
@@ -365,31 +457,20 @@ __global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, in
     }
 }
 
-// partials of p.q per lane
+// alpha = rho / p.q (0 on converged lanes), p.q summed from the stencil's
+// tpl work-item partials and the rows' nbu block partials of each lane
 template <int R>
-__global__ __launch_bounds__(256) void k_mr_pq(const double* __restrict__ p,
-                                               const double* __restrict__ q, int64_t n,
-                                               double* __restrict__ part,
-                                               const int* __restrict__ all_done) {
+__global__ __launch_bounds__(64 * R) void k_mr_alpha_st(const double* __restrict__ tpart, int tpl,
+                                                        const double* __restrict__ upart, int nbu,
+                                                        MrCtl* __restrict__ ctl,
+                                                        const int* __restrict__ all_done) {
   if (*all_done) return;
-  double acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] += p[r * n + i] * q[r * n + i];
-  mr_block_partials<R>(acc, 1, part);
-}
-
-// alpha = rho / p.q (0 on converged lanes)
-template <int R>
-__global__ __launch_bounds__(64 * R) void k_mr_alpha(const double* __restrict__ part, int nblk,
-                                                     MrCtl* __restrict__ ctl,
-                                                     const int* __restrict__ all_done) {
-  if (*all_done) return;
-  const int r = threadIdx.x >> 6;
-  const double pq = mr_wave_reduce<R>(part, nblk, 1, 0, r);
-  if ((threadIdx.x & 63) == 0) ctl[r].alpha = ctl[r].done ? 0.0 : ctl[r].rho / pq;
+  const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int b = lane; b < tpl; b += 64) v += tpart[(int64_t)r * tpl + b];
+  for (int b = lane; b < nbu; b += 64) v += upart[(int64_t)r * nbu + b];
+  const double pq = wave_sum(v);
+  if (lane == 0) ctl[r].alpha = ctl[r].done ? 0.0 : ctl[r].rho / pq;
 }
 
 // x += alpha p ; r -= alpha q ; rf = (float) r ; partials of r.(minv r)
@@ -421,7 +502,7 @@ __global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* 
       acc[r] += rn * (m * rn);
     }
   }
-  mr_block_partials<R>(acc, 1, part);
+  mr_block_partials<R, 1>(acc, part);
 }
 
 // rho_prev = rho ; iter++ ; stopping test on |S r| per lane
@@ -432,7 +513,7 @@ __global__ __launch_bounds__(64 * R) void k_mr_upd_fin(const double* __restrict_
   if (*all_done) return;
   __shared__ int done[R];
   const int r = threadIdx.x >> 6;
-  const double s = mr_wave_reduce<R>(part, nblk, 1, 0, r);
+  const double s = mr_wave_reduce<R>(part, nblk, 0, r);
   if ((threadIdx.x & 63) == 0) {
     MrCtl& c = ctl[r];
     if (!c.done) {
