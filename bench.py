@@ -319,6 +319,29 @@ def hbm_sweep(torch, _lib, fs):
     s2.close()
     del s2
     torch.cuda.empty_cache()
+    # what a plain device copy of the same size (x read, y written) reaches:
+    # the practical ceiling for a kernel that streams x in and y out
+    x = torch.rand(n2, device=_lib.device())
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        best = ms if best is None else min(best, ms)
+    cgbs = n2 * 8 / (best * 1e-3) / 1e9
+    res["copy_fp32"] = {"avg_us": round(best * 1e3, 1), "achieved_GBs": round(cgbs, 1),
+                        "frac": round(cgbs / HBM_PEAK_GBS, 4),
+                        "what": "torch copy_ of the 512^3 FP32 vector (read + write)"}
+    log(f"512^3 fp32 copy: {best * 1e3:.1f} us, {cgbs:.0f} GB/s")
+    del x, y
+    torch.cuda.empty_cache()
     return {"grid": "512^3 nodes, HBM-resident (x+y > 256 MB)", **res}
 
 

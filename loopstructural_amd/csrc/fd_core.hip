@@ -166,7 +166,8 @@ __device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __res
                    k[17] * r4[2] + k[18] * r4[3] + k[19] * r4[4] + k[20] * c + k[21] * xp2 +
                    k[22] * xm2 + k[23] * yp2 + k[24] * ym2;
       if (colok && z < zend) {
-        y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy] = yv;
+        // streaming store: y is not read again by this kernel
+        __builtin_nontemporal_store(yv, &y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy]);
         dot += (double)r2[0] * (double)yv;
       }
     }
@@ -243,7 +244,8 @@ __device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restr
                    o[17] * r4[2] + o[18] * r4[3] + o[19] * r4[4] + o[20] * c + o[21] * xp2 +
                    o[22] * xm2 + o[23] * yp2 + o[24] * ym2;
       if (colok) {
-        y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy] = yv;
+        // streaming store: y is not read again by this kernel
+        __builtin_nontemporal_store(yv, &y[(int64_t)ix + (int64_t)iy * g.nx + (int64_t)z * nxy]);
         dot += (double)r2[0] * (double)yv;
       }
     }
