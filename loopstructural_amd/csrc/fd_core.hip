@@ -2440,11 +2440,13 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   smooth(1);
   mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
   const MgLevel& C = m->lv[l + 1];
-  hipLaunchKernelGGL(k_mr_restrict, dim3((unsigned)ceil_div(C.n, 256), R), b256, 0, st,
-                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, R, ad);
+  constexpr int TG = 4;  // lanes per thread of the transfers
+  static_assert(R % TG == 0, "lane groups");
+  hipLaunchKernelGGL(k_mr_restrict_lg<TG>, dim3((unsigned)ceil_div(C.n, 256), R / TG), b256, 0, st,
+                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, ad);
   mr_vcycle<R>(s, w, l + 1, st);
-  hipLaunchKernelGGL(k_mr_prolong, nb, b256, 0, st, (const float*)w->x[l + 1], C.g, w->x[l], L.g,
-                     R, ad);
+  hipLaunchKernelGGL(k_mr_prolong_lg<TG>, dim3((unsigned)ceil_div(L.n, 256), R / TG), b256, 0, st,
+                     (const float*)w->x[l + 1], C.g, w->x[l], L.g, ad);
   mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
   smooth(0);
 }

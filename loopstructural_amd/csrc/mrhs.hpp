@@ -273,27 +273,79 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restric
 }
 
 // ------------------------------------------------- transfer, coarsest ----
-__global__ void k_mr_restrict(const float* __restrict__ bf, const float* __restrict__ qf,
-                              GridDev gf, float* __restrict__ bc, GridDev gc, int R,
-                              const int* __restrict__ all_done) {
+// Restriction and prolongation with LG lanes per thread (lane group in
+// blockIdx.y): the node's index arithmetic is done once for the LG lanes;
+// sums in the order of mg_restrict_node / mg_prolong_node
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_restrict_lg(const float* __restrict__ bf,
+                                                        const float* __restrict__ qf, GridDev gf,
+                                                        float* __restrict__ bc, GridDev gc,
+                                                        const int* __restrict__ all_done) {
   if (*all_done) return;
   const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
   const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nc) return;
-  const int64_t r = blockIdx.y;
-  bc[r * nc + id] = mg_restrict_node(bf + r * nf, qf + r * nf, gf, gc, id);
+  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  int I, J, K;
+  mg_ijk(id, gc.nx, gc.ny, I, J, K);
+  const int64_t fnxy = (int64_t)gf.nx * gf.ny;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int dk = -1; dk <= 1; ++dk) {
+    const int k = 2 * K + dk;
+    if (k < 0 || k >= gf.nz) continue;
+    const float wk = dk == 0 ? 1.f : 0.5f;
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j >= gf.ny) continue;
+      const float wj = wk * (dj == 0 ? 1.f : 0.5f);
+      for (int di = -1; di <= 1; ++di) {
+        const int i = 2 * I + di;
+        if (i < 0 || i >= gf.nx) continue;
+        const float w = wj * (di == 0 ? 1.f : 0.5f);
+        const int64_t f = (int64_t)i + (int64_t)j * gf.nx + (int64_t)k * fnxy;
+#pragma unroll
+        for (int l = 0; l < LG; ++l) {
+          const int64_t e = (r0 + l) * nf + f;
+          acc[l] += w * (bf[e] - qf[e]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < LG; ++l) bc[(r0 + l) * nc + id] = acc[l];
 }
 
-__global__ void k_mr_prolong(const float* __restrict__ xc, GridDev gc, float* __restrict__ xf,
-                             GridDev gf, int R, const int* __restrict__ all_done) {
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_prolong_lg(const float* __restrict__ xc, GridDev gc,
+                                                       float* __restrict__ xf, GridDev gf,
+                                                       const int* __restrict__ all_done) {
   if (*all_done) return;
   const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
   const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nf) return;
-  const int64_t r = blockIdx.y;
-  xf[r * nf + id] += mg_prolong_node(xc + r * nc, gc, gf, id);
+  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  int i, j, k;
+  mg_ijk(id, gf.nx, gf.ny, i, j, k);
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+  const float wi = ni == 2 ? 0.5f : 1.f, wj = nj == 2 ? 0.5f : 1.f, wk = nk == 2 ? 0.5f : 1.f;
+  const int64_t cnxy = (int64_t)gc.nx * gc.ny;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int c = 0; c < nk; ++c)
+    for (int b = 0; b < nj; ++b)
+      for (int a = 0; a < ni; ++a) {
+        const int64_t o = (int64_t)(i0 + a) + (int64_t)(j0 + b) * gc.nx + (int64_t)(k0 + c) * cnxy;
+#pragma unroll
+        for (int l = 0; l < LG; ++l) acc[l] += xc[(r0 + l) * nc + o];
+      }
+#pragma unroll
+  for (int l = 0; l < LG; ++l) xf[(r0 + l) * nf + id] += wi * wj * wk * acc[l];
 }
 
 // x = Cinv b on the coarsest level, one thread per (lane, row)
