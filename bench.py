@@ -202,6 +202,8 @@ def main():
                                   "copy-kernel calibrated)",
                 "kernel": f"regulariser stencil ({prec}): k_b_tiles<{prec}, zc>",
                 "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes": bytes_alg,
+                "profile": "profiles/stencil_roofline_kernel_trace.md (rocprofv3 per-dispatch "
+                           "durations of the same launches)",
                 "note": f"{nn}^3 working set is Infinity-Cache (MALL) resident; the HBM-bound "
                         f"figure is roofline_hbm_512",
                 "autotune": info_st}
