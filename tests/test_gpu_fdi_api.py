@@ -197,3 +197,24 @@ def test_support_helpers_match_oracle():
     assert np.array_equal(idc, ridc) and np.array_equal(ins, rins)
     assert np.array_equal(grid.position_to_dof_coefs(pts[rins]),
                           ofd.trilinear(ofd.local_coords(og, pts[rins])))
+
+
+def test_factory_round_trip_solves_to_the_same_minimiser():
+    """InterpolatorFactory.from_dict(to_dict()) (LI/_interpolator_factory.py:73-140):
+    the rebuilt interpolator, set up and solved again, reaches the minimiser."""
+    import loopstructural_amd as L
+
+    values, normals = _data(5)
+    interp = L.InterpolatorFactory.create_interpolator("FDI", support=_grid(12))
+    interp.set_value_constraints(values)
+    interp.set_normal_constraints(normals)
+    interp.setup_interpolator(regularisation=0.1)
+    assert interp.solve_system("cg", solver_kwargs={"rtol": 1e-12, "atol": 0.0})
+    back = L.InterpolatorFactory.from_dict(interp.to_dict())
+    assert back.up_to_date and np.array_equal(back.c, interp.c)
+    back.setup_interpolator(regularisation=0.1)
+    assert back.solve_system("cg", solver_kwargs={"rtol": 1e-12, "atol": 0.0})
+    _, ref = _oracle(12, {"value": values, "normal": normals}, regularisation=0.1)
+    cx = ofd.solve_exact(ref)
+    assert _relerr(back.c, cx) <= 1e-8
+    assert _relerr(back.c, interp.c) <= 1e-10

@@ -149,3 +149,44 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_factory_creates_and_rebuilds_interpolators():
+    """InterpolatorFactory (LI/_interpolator_factory.py:40-140): type strings,
+    support-given creation, refusals, and a to_dict -> from_dict round trip."""
+    from loopstructural_amd import InterpolatorFactory, InterpolatorType
+
+    grid = L.StructuredGrid(origin=[1.0, 2.0, 3.0], nsteps=[4, 5, 6], step_vector=[0.5, 0.5, 0.25])
+    it = InterpolatorFactory.create_interpolator("FDI", support=grid, solver="lsmr")
+    assert isinstance(it, L.FiniteDifferenceInterpolator) and it.solver == "lsmr"
+    mesh = L.TetMesh(origin=[0, 0, 0], nsteps_cells=[3, 4, 5], step_vector=[1, 1, 1])
+    for name in ("PLI", "P1", InterpolatorType.PIECEWISE_LINEAR, "PIECEWISE_LINEAR"):
+        assert isinstance(InterpolatorFactory.create_interpolator(name, support=mesh),
+                          L.P1Interpolator)
+    with pytest.raises(ValueError):
+        InterpolatorFactory.create_interpolator(None, support=grid)
+    with pytest.raises(ValueError):
+        InterpolatorFactory.create_interpolator("FDI")
+    with pytest.raises(NotImplementedError):
+        InterpolatorFactory.create_interpolator("P2", support=mesh)
+    with pytest.raises(NotImplementedError):
+        InterpolatorFactory.create_interpolator("FDI", boundingbox=object())
+
+    vals = np.array([[1.5, 2.5, 3.5, 2.0, 1.0], [2.0, 3.0, 4.0, -1.0, 0.5]])
+    nrms = np.array([[1.5, 2.5, 3.5, 0.0, 0.0, 1.0, 1.0]])
+    for interp in (L.FiniteDifferenceInterpolator(grid), L.P1Interpolator(
+            L.TetMesh(origin=[1.0, 2.0, 3.0], nsteps_cells=[4, 5, 6],
+                      step_vector=[0.5, 0.5, 0.25]))):
+        interp.set_value_constraints(vals)
+        interp.set_normal_constraints(nrms)
+        interp.c = np.arange(interp.dof, dtype=float)
+        d = interp.to_dict()
+        back = InterpolatorFactory.from_dict(d)
+        assert type(back) is type(interp)
+        assert np.array_equal(back.support.nsteps, interp.support.nsteps)
+        assert np.allclose(back.support.origin, interp.support.origin)
+        assert np.allclose(back.support.step_vector, interp.support.step_vector)
+        assert np.array_equal(back.data["value"], interp.data["value"])
+        assert np.array_equal(back.data["normal"], interp.data["normal"])
+        assert np.array_equal(back.c, interp.c)
+        assert back.up_to_date == interp.up_to_date
