@@ -9,13 +9,14 @@ from .interpolators import (DiscreteInterpolator, FiniteDifferenceInterpolator,
                             GeologicalInterpolator, InterpolatorType, P1Interpolator)
 from .supports import Operator, StructuredGrid, SupportType, TetMesh
 from .factory import (InterpolatorFactory, SupportFactory, interpolator_map,
-                      interpolator_string_map, support_map)
+                      interpolator_string_map, support_interpolator_map, support_map)
 
 __all__ = [
     "ConstraintDiagnosticsReport", "ConstraintFamilyDiagnostics", "DiscreteInterpolator",
     "DtypeError", "FiniteDifferenceInterpolator", "FiniteValueError", "GeologicalInterpolator",
     "InterpolatorFactory", "InterpolatorType", "Operator", "P1Interpolator", "RegionCoverageDiagnostics", "ShapeError", "StructuredGrid",
     "SupportFactory", "SupportType", "TetMesh", "UnsupportedCombinationError", "ValidationError", "VectorError",
-    "WeightError", "interpolator_map", "interpolator_string_map", "support_map",
+    "WeightError", "interpolator_map", "interpolator_string_map", "support_interpolator_map",
+    "support_map",
 ]
 __version__ = "0.1.0"

@@ -6,9 +6,11 @@ LI/_interpolator_factory.py:24-140: InterpolatorFactory;
 LC/supports/_support_factory.py:10-25: SupportFactory), restricted to the
 two interpolators built here: 'FDI' -> FiniteDifferenceInterpolator on a
 StructuredGrid and 'PLI' / 'P1' -> P1Interpolator on a TetMesh.  Other types
-are recognised and refused with NotImplementedError.  Supports are passed
-in; building one from a bounding box (SupportFactory.create_support_from_bbox)
-belongs to the model layer and is refused the same way.
+are recognised and refused with NotImplementedError.  A support is passed
+in, or built from the caller's bounding box object (the reference's
+BoundingBox or anything with its origin / maximum / nsteps / nelements /
+dimensions / project / with_buffer members), as
+SupportFactory.create_support_from_bbox does.
 """
 from __future__ import annotations
 
@@ -38,6 +40,12 @@ support_map = {
     SupportType.TetMesh: TetMesh,
 }
 
+# LI/__init__.py:119-126, 3-D entries of the two interpolators built here
+support_interpolator_map = {
+    InterpolatorType.FINITE_DIFFERENCE: {3: SupportType.StructuredGrid},
+    InterpolatorType.PIECEWISE_LINEAR: {3: SupportType.TetMesh},
+}
+
 
 class SupportFactory:
     """LC/supports/_support_factory.py:10-25 for StructuredGrid and TetMesh."""
@@ -60,6 +68,37 @@ class SupportFactory:
         if support_type is None:
             raise ValueError("No support type specified")
         return SupportFactory.create_support(support_type, **d)
+
+    @staticmethod
+    def create_support_from_bbox(support_type, bounding_box, nelements, element_volume=None,
+                                 buffer=None, local_coordinates=True):
+        """LC/supports/_support_factory.py:37-78: the box (buffered when asked)
+        takes the element count, its own nelements setter picks near-cubic
+        cells, and the support spans the box in its local frame (or in world
+        coordinates with the box's step vector)."""
+        if isinstance(support_type, str):
+            support_type = SupportType[support_type]
+        support_type = SupportType(int(support_type))
+        if support_type not in support_map:
+            raise NotImplementedError(f"support {support_type.name} is not part of this package")
+        if buffer is not None:
+            bounding_box = bounding_box.with_buffer(buffer=buffer)
+        if element_volume is not None:
+            length = np.asarray(bounding_box.maximum, float) - np.asarray(bounding_box.origin, float)
+            nelements = int(np.prod(length) / element_volume)
+        if nelements is not None:
+            bounding_box.nelements = nelements
+        nsteps = np.asarray(bounding_box.nsteps).astype(int)
+        if local_coordinates:
+            local = np.asarray(bounding_box.project(
+                np.array([bounding_box.origin, bounding_box.maximum], dtype=float)), dtype=float)
+            origin = local.min(axis=0)
+            step_vector = (local.max(axis=0) - origin) / nsteps
+        else:
+            origin = np.asarray(bounding_box.origin, dtype=float)
+            step_vector = np.asarray(bounding_box.step_vector, dtype=float)
+        return support_map[support_type](origin=origin, step_vector=step_vector,
+                                         nsteps_cells=nsteps)
 
 
 class InterpolatorFactory:
@@ -88,8 +127,13 @@ class InterpolatorFactory:
                 f"{interpolatortype.name} has no GPU implementation in this package "
                 f"(available: {', '.join(t.name for t in interpolator_map)})")
         if support is None:
-            raise NotImplementedError(
-                "building a support from a bounding box is outside this package; pass support=")
+            supporttype = support_interpolator_map[interpolatortype].get(
+                int(getattr(boundingbox, "dimensions", 3)))
+            if supporttype is None:
+                raise NotImplementedError("only 3-D supports are part of this package")
+            support = SupportFactory.create_support_from_bbox(
+                supporttype, bounding_box=boundingbox, nelements=nelements,
+                element_volume=element_volume, buffer=buffer)
         interpolator = interpolator_map[interpolatortype](support)
         if boundingbox is not None:
             interpolator.bounding_box = boundingbox

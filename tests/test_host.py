@@ -169,8 +169,10 @@ def test_factory_creates_and_rebuilds_interpolators():
         InterpolatorFactory.create_interpolator("FDI")
     with pytest.raises(NotImplementedError):
         InterpolatorFactory.create_interpolator("P2", support=mesh)
+    flat = _Box([0, 0, 0], [1, 1, 1])
+    flat.dimensions = 2
     with pytest.raises(NotImplementedError):
-        InterpolatorFactory.create_interpolator("FDI", boundingbox=object())
+        InterpolatorFactory.create_interpolator("FDI", boundingbox=flat, nelements=100)
 
     vals = np.array([[1.5, 2.5, 3.5, 2.0, 1.0], [2.0, 3.0, 4.0, -1.0, 0.5]])
     nrms = np.array([[1.5, 2.5, 3.5, 0.0, 0.0, 1.0, 1.0]])
@@ -190,3 +192,52 @@ def test_factory_creates_and_rebuilds_interpolators():
         assert np.array_equal(back.data["normal"], interp.data["normal"])
         assert np.array_equal(back.c, interp.c)
         assert back.up_to_date == interp.up_to_date
+
+
+class _Box:
+    """Minimal stand-in for a caller's bounding box: translation-only local
+    frame (project subtracts the origin), a settable element count that
+    picks cubic cells, and a buffered copy."""
+
+    def __init__(self, origin, maximum):
+        self.origin = np.asarray(origin, float)
+        self.maximum = np.asarray(maximum, float)
+        self.nsteps = np.array([2, 2, 2])
+        self.dimensions = 3
+
+    @property
+    def nelements(self):
+        return int(np.prod(self.nsteps))
+
+    @nelements.setter
+    def nelements(self, n):
+        h = (np.prod(self.maximum - self.origin) / n) ** (1 / 3)
+        self.nsteps = np.ceil((self.maximum - self.origin) / h).astype(int)
+
+    def project(self, pts):
+        return np.asarray(pts, float) - self.origin
+
+    def with_buffer(self, buffer=0.2):
+        pad = (self.maximum - self.origin) * buffer
+        return _Box(self.origin - pad, self.maximum + pad)
+
+
+@pytest.mark.parametrize("kind, cls", [("FDI", "StructuredGrid"), ("PLI", "TetMesh")])
+def test_factory_builds_support_from_bounding_box(kind, cls):
+    from loopstructural_amd import InterpolatorFactory
+
+    box = _Box([10.0, -5.0, 2.0], [14.0, -3.0, 3.0])
+    it = InterpolatorFactory.create_interpolator(kind, boundingbox=box, nelements=4000)
+    sup = it.support
+    assert type(sup).__name__ == cls and it.bounding_box is box
+    # the support spans the box in its local frame, with the box's cell counts
+    assert np.allclose(sup.origin, 0.0)
+    assert np.allclose(sup.maximum, box.maximum - box.origin)
+    cells = np.asarray(sup.nsteps) - 1
+    assert np.array_equal(cells, box.nsteps)
+    assert 4000 <= np.prod(cells) <= 4000 * 1.6
+    # element_volume sets the count; buffer enlarges the box, not the caller's
+    it2 = InterpolatorFactory.create_interpolator(kind, boundingbox=box, element_volume=0.01,
+                                                  buffer=0.1)
+    assert np.allclose(it2.support.maximum - it2.support.origin, 1.2 * (box.maximum - box.origin))
+    assert it2.bounding_box is box
