@@ -2431,10 +2431,11 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
     }
   };
   auto smooth = [&](int from_zero) {
-    step(from_zero ? 0 : 1, 0.f, 0.f);
-    for (int k = 1; k < (l == 0 ? m->deg0 : m->deg); ++k) {
+    const int deg = l == 0 ? m->deg0 : m->deg;
+    step((from_zero ? 0 : 1) | (deg == 1 ? kMrLast : 0), 0.f, 0.f);
+    for (int k = 1; k < deg; ++k) {
       mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
-      step(2, L.c1[k], L.c2[k]);
+      step(2 | (k == deg - 1 ? kMrLast : 0), L.c1[k], L.c2[k]);
     }
   };
   smooth(1);

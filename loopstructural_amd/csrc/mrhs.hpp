@@ -184,8 +184,11 @@ __global__ __launch_bounds__(256) void k_mr_apply_coarse_all(MgDev L,
 // ----------------------------------------------------------- smoother ----
 // Chebyshev step with the Schwarz preconditioner M: mode 0 first step from
 // zero (d = c0 M b, x = d), 1 first step (d = c0 M (b - q), x += d), 2 later
-// step (d = c1 d + c2 M (b - q), x += d).  skip_cov (large levels): the
-// covered nodes are left to k_mr_cheb_cov.
+// step (d = c1 d + c2 M (b - q), x += d); mode | kMrLast: the last step of
+// the smoothing, whose d is dead and not stored.  skip_cov (large levels):
+// the covered nodes are left to k_mr_cheb_cov.
+constexpr int kMrLast = 4;
+
 __global__ __launch_bounds__(256) void k_mr_cheb(const float* __restrict__ b,
                                                  const float* __restrict__ q, MgSm S,
                                                  float* __restrict__ x, float* __restrict__ d,
@@ -197,10 +200,11 @@ __global__ __launch_bounds__(256) void k_mr_cheb(const float* __restrict__ b,
   if (i >= n) return;
   if (skip_cov && S.sm[i] < 0.f) return;
   const int64_t r = blockIdx.y, e = r * n + i;
-  const float z = mg_precond(i, S, b + r * n, q + r * n, mode == 0);
-  const float dv = mode == 2 ? c1 * d[e] + c2 * z : c0 * z;
-  d[e] = dv;
-  x[e] = mode == 0 ? dv : x[e] + dv;
+  const int m = mode & 3;
+  const float z = mg_precond(i, S, b + r * n, q + r * n, m == 0);
+  const float dv = m == 2 ? c1 * d[e] + c2 * z : c0 * z;
+  if (!(mode & kMrLast)) d[e] = dv;
+  x[e] = m == 0 ? dv : x[e] + dv;
 }
 
 // the same step for the covered nodes of a large level (compact list), one
@@ -216,10 +220,11 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov(const int32_t* __restrict__
   if (t >= ncov) return;
   const int64_t i = cov[t];
   const int64_t r = blockIdx.y, e = r * n + i;
-  const float z = mg_precond(i, S, b + r * n, q + r * n, mode == 0);
-  const float dv = mode == 2 ? c1 * d[e] + c2 * z : c0 * z;
-  d[e] = dv;
-  x[e] = mode == 0 ? dv : x[e] + dv;
+  const int m = mode & 3;
+  const float z = mg_precond(i, S, b + r * n, q + r * n, m == 0);
+  const float dv = m == 2 ? c1 * d[e] + c2 * z : c0 * z;
+  if (!(mode & kMrLast)) d[e] = dv;
+  x[e] = m == 0 ? dv : x[e] + dv;
 }
 
 // the same step for LG lanes per thread: the covered node's row of summed
@@ -240,7 +245,8 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restric
   const int64_t r0 = (int64_t)blockIdx.y * LG;
   const float* bl = b + r0 * n;
   const float* ql = q + r0 * n;
-  const int fz = mode == 0;
+  const int m = mode & 3;
+  const int fz = m == 0;
   float acc[LG];
 #pragma unroll
   for (int l = 0; l < LG; ++l) acc[l] = 0.f;
@@ -266,9 +272,9 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restric
 #pragma unroll
   for (int l = 0; l < LG; ++l) {
     const int64_t el = (r0 + l) * n + i;
-    const float dv = mode == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
-    d[el] = dv;
-    x[el] = mode == 0 ? dv : x[el] + dv;
+    const float dv = m == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
+    if (!(mode & kMrLast)) d[el] = dv;
+    x[el] = m == 0 ? dv : x[el] + dv;
   }
 }
 
