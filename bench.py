@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--values", type=int, default=5000)
     ap.add_argument("--normals", type=int, default=1000)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--batch", type=int, default=16, help="realisations in the batched form")
+    ap.add_argument("--batch", type=int, default=32, help="realisations in the batched form")
     ap.add_argument("--ensemble", type=int, default=256, help="systems in the uneven ensemble")
     ap.add_argument("--forms", default="plain,mixed,mg,batched")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -2457,7 +2457,7 @@ void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
   const int64_t n = s->n;
   const int vb = vec_blocks(n);
   const int* ad = w->all_done;
-  const dim3 b256(256), fin(64 * R);
+  const dim3 b256(256), fin(mr_fin_threads(R));
   mr_vcycle<R>(s, w, 0, st);  // from w->b[0] = (float) r
   hipLaunchKernelGGL(k_mr_dots<R>, dim3(vb), b256, 0, st, (const float*)w->x[0],
                      (const double*)w->r64, (const double*)w->q64, n, w->part, ad);
@@ -2497,7 +2497,7 @@ int mr_solve_group(lsamd_fd_system* s, MrWork* w, const lsamd_solve_opts* o, int
                      R, n, w->rhs);
   hipLaunchKernelGGL(k_mr_init<R>, dim3(vb), dim3(256), 0, st, (const double*)w->rhs,
                      (const double*)s->minv64, w->x64, w->r64, w->b[0], n, w->part);
-  hipLaunchKernelGGL(k_mr_init_fin<R>, dim3(1), dim3(64 * R), 0, st, (const double*)w->part, vb,
+  hipLaunchKernelGGL(k_mr_init_fin<R>, dim3(1), dim3(mr_fin_threads(R)), 0, st, (const double*)w->part, vb,
                      nr, o->rtol, o->atol, maxiter, w->ctl, w->all_done);
   LSAMD_CHECK_LAUNCH("multi-rhs init");
   if (w->graph && w->graph_chunk != chunk) mr_invalidate(w);
@@ -2558,15 +2558,18 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
   }
   for (int r0 = 0; r0 < nrhs;) {
     const int left = nrhs - r0;
-    const int R = left <= 4 ? 4 : (left <= 8 ? 8 : 16);
+    // lanes per group: 32 only when at least 24 are left (an idle lane costs
+    // as much as a busy one)
+    const int R = left <= 4 ? 4 : (left <= 8 ? 8 : (left < 24 ? 16 : 32));
     const int nr = std::min(left, R);
     MrWork* w = nullptr;
     if (int rc = mr_ensure(s, R, &w)) return rc;
     const double* rr = rhs + (int64_t)r0 * n;
     double* cc = c + (int64_t)r0 * n;
-    const int rc = R == 4   ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
-                   : R == 8 ? mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st)
-                            : mr_solve_group<16>(s, w, o, nr, rr, cc, infos + r0, st);
+    const int rc = R == 4    ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
+                   : R == 8  ? mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st)
+                   : R == 16 ? mr_solve_group<16>(s, w, o, nr, rr, cc, infos + r0, st)
+                             : mr_solve_group<32>(s, w, o, nr, rr, cc, infos + r0, st);
     if (rc) return rc;
     r0 += nr;
   }

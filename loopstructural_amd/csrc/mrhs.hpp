@@ -429,25 +429,29 @@ __global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
   mr_block_partials<R, 1>(acc, part);
 }
 
-// one block of R wavefronts, wavefront r for lane r
+// The one-block kernels below run min(R, 16) wavefronts; wavefront w
+// serves lanes w, w + nw, ...
+constexpr int mr_fin_threads(int R) { return 64 * (R < 16 ? R : 16); }
+
 template <int R>
-__global__ __launch_bounds__(64 * R) void k_mr_init_fin(const double* __restrict__ part, int nblk,
+__global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_init_fin(const double* __restrict__ part, int nblk,
                                                         int nr, double rtol, double atol,
                                                         int maxiter, MrCtl* __restrict__ ctl,
                                                         int* __restrict__ all_done) {
   __shared__ int done[R];
-  const int r = threadIdx.x >> 6;
-  const double s = mr_wave_reduce<R>(part, nblk, 0, r);
-  if ((threadIdx.x & 63) == 0) {
-    MrCtl c = {};
-    c.bnorm = sqrt(s);
-    c.thr = fmax(atol, rtol * c.bnorm);
-    c.res2 = s;
-    c.maxiter = maxiter;
-    c.converged = r < nr ? (sqrt(s) < c.thr || s == 0.0) : 1;
-    c.done = c.converged || maxiter <= 0;
-    ctl[r] = c;
-    done[r] = c.done;
+  for (int r = threadIdx.x >> 6; r < R; r += blockDim.x >> 6) {
+    const double s = mr_wave_reduce<R>(part, nblk, 0, r);
+    if ((threadIdx.x & 63) == 0) {
+      MrCtl c = {};
+      c.bnorm = sqrt(s);
+      c.thr = fmax(atol, rtol * c.bnorm);
+      c.res2 = s;
+      c.maxiter = maxiter;
+      c.converged = r < nr ? (sqrt(s) < c.thr || s == 0.0) : 1;
+      c.done = c.converged || maxiter <= 0;
+      ctl[r] = c;
+      done[r] = c.done;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -482,16 +486,18 @@ __global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
 // rho = z.r ; beta = (z.r - z.r_prev) / rho_prev = -alpha_prev z.q / rho_prev
 // (0 on the first step)
 template <int R>
-__global__ __launch_bounds__(64 * R) void k_mr_dots_fin(const double* __restrict__ part, int nblk,
-                                                        MrCtl* __restrict__ ctl,
-                                                        const int* __restrict__ all_done) {
+__global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_dots_fin(const double* __restrict__ part,
+                                                                   int nblk,
+                                                                   MrCtl* __restrict__ ctl,
+                                                                   const int* __restrict__ all_done) {
   if (*all_done) return;
-  const int r = threadIdx.x >> 6;
-  const double a = mr_wave_reduce<R>(part, nblk, 0, r);
-  const double b = mr_wave_reduce<R>(part, nblk, 1, r);
-  if ((threadIdx.x & 63) == 0 && !ctl[r].done) {
-    ctl[r].beta = ctl[r].iter == 0 ? 0.0 : -ctl[r].alpha * b / ctl[r].rho_prev;
-    ctl[r].rho = a;
+  for (int r = threadIdx.x >> 6; r < R; r += blockDim.x >> 6) {
+    const double a = mr_wave_reduce<R>(part, nblk, 0, r);
+    const double b = mr_wave_reduce<R>(part, nblk, 1, r);
+    if ((threadIdx.x & 63) == 0 && !ctl[r].done) {
+      ctl[r].beta = ctl[r].iter == 0 ? 0.0 : -ctl[r].alpha * b / ctl[r].rho_prev;
+      ctl[r].rho = a;
+    }
   }
 }
 
@@ -518,17 +524,18 @@ __global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, in
 // alpha = rho / p.q (0 on converged lanes), p.q summed from the stencil's
 // tpl work-item partials and the rows' nbu block partials of each lane
 template <int R>
-__global__ __launch_bounds__(64 * R) void k_mr_alpha_st(const double* __restrict__ tpart, int tpl,
-                                                        const double* __restrict__ upart, int nbu,
-                                                        MrCtl* __restrict__ ctl,
-                                                        const int* __restrict__ all_done) {
+__global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_alpha_st(
+    const double* __restrict__ tpart, int tpl, const double* __restrict__ upart, int nbu,
+    MrCtl* __restrict__ ctl, const int* __restrict__ all_done) {
   if (*all_done) return;
-  const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double v = 0.0;
-  for (int b = lane; b < tpl; b += 64) v += tpart[(int64_t)r * tpl + b];
-  for (int b = lane; b < nbu; b += 64) v += upart[(int64_t)r * nbu + b];
-  const double pq = wave_sum(v);
-  if (lane == 0) ctl[r].alpha = ctl[r].done ? 0.0 : ctl[r].rho / pq;
+  const int lane = threadIdx.x & 63;
+  for (int r = threadIdx.x >> 6; r < R; r += blockDim.x >> 6) {
+    double v = 0.0;
+    for (int b = lane; b < tpl; b += 64) v += tpart[(int64_t)r * tpl + b];
+    for (int b = lane; b < nbu; b += 64) v += upart[(int64_t)r * nbu + b];
+    const double pq = wave_sum(v);
+    if (lane == 0) ctl[r].alpha = ctl[r].done ? 0.0 : ctl[r].rho / pq;
+  }
 }
 
 // x += alpha p ; r -= alpha q ; rf = (float) r ; partials of r.(minv r)
@@ -565,23 +572,24 @@ __global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* 
 
 // rho_prev = rho ; iter++ ; stopping test on |S r| per lane
 template <int R>
-__global__ __launch_bounds__(64 * R) void k_mr_upd_fin(const double* __restrict__ part, int nblk,
-                                                       MrCtl* __restrict__ ctl,
-                                                       int* __restrict__ all_done) {
+__global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_upd_fin(const double* __restrict__ part,
+                                                                  int nblk, MrCtl* __restrict__ ctl,
+                                                                  int* __restrict__ all_done) {
   if (*all_done) return;
   __shared__ int done[R];
-  const int r = threadIdx.x >> 6;
-  const double s = mr_wave_reduce<R>(part, nblk, 0, r);
-  if ((threadIdx.x & 63) == 0) {
-    MrCtl& c = ctl[r];
-    if (!c.done) {
-      c.rho_prev = c.rho;
-      c.iter += 1;
-      c.res2 = s;
-      c.converged = sqrt(s) < c.thr;
-      c.done = c.converged || c.iter >= c.maxiter;
+  for (int r = threadIdx.x >> 6; r < R; r += blockDim.x >> 6) {
+    const double s = mr_wave_reduce<R>(part, nblk, 0, r);
+    if ((threadIdx.x & 63) == 0) {
+      MrCtl& c = ctl[r];
+      if (!c.done) {
+        c.rho_prev = c.rho;
+        c.iter += 1;
+        c.res2 = s;
+        c.converged = sqrt(s) < c.thr;
+        c.done = c.converged || c.iter >= c.maxiter;
+      }
+      done[r] = c.done;
     }
-    done[r] = c.done;
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -57,7 +57,7 @@ def test_rhs_for_matches_set_rows():
     assert float(torch.linalg.vector_norm(rhs[1] - ref) / torch.linalg.vector_norm(ref)) <= 1e-14
 
 
-@pytest.mark.parametrize("nreal", [3, 8, 11, 21])
+@pytest.mark.parametrize("nreal", [3, 8, 11, 21, 27])
 def test_multi_rhs_lanes_match_single_solves(nreal):
     s, rhs, reals, g, n = _setup(23, nreal)
     c, infos = s.solve_multi(rhs, rtol=1e-12)
