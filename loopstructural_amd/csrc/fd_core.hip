@@ -2067,18 +2067,20 @@ void mg_vcycle(Mg* m, int l, const CgCtl* ctl, hipStream_t st) {
   auto smooth = [&](int from_zero) {
     hipLaunchKernelGGL(k_mg_cheb_first, nb, dim3(256), 0, st, (const float*)L.b,
                        (const float*)L.q, S, L.x, L.d, L.n, L.c0, from_zero, split, ctl);
+    const int deg = l == 0 ? m->deg0 : m->deg;
     if (split && L.ncov > 0)
       hipLaunchKernelGGL(k_mg_cheb_cov, ncb, dim3(256), 0, st, (const int32_t*)L.cov, L.ncov,
                          (const float*)L.b, (const float*)L.q, S, L.x, L.d, from_zero ? 0 : 1,
-                         L.c0, 0.f, 0.f, ctl);
-    for (int k = 1; k < (l == 0 ? m->deg0 : m->deg); ++k) {
+                         L.c0, 0.f, 0.f, 0, ctl);
+    for (int k = 1; k < deg; ++k) {
+      const int last = k == deg - 1;
       mg_level_apply(L, L.x, L.q, ctl, st);
       hipLaunchKernelGGL(k_mg_cheb_step, nb, dim3(256), 0, st, (const float*)L.b,
-                         (const float*)L.q, S, L.x, L.d, L.n, L.c1[k], L.c2[k], split, ctl);
+                         (const float*)L.q, S, L.x, L.d, L.n, L.c1[k], L.c2[k], split, last, ctl);
       if (split && L.ncov > 0)
         hipLaunchKernelGGL(k_mg_cheb_cov, ncb, dim3(256), 0, st, (const int32_t*)L.cov, L.ncov,
                            (const float*)L.b, (const float*)L.q, S, L.x, L.d, 2, 0.f, L.c1[k],
-                           L.c2[k], ctl);
+                           L.c2[k], last, ctl);
     }
   };
   smooth(1);

@@ -86,32 +86,33 @@ __global__ void k_mg_cheb_first(const float* __restrict__ b, const float* __rest
   x[i] = from_zero ? dv : x[i] + dv;
 }
 
-// later steps: d = c1 d + c2 M (b - q), x += d
+// later steps: d = c1 d + c2 M (b - q), x += d; last: the smoothing's final
+// step, whose d is dead and not stored
 __global__ void k_mg_cheb_step(const float* __restrict__ b, const float* __restrict__ q,
                                MgSm S, float* __restrict__ x, float* __restrict__ d, int64_t n,
-                               float c1, float c2, int skip_cov, const CgCtl* ctl) {
+                               float c1, float c2, int skip_cov, int last, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (skip_cov && S.sm[i] < 0.f) return;
   const float dv = c1 * d[i] + c2 * mg_precond(i, S, b, q, 0);
-  d[i] = dv;
+  if (!last) d[i] = dv;
   x[i] += dv;
 }
 
 // the covered nodes of a level (compact list): mode 0 first step from zero,
-// 1 first step, 2 later step (d = c1 d + c2 M r)
+// 1 first step, 2 later step (d = c1 d + c2 M r); last as k_mg_cheb_step
 __global__ void k_mg_cheb_cov(const int32_t* __restrict__ cov, int64_t ncov,
                               const float* __restrict__ b, const float* __restrict__ q, MgSm S,
                               float* __restrict__ x, float* __restrict__ d, int mode, float c0,
-                              float c1, float c2, const CgCtl* ctl) {
+                              float c1, float c2, int last, const CgCtl* ctl) {
   if (ctl->done) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncov) return;
   const int64_t i = cov[t];
   const float z = mg_precond(i, S, b, q, mode == 0);
   const float dv = mode == 2 ? c1 * d[i] + c2 * z : c0 * z;
-  d[i] = dv;
+  if (!last) d[i] = dv;
   x[i] = mode == 0 ? dv : x[i] + dv;
 }
 
