@@ -221,6 +221,120 @@ def test_value_and_normal_rows_closed_form():
         assert np.isclose(s.constraints[f"norm_{d}"]["w"][0], 10.0 / 3 * 2.0)
 
 
+# Rows built from shape-function gradients are pinned through their first
+# moments: for a trilinear element, sum_k dN_k/dx_d X_k = e_d exactly, so a row
+# a = v.T (normalised) maps the node coordinates X to v / |v.T| and sums to 0.
+# The direction v comes from an independent closed form below; a wrong sign,
+# a swapped axis or a strike/dip mix-up changes that direction.
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _closed_form_strike_dip(v):
+    """Strike = (n_y, -n_x, 0)/|.| ((0,-1,0) for a vertical normal) and
+    dip = strike x n normalised, n = v/|v| (LC/math/_maths.py:126-160,
+    :276-332 reduce to these)."""
+    n = _unit(v)
+    r = np.hypot(n[:, 0], n[:, 1])
+    s = np.zeros_like(n)
+    s[:, 1] = -1.0
+    m = r > 0
+    s[m] = np.column_stack([n[m, 1], -n[m, 0], np.zeros(m.sum())]) / r[m, None]
+    return s, _unit(np.cross(s, n))
+
+
+def _grid_and_points(seed, n):
+    g = ofd.Grid((0.5, -1.0, 0.2), (5, 4, 6), (0.3, 0.5, 0.2))
+    rng = np.random.default_rng(seed)
+    p = g.origin + rng.uniform(0.02, 0.98, (n, 3)) * g.ncells * g.step
+    return g, p, rng
+
+
+def test_gradient_rows_are_strike_and_dip_derivatives():
+    """LI/_finite_difference_interpolator.py:342-401: two families
+    ('gradient' = strike, then 'gradient_0' = dip), rows (v.T)/|v.T| with
+    the input vector's length cancelled, b = 0, weight gpw."""
+    g, p, rng = _grid_and_points(21, 40)
+    v = rng.standard_normal((40, 3)) * rng.uniform(0.2, 5.0, (40, 1))
+    v[0] = [0.0, 0.0, 2.5]
+    v[1] = [0.0, 0.0, -0.3]
+    v[2] = [1.0, 0.0, 0.0]
+    s = ofd.assemble_fd(g, {"gradient": np.column_stack([p, v, np.ones(40)])},
+                        weights={"gpw": 3.0})
+    assert list(s.constraints)[:2] == ["gradient", "gradient_0"]
+    strike, dip = _closed_form_strike_dip(v)
+    T = ofd.element_gradient(g, p)
+    X = g.nodes()
+    for fam, vec in (("gradient", strike), ("gradient_0", dip)):
+        M = s.constraints[fam]["matrix"]
+        mom = M @ X
+        assert np.allclose(_unit(mom), vec, atol=1e-12)
+        # scale 1/|v.T| (T pinned independently), constants annihilated
+        tn = np.linalg.norm(np.einsum("nd,ndk->nk", vec, T), axis=1)
+        assert np.allclose(np.linalg.norm(mom, axis=1) * tn, 1.0, rtol=1e-12)
+        assert np.allclose(M @ np.ones(g.n_nodes), 0.0, atol=1e-13)
+        assert np.allclose(sparse.linalg.norm(M, axis=1), 1.0)
+        assert np.all(s.constraints[fam]["b"] == 0)
+        assert np.all(s.constraints[fam]["w"] == 3.0)
+    # a linear field whose gradient is parallel to the normal satisfies both rows
+    f = X @ v[5]
+    assert np.allclose(s.constraints["gradient"]["matrix"][5] @ f, 0.0, atol=1e-12)
+    assert np.allclose(s.constraints["gradient_0"]["matrix"][5] @ f, 0.0, atol=1e-12)
+
+
+def test_tangent_rows_and_zero_vectors():
+    """LI/_finite_difference_interpolator.py:478-539: one row t_hat.T/|t_hat.T|,
+    b = 0, weight tpw; a zero vector gives an all-zero row (normalisation
+    skips it, LI/_discrete_interpolator.py:361-366)."""
+    g, p, rng = _grid_and_points(22, 30)
+    v = rng.standard_normal((30, 3)) * rng.uniform(0.2, 5.0, (30, 1))
+    v[3] = 0.0
+    s = ofd.assemble_fd(g, {"tangent": np.column_stack([p, v, np.ones(30)])},
+                        weights={"tpw": 0.7})
+    M = s.constraints["gradient orthogonal"]["matrix"]
+    mom = M @ g.nodes()
+    ok = np.ones(30, dtype=bool)
+    ok[3] = False
+    assert np.allclose(_unit(mom[ok]), _unit(v[ok]), atol=1e-12)
+    T = ofd.element_gradient(g, p)
+    tn = np.linalg.norm(np.einsum("nd,ndk->nk", _unit(v[ok]), T[ok]), axis=1)
+    assert np.allclose(np.linalg.norm(mom[ok], axis=1) * tn, 1.0, rtol=1e-12)
+    assert M[3].nnz == 0 or np.all(M[3].toarray() == 0)
+    assert np.allclose(M @ np.ones(g.n_nodes), 0.0, atol=1e-13)
+    assert np.all(s.constraints["gradient orthogonal"]["w"] == 0.7)
+
+
+@pytest.mark.parametrize("n0,n1", [(7, 3), (200, 201)])
+def test_interface_rows_pair_differences(n0, n1):
+    """LI/_finite_difference_interpolator.py:271-340: per id, rows
+    [N_i, -N_j]/|.| over the pairs i<j of that id when it has at most 200
+    points, else the star (0, j); b = 0, weight ipw; a NaN id and a lone
+    point add nothing."""
+    g, p, rng = _grid_and_points(23, n0 + n1 + 2)
+    ids = np.concatenate([np.zeros(n0), np.ones(n1), [np.nan, 5.0]])
+    s = ofd.assemble_fd(g, {"interface": np.column_stack([p, ids, np.ones(len(ids))])},
+                        weights={"ipw": 2.0})
+    names = [k for k in s.constraints if k.startswith("interface")]
+    assert names == ["interface_0.0", "interface_1.0"]
+    X = g.nodes()
+    W = ofd.trilinear(ofd.local_coords(g, p))
+    for name, sel, n in (("interface_0.0", ids == 0, n0), ("interface_1.0", ids == 1, n1)):
+        if n > 200:
+            i, j = np.zeros(n - 1, dtype=int), np.arange(1, n)
+        else:
+            i, j = np.triu_indices(n, k=1)
+        M = s.constraints[name]["matrix"]
+        assert M.shape[0] == len(i) == (n * (n - 1) // 2 if n <= 200 else n - 1)
+        ps, ws = p[sel], W[sel]
+        rn = np.sqrt(np.sum(ws[i] ** 2, 1) + np.sum(ws[j] ** 2, 1))
+        # trilinear rows reproduce positions: row . X = (p_i - p_j)/|row|
+        assert np.allclose(M @ X, (ps[i] - ps[j]) / rn[:, None], atol=1e-12)
+        f = X @ np.array([0.3, -1.2, 2.0]) + 4.0
+        assert np.allclose(M @ f, ((ps[i] - ps[j]) @ [0.3, -1.2, 2.0]) / rn, atol=1e-12)
+        assert np.allclose(M @ np.ones(g.n_nodes), 0.0, atol=1e-13)
+        assert np.all(s.constraints[name]["b"] == 0) and np.all(s.constraints[name]["w"] == 2.0)
+
+
 # ---------------------------------------------------------------- solve ----
 def test_exact_solution_is_the_dense_least_squares_minimiser():
     g = ofd.Grid((0, 0, 0), (3, 3, 3), (1 / 3,) * 3)
@@ -268,3 +382,24 @@ def test_lsmr_and_cg_reach_the_same_minimiser():
     cx = ofd.solve_exact(s)
     assert np.allclose(ofd.solve_lsmr(s, tol=1e-14, maxiter=100000), cx, atol=1e-6)
     assert np.allclose(ofd.solve_cg(s, rtol=1e-13), cx, atol=1e-8)
+
+
+def test_regularisation_scale_against_brute_force_distances():
+    """compute_weighting (LI/_finite_difference_interpolator.py:66-88, a scipy
+    KDTree query) equals 1 + alpha exp(-d^2 / 2 sigma^2) with d the brute-force
+    nearest distance; the scale adds it to ones for the inside normal points
+    only, sigma defaulting to 10 x the node count on x (:436-444)."""
+    g = ofd.Grid((0.0, 0.0, 0.0), (4, 5, 3), (0.25, 0.2, 0.3))
+    rng = np.random.default_rng(24)
+    pts = rng.uniform(-0.2, 1.1, (17, 3))
+    X = g.nodes()
+    d2 = ((X[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    for sigma in (0.15, 2.0):
+        w = ofd.compute_weighting(X, pts, alpha=3.0, sigma=sigma)
+        assert np.allclose(w, 1 + 3.0 * np.exp(-d2.min(1) / (2 * sigma ** 2)), rtol=1e-13)
+    normals = np.column_stack([pts, np.ones((17, 3))])
+    ins = ofd.inside(g, pts)
+    assert 0 < ins.sum() < 17
+    sigma = g.nnodes[0] * 10
+    ref = 2 + 10.0 * np.exp(-d2[:, ins].min(1) / (2 * sigma ** 2))
+    assert np.allclose(ofd.regularisation_scale(g, normals), ref, rtol=1e-13)

@@ -232,3 +232,76 @@ def test_operator_table_of_the_library_matches_oracle(cells):
                 assert row[q] == 0
     ref = N @ x
     assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+# P1 rows built from element gradients are pinned through their first moments:
+# sum_k dl_k/dx_d v_k = e_d for the four vertices v_k of a tet, so a row
+# a = u . grad(l) (normalised) maps the node coordinates to u/|a| and sums to 0.
+def _inside_points(m, n, seed):
+    rng = np.random.default_rng(seed)
+    return m.origin + rng.uniform(0.02, 0.98, (n, 3)) * m.ncells * m.step, rng
+
+
+def test_normal_rows_are_axis_derivatives():
+    """LI/_p1interpolator.py:73-96: three rows per point (row 3i + d), grad_d(l)
+    normalised, b = n_d/|grad_d(l)|, weight npw (point weights ignored)."""
+    m = _mesh()
+    p, rng = _inside_points(m, 25, 31)
+    nv = rng.standard_normal((25, 3))
+    s = p1.assemble_p1(m, {"normal": np.column_stack([p, nv, rng.uniform(0.1, 9, 25)])},
+                       weights={"npw": 4.0})
+    fam = s.constraints["norm"]
+    M = fam["matrix"]
+    assert M.shape[0] == 75
+    mom = M @ m.nodes()
+    d = np.tile(np.arange(3), 25)
+    e = np.eye(3)[d]
+    scale = mom[np.arange(75), d]
+    assert np.all(scale > 0)
+    assert np.allclose(mom, e * scale[:, None], atol=1e-12 * scale.max())
+    assert np.allclose(fam["b"], nv.reshape(-1) * scale, rtol=1e-12)
+    assert np.allclose(M @ np.ones(m.n_nodes), 0.0, atol=1e-12 * scale.max())
+    assert np.allclose(sparse.linalg.norm(M, axis=1), 1.0)
+    assert np.all(fam["w"] == 4.0)
+
+
+def test_tangent_rows_and_tet_volume_weights():
+    """LI/_p1interpolator.py:248-296: one row t_hat . grad(l) per point, b = 0,
+    weight tpw x the volume of the containing tet (a third of the cell for the
+    central tet, local id 0, a sixth for the four corner tets)."""
+    m = _mesh()
+    p, rng = _inside_points(m, 40, 32)
+    v = rng.standard_normal((40, 3)) * rng.uniform(0.2, 5.0, (40, 1))
+    s = p1.assemble_p1(m, {"tangent": np.column_stack([p, v])}, weights={"tpw": 0.5})
+    fam = s.constraints["undefined gradient orthogonal constraint"]
+    mom = fam["matrix"] @ m.nodes()
+    u = v / np.linalg.norm(v, axis=1)[:, None]
+    assert np.allclose(mom / np.linalg.norm(mom, axis=1)[:, None], u, atol=1e-12)
+    assert np.allclose(fam["matrix"] @ np.ones(m.n_nodes), 0.0, atol=1e-10)
+    tet = m.element_for_location(p)[2]
+    cell = np.prod(m.step)
+    vol = np.where(tet % 5 == 0, cell / 3, cell / 6)
+    assert np.allclose(fam["w"], 0.5 * vol, rtol=1e-12)
+    assert np.all(fam["b"] == 0)
+
+
+def test_scaled_edge_jump_weights_average_the_two_tets():
+    """LI/_p1interpolator.py:150-154: with the weight scale on, a face pair's
+    weight is cgw x the mean of the two tets' node-averaged scales.  For a scale
+    linear in the node coordinates that is the scale at the midpoint of the
+    two tet barycentres, taken here from the brute-force face pairs."""
+    m = _mesh(cells=(3, 2, 2))
+    nd = m.nodes()
+    a = np.array([0.5, -1.5, 2.0])
+    scale = nd @ a + 3.0
+    s = p1.assemble_p1(m, {}, regularisation=0.2, scale=scale)
+    w = s.constraints["edge jump"]["w"]
+    el = m.elements
+    sets = [frozenset(e) for e in el]
+    pairs = [(i, j) for i in range(len(el)) for j in range(i) if len(sets[i] & sets[j]) == 3]
+    assert len(pairs) == len(w)
+    bc = nd[el].mean(axis=1)
+    expect = sorted(0.2 * (0.5 * (bc[i] + bc[j]) @ a + 3.0) for i, j in pairs)
+    assert np.allclose(np.sort(w), expect, rtol=1e-12)
+    s1 = p1.assemble_p1(m, {}, regularisation=0.2, scale=np.full(m.n_nodes, 1.5))
+    assert np.allclose(s1.constraints["edge jump"]["w"], 0.3, rtol=1e-14)
