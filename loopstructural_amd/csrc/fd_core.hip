@@ -741,7 +741,8 @@ void mr_invalidate(MrWork* w);
 struct lsamd_fd_system {
   lsamd::Mg* mg = nullptr;  // multigrid hierarchy (built on first MG solve after set_rows)
   lsamd::Ls* ls = nullptr;  // LSMR work (built on the first LSMR solve after set_rows)
-  lsamd::MrWork* mr = nullptr;  // multi-rhs multigrid work (kept across set_rows)
+  // multi-rhs multigrid work per group width 4/8/16/32 (kept across set_rows)
+  lsamd::MrWork* mr[4] = {nullptr, nullptr, nullptr, nullptr};
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -2249,81 +2250,106 @@ int mr_alloc(MrWork* w, P** p, int64_t count) {
   return LSAMD_OK;
 }
 
-// work of R lanes for the system's hierarchy; row buffers follow the rows
-int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
-  MrWork* w = s->mr;
-  if (w && w->R != R) {
-    mr_destroy(w);
-    w = s->mr = nullptr;
-  }
+// lane vectors, controls and the level-0 stencil plan of R lanes for the
+// system's hierarchy
+int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
   const Mg* m = s->mg;
   const int64_t n = s->n;
+  w->R = R;
+  for (const MgLevel& L : m->lv) {
+    float *b, *x, *q, *d;
+    for (float** v : {&b, &x, &q, &d}) {
+      if (int rc = mr_alloc(w, v, L.n * R)) return rc;
+      LSAMD_HIP(hipMemset(*v, 0, L.n * R * 4));
+    }
+    w->b.push_back(b);
+    w->x.push_back(x);
+    w->q.push_back(q);
+    w->d.push_back(d);
+  }
+  if (int rc = injected("multi_rhs_work")) return rc;
+  for (double** v : {&w->rhs, &w->x64, &w->r64, &w->p64, &w->q64}) {
+    if (int rc = mr_alloc(w, v, n * R)) return rc;
+    LSAMD_HIP(hipMemset(*v, 0, n * R * 8));
+  }
+  if (int rc = mr_alloc(w, &w->part, (int64_t)vec_blocks(n) * 2 * R)) return rc;
+  if (int rc = mr_alloc(w, &w->ctl, R)) return rc;
+  if (int rc = mr_alloc(w, &w->all_done, 1)) return rc;
+  if (hipHostMalloc((void**)&w->host, (R + 1) * sizeof(MrCtl), hipHostMallocDefault) !=
+      hipSuccess) {
+    w->host = nullptr;
+    return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed (multi-rhs multigrid)");
+  }
+  // R-system plan of the level-0 stencil
+  std::vector<BSys> hs(R);
+  memset(hs.data(), 0, R * sizeof(BSys));
+  for (int r = 0; r < R; ++r) {
+    hs[r].g = s->gd;
+    hs[r].n = n;
+    hs[r].off = (int64_t)r * n;
+  }
+  if (int rc = mr_alloc(w, &w->sysR, R)) return rc;
+  LSAMD_HIP(hipMemcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
+  int64_t maxt = 1;
+  for (int p = 0; p < 2; ++p) {
+    const int zc = s->zc[p] == 8 || s->zc[p] == 16 || s->zc[p] == 32 ? s->zc[p] : 16;
+    std::vector<int4> t;
+    for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, zc, t);
+    if (int rc = mr_alloc(w, &w->tilesR[p], (int64_t)t.size())) return rc;
+    LSAMD_HIP(hipMemcpy(w->tilesR[p], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    w->ntR[p] = (int)t.size();
+    w->zcR[p] = zc;
+    maxt = std::max<int64_t>(maxt, (int64_t)t.size());
+  }
+  if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
+  LSAMD_HIP(hipMemset(w->ctlR, 0, R * sizeof(CgCtl)));
+  if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
+  return LSAMD_OK;
+}
+
+// per-lane row buffers, sized by the system's current rows
+int mr_rows(lsamd_fd_system* s, MrWork* w) {
+  if (w->t_rows == s->nrows) return LSAMD_OK;
+  const int R = w->R;
+  mr_invalidate(w);
+  for (void* p : {(void*)w->t32, (void*)w->t64, (void*)w->upart}) {
+    if (!p) continue;
+    for (size_t i = 0; i < w->mem.size(); ++i)
+      if (w->mem[i] == p) {
+        hipFree(p);
+        w->mem.erase(w->mem.begin() + i);
+        break;
+      }
+  }
+  w->t32 = nullptr;
+  w->t64 = nullptr;
+  w->upart = nullptr;
+  w->t_rows = -1;
+  if (int rc = mr_alloc(w, &w->t32, std::max<int64_t>(s->nrows, 1) * R)) return rc;
+  if (int rc = mr_alloc(w, &w->t64, std::max<int64_t>(s->nrows, 1) * R)) return rc;
+  if (int rc = mr_alloc(w, &w->upart, std::max<int64_t>(ceil_div(s->nrows, 256), 1) * R))
+    return rc;
+  w->t_rows = s->nrows;
+  return LSAMD_OK;
+}
+
+// work of R lanes (R = 4, 8, 16 or 32), one per width, kept on the handle
+// only when complete: a failed build or row resize frees it
+int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
+  const int slot = R == 4 ? 0 : R == 8 ? 1 : R == 16 ? 2 : 3;
+  MrWork* w = s->mr[slot];
   if (!w) {
     w = new MrWork();
-    w->R = R;
-    s->mr = w;
-    for (const MgLevel& L : m->lv) {
-      float *b, *x, *q, *d;
-      for (float** v : {&b, &x, &q, &d}) {
-        if (int rc = mr_alloc(w, v, L.n * R)) return rc;
-        LSAMD_HIP(hipMemset(*v, 0, L.n * R * 4));
-      }
-      w->b.push_back(b);
-      w->x.push_back(x);
-      w->q.push_back(q);
-      w->d.push_back(d);
-    }
-    for (double** v : {&w->rhs, &w->x64, &w->r64, &w->p64, &w->q64}) {
-      if (int rc = mr_alloc(w, v, n * R)) return rc;
-      LSAMD_HIP(hipMemset(*v, 0, n * R * 8));
-    }
-    if (int rc = mr_alloc(w, &w->part, (int64_t)vec_blocks(n) * 2 * R)) return rc;
-    if (int rc = mr_alloc(w, &w->ctl, R)) return rc;
-    if (int rc = mr_alloc(w, &w->all_done, 1)) return rc;
-    if (hipHostMalloc((void**)&w->host, (R + 1) * sizeof(MrCtl), hipHostMallocDefault) !=
-        hipSuccess)
-      return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed (multi-rhs multigrid)");
-    // R-system plan of the level-0 stencil
-    std::vector<BSys> hs(R);
-    memset(hs.data(), 0, R * sizeof(BSys));
-    for (int r = 0; r < R; ++r) {
-      hs[r].g = s->gd;
-      hs[r].n = n;
-      hs[r].off = (int64_t)r * n;
-    }
-    if (int rc = mr_alloc(w, &w->sysR, R)) return rc;
-    LSAMD_HIP(hipMemcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
-    int64_t maxt = 1;
-    for (int p = 0; p < 2; ++p) {
-      const int zc = s->zc[p] == 8 || s->zc[p] == 16 || s->zc[p] == 32 ? s->zc[p] : 16;
-      std::vector<int4> t;
-      for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, zc, t);
-      if (int rc = mr_alloc(w, &w->tilesR[p], (int64_t)t.size())) return rc;
-      LSAMD_HIP(hipMemcpy(w->tilesR[p], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
-      w->ntR[p] = (int)t.size();
-      w->zcR[p] = zc;
-      maxt = std::max<int64_t>(maxt, (int64_t)t.size());
-    }
-    if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
-    LSAMD_HIP(hipMemset(w->ctlR, 0, R * sizeof(CgCtl)));
-    if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
-  }
-  if (w->t_rows != s->nrows) {
-    mr_invalidate(w);
-    if (w->t32) {
-      for (void* p : {(void*)w->t32, (void*)w->t64, (void*)w->upart})
-        for (size_t i = 0; i < w->mem.size(); ++i)
-          if (w->mem[i] == p) {
-            hipFree(p);
-            w->mem.erase(w->mem.begin() + i);
-            break;
-          }
-    }
-    if (int rc = mr_alloc(w, &w->t32, std::max<int64_t>(s->nrows, 1) * R)) return rc;
-    if (int rc = mr_alloc(w, &w->t64, std::max<int64_t>(s->nrows, 1) * R)) return rc;
-    if (int rc = mr_alloc(w, &w->upart, std::max<int64_t>(ceil_div(s->nrows, 256), 1) * R))
+    if (int rc = mr_build(s, w, R)) {
+      mr_destroy(w);
       return rc;
-    w->t_rows = s->nrows;
+    }
+    s->mr[slot] = w;
+  }
+  if (int rc = mr_rows(s, w)) {
+    mr_destroy(w);
+    s->mr[slot] = nullptr;
+    return rc;
   }
   *out = w;
   return LSAMD_OK;
@@ -2868,7 +2894,7 @@ int lsamd_fd_system_destroy(lsamd_fd_system* s) {
   }
   if (s->mg) mg_destroy(s->mg);
   if (s->ls) ls_destroy(s->ls);
-  if (s->mr) mr_destroy(s->mr);
+  for (lsamd::MrWork* w : s->mr) mr_destroy(w);
   for (void* p : s->owned) hipFree(p);
   if (s->ctl_host) hipHostFree(s->ctl_host);
   if (s->bctl_host) hipHostFree(s->bctl_host);
@@ -2889,7 +2915,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
   hipStream_t st = (hipStream_t)stream;
   LSAMD_HIP(hipStreamSynchronize(st));
   if (s->mg) mg_invalidate(s->mg);
-  if (s->mr) mr_invalidate(s->mr);
+  for (lsamd::MrWork* w : s->mr) mr_invalidate(w);
   if (s->ls) {
     ls_destroy(s->ls);
     s->ls = nullptr;

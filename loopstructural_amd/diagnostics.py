@@ -1,11 +1,42 @@
-"""Setup diagnostics report with the reference's fields (LI/_diagnostics.py:7-102)."""
+"""Set-up diagnostics of a discrete interpolator.
+
+The three record types carry the field names of the reference's report
+(LI/_diagnostics.py:7-102) so that code reading ``report.families[name]
+.row_count`` or ``report.to_dict()`` keeps working; everything else here is
+generated from the dataclass fields: ``to_dict`` walks the fields in
+declaration order (nested records become dicts), and ``summary`` renders the
+report as an aligned text table.
+"""
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, fields, is_dataclass
+
+
+def _plain(value):
+    """Nested records -> dicts, dicts of records -> dicts of dicts."""
+    if is_dataclass(value):
+        return value.to_dict()
+    if isinstance(value, dict):
+        return {k: _plain(v) for k, v in value.items()}
+    return value
+
+
+class _Record:
+    """Dict form of a frozen dataclass; fields listed in ``_hidden`` stay out
+    (a family's name is the key it is stored under)."""
+
+    _hidden: tuple = ()
+
+    def to_dict(self) -> dict:
+        return {f.name: _plain(getattr(self, f.name)) for f in fields(self)
+                if f.name not in self._hidden}
 
 
 @dataclass(frozen=True)
-class ConstraintFamilyDiagnostics:
+class ConstraintFamilyDiagnostics(_Record):
+    """Rows one constraint family contributed to the least-squares system."""
+
+    _hidden = ("name",)
     name: str
     active: bool
     row_count: int
@@ -18,7 +49,9 @@ class ConstraintFamilyDiagnostics:
 
 
 @dataclass(frozen=True)
-class RegionCoverageDiagnostics:
+class RegionCoverageDiagnostics(_Record):
+    """How many support nodes the active region holds."""
+
     total_support_nodes: int
     active_region_nodes: int
     inactive_region_nodes: int
@@ -26,7 +59,9 @@ class RegionCoverageDiagnostics:
 
 
 @dataclass(frozen=True)
-class ConstraintDiagnosticsReport:
+class ConstraintDiagnosticsReport(_Record):
+    """Families by name, region coverage and per-family outside-point counts."""
+
     interpolator_type: str
     families: dict = field(default_factory=dict)
     region_coverage: RegionCoverageDiagnostics | None = None
@@ -41,42 +76,35 @@ class ConstraintDiagnosticsReport:
         return {k: v for k, v in self.families.items() if v.active}
 
     def to_dict(self) -> dict:
-        rc = self.region_coverage
-        return {
-            "interpolator_type": self.interpolator_type,
-            "total_rows": self.total_rows,
-            "families": {n: {"active": f.active, "row_count": f.row_count,
-                             "dropped_rows": f.dropped_rows,
-                             "effective_weight_mean": f.effective_weight_mean,
-                             "effective_weight_min": f.effective_weight_min,
-                             "effective_weight_max": f.effective_weight_max,
-                             "source_point_count": f.source_point_count,
-                             "outside_model_point_count": f.outside_model_point_count}
-                         for n, f in self.families.items()},
-            "region_coverage": None if rc is None else {
-                "total_support_nodes": rc.total_support_nodes,
-                "active_region_nodes": rc.active_region_nodes,
-                "inactive_region_nodes": rc.inactive_region_nodes,
-                "active_fraction": rc.active_fraction},
-            "outside_model_points": dict(self.outside_model_points),
-        }
+        d = super().to_dict()
+        # the row total is derived, listed right after the type
+        return {"interpolator_type": d.pop("interpolator_type"), "total_rows": self.total_rows,
+                **d}
 
     def summary(self) -> str:
-        lines = [f"Constraint diagnostics for {self.interpolator_type}",
-                 f"Total rows: {self.total_rows}", "Active families:"]
-        active = self.active_families
-        if not active:
-            lines.append("  - none")
-        for name, f in sorted(active.items()):
-            dropped = "unknown" if f.dropped_rows is None else str(f.dropped_rows)
-            mw = "n/a" if f.effective_weight_mean is None else f"{f.effective_weight_mean:.4g}"
-            lines.append(f"  - {name}: rows={f.row_count}, dropped={dropped}, mean_weight={mw}")
-        if self.region_coverage is not None:
-            rc = self.region_coverage
-            lines.append(f"Region coverage: {rc.active_region_nodes}/{rc.total_support_nodes} "
-                         f"({rc.active_fraction:.1%})")
+        """Text table: one line per active family (rows, dropped rows, weight
+        range), then the region coverage and the outside-point counts."""
+        head = ("family", "rows", "dropped", "weight mean", "weight range")
+        body = []
+        for name in sorted(self.active_families):
+            f = self.families[name]
+            w = [f.effective_weight_mean, f.effective_weight_min, f.effective_weight_max]
+            fmt = ["-" if v is None else f"{v:.4g}" for v in w]
+            body.append((name, str(f.row_count), "-" if f.dropped_rows is None
+                         else str(f.dropped_rows), fmt[0], f"{fmt[1]} .. {fmt[2]}"))
+        width = [max(len(r[i]) for r in [head, *body]) for i in range(len(head))]
+
+        def row(cells):
+            return "  ".join(c.ljust(wd) for c, wd in zip(cells, width)).rstrip()
+
+        out = [f"{self.interpolator_type}: {self.total_rows} rows, "
+               f"{len(body)} active families", row(head)]
+        out += [row(r) for r in body] or ["(no active family)"]
+        rc = self.region_coverage
+        if rc is not None:
+            out.append(f"active region: {rc.active_region_nodes} of {rc.total_support_nodes} "
+                       f"nodes ({100.0 * rc.active_fraction:.1f}%)")
         if self.outside_model_points:
-            lines.append("Outside-model points:")
-            for k, v in sorted(self.outside_model_points.items()):
-                lines.append(f"  - {k}: {v}")
-        return "\n".join(lines)
+            out.append("points outside the model: " + ", ".join(
+                f"{k}={v}" for k, v in sorted(self.outside_model_points.items())))
+        return "\n".join(out)

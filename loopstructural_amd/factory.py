@@ -101,90 +101,124 @@ class SupportFactory:
                                          nsteps_cells=nsteps)
 
 
+# data keys of a to_dict() payload and the setter that takes each array
+# (LI/_geological_interpolator.py:799-862 writes them under "data")
+_DATA_SETTERS = {
+    "value": "set_value_constraints",
+    "gradient": "set_gradient_constraints",
+    "normal": "set_normal_constraints",
+    "tangent": "set_tangent_constraints",
+    "interface": "set_interface_constraints",
+    "inequality": "set_value_inequality_constraints",
+    "inequality_pairs": "set_inequality_pairs_constraints",
+}
+
+
+def _type_aliases():
+    """Every spelling a caller may use for an interpolator type: the short
+    registry keys ('FDI', 'PLI', ...) and the enum member names."""
+    table = {m.name: m for m in InterpolatorType}
+    table.update(interpolator_string_map)
+    return table
+
+
+def _support_from_payload(payload: dict):
+    """A support from its to_dict() payload.  Keys the support class does not
+    take (a TetMesh has no rotation_xy) are left out, found from the
+    constructor's signature rather than by retrying on TypeError."""
+    import inspect
+
+    payload = dict(payload)
+    kind = payload.pop("type", None)
+    if kind is None:
+        raise ValueError("No support type specified")
+    if isinstance(kind, str):
+        kind = SupportType[kind]
+    cls = support_map.get(SupportType(int(kind)))
+    if cls is None:
+        raise NotImplementedError(f"support {SupportType(int(kind)).name} is not part of this "
+                                  f"package")
+    params = inspect.signature(cls.__init__).parameters
+    takes_any = any(p.kind is inspect.Parameter.VAR_KEYWORD for p in params.values())
+    kwargs = {k: v for k, v in payload.items() if takes_any or k in params}
+    return cls(**kwargs)
+
+
 class InterpolatorFactory:
-    """LI/_interpolator_factory.py:24-140 for the GPU interpolators."""
+    """LI/_interpolator_factory.py:24-140 for the GPU interpolators: the same
+    entry points, arguments and errors; types without a GPU class here are
+    refused with NotImplementedError."""
 
-    @staticmethod
-    def _normalise_interpolator_type(interpolatortype) -> InterpolatorType:
-        if isinstance(interpolatortype, str):
-            if interpolatortype in interpolator_string_map:
-                return interpolator_string_map[interpolatortype]
-            if interpolatortype in InterpolatorType.__members__:
-                return InterpolatorType[interpolatortype]
-            return InterpolatorType(interpolatortype)
-        return interpolatortype
+    _aliases = _type_aliases()
 
-    @staticmethod
-    def create_interpolator(interpolatortype=None, boundingbox=None, nelements=None,
+    @classmethod
+    def _normalise_interpolator_type(cls, interpolatortype) -> InterpolatorType:
+        if isinstance(interpolatortype, InterpolatorType):
+            return interpolatortype
+        if isinstance(interpolatortype, str) and interpolatortype in cls._aliases:
+            return cls._aliases[interpolatortype]
+        return InterpolatorType(interpolatortype)
+
+    @classmethod
+    def _class_of(cls, kind: InterpolatorType):
+        impl = interpolator_map.get(kind)
+        if impl is None:
+            raise NotImplementedError(
+                f"{kind.name} has no GPU implementation in this package "
+                f"(available: {', '.join(t.name for t in interpolator_map)})")
+        return impl
+
+    @classmethod
+    def create_interpolator(cls, interpolatortype=None, boundingbox=None, nelements=None,
                             element_volume=None, support=None, buffer=None, solver=None):
         if interpolatortype is None:
             raise ValueError("No interpolator type specified")
         if support is None and boundingbox is None:
             raise ValueError("No bounding box specified")
-        interpolatortype = InterpolatorFactory._normalise_interpolator_type(interpolatortype)
-        if interpolatortype not in interpolator_map:
-            raise NotImplementedError(
-                f"{interpolatortype.name} has no GPU implementation in this package "
-                f"(available: {', '.join(t.name for t in interpolator_map)})")
+        kind = cls._normalise_interpolator_type(interpolatortype)
+        impl = cls._class_of(kind)
         if support is None:
-            supporttype = support_interpolator_map[interpolatortype].get(
-                int(getattr(boundingbox, "dimensions", 3)))
-            if supporttype is None:
+            dim = int(getattr(boundingbox, "dimensions", 3))
+            stype = support_interpolator_map[kind].get(dim)
+            if stype is None:
                 raise NotImplementedError("only 3-D supports are part of this package")
             support = SupportFactory.create_support_from_bbox(
-                supporttype, bounding_box=boundingbox, nelements=nelements,
+                stype, bounding_box=boundingbox, nelements=nelements,
                 element_volume=element_volume, buffer=buffer)
-        interpolator = interpolator_map[interpolatortype](support)
-        if boundingbox is not None:
-            interpolator.bounding_box = boundingbox
-        if solver is not None:
-            interpolator.solver = solver
-        return interpolator
+        interp = impl(support)
+        for attr, val in (("bounding_box", boundingbox), ("solver", solver)):
+            if val is not None:
+                setattr(interp, attr, val)
+        return interp
 
-    @staticmethod
-    def from_dict(d):
+    @classmethod
+    def from_dict(cls, d):
         """Rebuild an interpolator from its to_dict() payload: support,
-        constraint arrays, c and the up_to_date / valid flags."""
-        d = dict(d)
-        interpolator_type = d.pop("type", None)
-        if interpolator_type is None:
+        constraint arrays (each non-empty one through its setter), c and the
+        up_to_date / valid flags; other keys go to create_interpolator."""
+        rest = dict(d)
+        kind = rest.pop("type", None)
+        if kind is None:
             raise ValueError("No interpolator type specified")
-        interpolator_type = InterpolatorFactory._normalise_interpolator_type(interpolator_type)
-        support = d.pop("support", None)
+        support = rest.pop("support", None)
         if isinstance(support, dict):
-            payload = dict(support)
-            try:
-                support = SupportFactory.from_dict(payload)
-            except TypeError as exc:
-                # TetMesh takes no rotation_xy
-                if "rotation_xy" in payload and "rotation_xy" in str(exc):
-                    payload.pop("rotation_xy", None)
-                    support = SupportFactory.from_dict(payload)
-                else:
-                    raise
-        data = d.pop("data", None)
-        c = d.pop("c", None)
-        up_to_date = bool(d.pop("up_to_date", False))
-        valid = bool(d.pop("valid", True))
-        interpolator = InterpolatorFactory.create_interpolator(interpolator_type, support=support,
-                                                               **d)
-        if data is not None:
-            for key, setter in (("value", "set_value_constraints"),
-                                ("gradient", "set_gradient_constraints"),
-                                ("normal", "set_normal_constraints"),
-                                ("tangent", "set_tangent_constraints"),
-                                ("interface", "set_interface_constraints"),
-                                ("inequality", "set_value_inequality_constraints"),
-                                ("inequality_pairs", "set_inequality_pairs_constraints")):
-                if data.get(key) is not None:
-                    arr = np.asarray(data[key], dtype=float)
-                    if arr.size > 0:
-                        getattr(interpolator, setter)(arr)
-        if c is not None and hasattr(interpolator, "c"):
-            interpolator.c = np.asarray(c, dtype=float)
-        interpolator.up_to_date = up_to_date
-        interpolator.valid = valid
-        return interpolator
+            support = _support_from_payload(support)
+        data = rest.pop("data", None) or {}
+        c = rest.pop("c", None)
+        flags = {"up_to_date": bool(rest.pop("up_to_date", False)),
+                 "valid": bool(rest.pop("valid", True))}
+        interp = cls.create_interpolator(cls._normalise_interpolator_type(kind), support=support,
+                                         **rest)
+        arrays = {k: np.asarray(data[k], dtype=float) for k in _DATA_SETTERS
+                  if data.get(k) is not None}
+        for key, arr in arrays.items():
+            if arr.size:
+                getattr(interp, _DATA_SETTERS[key])(arr)
+        if c is not None and hasattr(interp, "c"):
+            interp.c = np.asarray(c, dtype=float)
+        for name, val in flags.items():
+            setattr(interp, name, val)
+        return interp
 
     @staticmethod
     def get_supported_interpolators():

@@ -11,6 +11,7 @@ regulariser is never materialised.
 from __future__ import annotations
 
 import ctypes
+import itertools
 import json
 import logging
 from collections import OrderedDict, defaultdict
@@ -31,6 +32,14 @@ logger = logging.getLogger(__name__)
 
 ValidationError = cons.ValidationError
 
+
+
+# pair generators of the interface constraints, by mode: (n_pairs, 2) indices
+_PAIR_MODES = {
+    "all": lambda n: np.column_stack(np.triu_indices(n, k=1)),
+    "star": lambda n: np.column_stack([np.zeros(n - 1, dtype=int), np.arange(1, n)]),
+    "chain": lambda n: np.column_stack([np.arange(n - 1), np.arange(1, n)]),
+}
 
 class InterpolatorType(Enum):
     """LI/_interpolatortype.py."""
@@ -335,7 +344,6 @@ class DiscreteInterpolator(GeologicalInterpolator):
         self.dimensions = support.dimension
         self.c = (np.array(c) if c is not None and np.array(c).shape[0] == support.n_nodes
                   else np.zeros(support.n_nodes))
-        self._c_dev = None
         self.shape = "rectangular"
         self.c_ = 0
         self.solver = "cg"
@@ -442,14 +450,17 @@ class DiscreteInterpolator(GeologicalInterpolator):
 
     # ------------------------------------------------------------- rows --
     def _unique_name(self, name):
-        """Collision renaming of LI/_discrete_interpolator.py:375-380."""
-        base = name
-        while name in self.constraints:
-            count = 0
-            if "_" in name:
-                count = int(name.split("_")[1]) + 1
-            name = base + f"_{count}"
-        return name
+        """Name under which a new family is stored: ``name`` itself, or the
+        first free of ``name_k, name_(k+1), ...`` with k = 0, or one past the
+        number after the first underscore of ``name`` (the rename of
+        LI/_discrete_interpolator.py:375-380; where that loop would revisit a
+        taken name, this takes the next number)."""
+        if name not in self.constraints:
+            return name
+        parts = name.split("_")
+        start = int(parts[1]) + 1 if len(parts) > 1 and parts[1].isdigit() else 0
+        return next(c for c in (f"{name}_{k}" for k in itertools.count(start))
+                    if c not in self.constraints)
 
     def _register_rows(self, blk: fs.FdRows, name: str):
         if blk.nrows == 0:
@@ -585,7 +596,6 @@ class DiscreteInterpolator(GeologicalInterpolator):
         if not info["converged"]:
             logger.warning("CG reached iteration limit (%s) and did not converge; using last "
                            "iteration", info["iterations"])
-        self._c_dev = c
         self.c = c.cpu().numpy()
         self.up_to_date = True
         timing["total_seconds"] = perf_counter() - started
@@ -624,7 +634,6 @@ class DiscreteInterpolator(GeologicalInterpolator):
             logger.warning("Solution to least squares problem is all zeros, check input data")
         elif info["istop"] in (3, 6):
             logger.warning("COND(A) seems to be greater than CONLIM, check input data")
-        self._c_dev = c
         self.c = c.cpu().numpy()
         self.up_to_date = True
         timing["total_seconds"] = perf_counter() - started
@@ -643,10 +652,13 @@ class DiscreteInterpolator(GeologicalInterpolator):
         return bool(self.up_to_date)
 
     def _c_device(self):
-        if self._c_dev is None or self._c_dev.shape[0] != self.dof:
-            self._c_dev = torch.from_numpy(np.ascontiguousarray(self.c, dtype=float)).to(
-                _lib.device())
-        return self._c_dev
+        """Device copy of ``self.c`` as it is now.  The reference reads ``c`` on
+        every evaluation, and callers may assign it or change it in place
+        (from_dict, a saved model), so it is uploaded on every call."""
+        c = np.ascontiguousarray(self.c, dtype=float)
+        if c.shape != (self.dof,):
+            raise ValueError(f"c has shape {c.shape}, expected ({self.dof},)")
+        return torch.from_numpy(c).to(_lib.device())
 
     def to_dict(self):
         # LI/_discrete_interpolator.py:1381-1388: base-class keys win
@@ -907,25 +919,22 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
             self.up_to_date = False
 
     def _interface_pair_indices(self, n: int) -> np.ndarray:
-        """LI/_finite_difference_interpolator.py:271-295."""
+        """Point pairs tied by one interface id (LI/_finite_difference_interpolator.py:271-295):
+        mode 'all' (every i < j), 'star' (0 with each other) or 'chain'
+        (neighbours in order); 'auto' is 'all' up to the threshold and the
+        fallback ('star' unless 'chain') above it."""
         if n < 2:
             return np.zeros((0, 2), dtype=int)
         mode = str(self.interface_pair_mode).lower()
-        threshold = max(2, int(self.interface_pair_threshold))
-        fallback = str(self.interface_pair_fallback).lower()
-        if fallback not in ("star", "chain"):
-            fallback = "star"
         if mode == "auto":
-            mode = fallback if n > threshold else "all"
-        if mode == "all":
-            ii, jj = np.triu_indices(n, k=1)
-            return np.column_stack([ii, jj]).astype(int)
-        if mode == "star":
-            return np.column_stack([np.zeros(n - 1, dtype=int), np.arange(1, n, dtype=int)])
-        if mode == "chain":
-            return np.column_stack([np.arange(0, n - 1, dtype=int), np.arange(1, n, dtype=int)])
-        raise ValueError(f"Unknown interface_pair_mode '{self.interface_pair_mode}'. Use one "
-                         f"of: auto, all, star, chain")
+            fallback = str(self.interface_pair_fallback).lower()
+            large = n > max(2, int(self.interface_pair_threshold))
+            mode = (fallback if fallback == "chain" else "star") if large else "all"
+        make = _PAIR_MODES.get(mode)
+        if make is None:
+            raise ValueError(f"Unknown interface_pair_mode '{self.interface_pair_mode}'. Use one "
+                             f"of: auto, all, star, chain")
+        return make(n).astype(int)
 
     def add_interface_constraints(self, w=1.0):
         """Pairwise equal-value rows per interface id (:297-340), 16 entries per

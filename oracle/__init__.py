@@ -23,7 +23,14 @@ Pinning (what checks each function, see ``tests/test_oracle_*.py``):
     linear fields, symmetric Gram, brute-force comparison against an
     independent dense finite-difference construction on tiny grids;
   * data rows: closed forms (partition of unity, gradient of linear fields),
-    strike/dip orthonormality against the normal;
+    strike/dip orthonormality against the normal; every row family built from
+    shape-function gradients (normal, strike, dip, tangent; P1 normal and
+    tangent) through its first moment, which equals a closed-form direction
+    (axis, (n_y, -n_x, 0), strike x n, the unit tangent) over the row norm;
+    interface rows through position differences and pair counts (all pairs up
+    to 200 points per id, the star above); tangent weights against tet
+    volumes; the weight scale against brute-force nearest distances and the
+    scaled edge-jump weights against tet barycentres;
   * solve: the unique least-squares minimiser via ``scipy.sparse.linalg.spsolve``
     on the normal equations (library routine) and ``numpy.linalg.lstsq`` on
     tiny dense systems;

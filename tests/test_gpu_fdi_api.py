@@ -69,6 +69,13 @@ def test_fdi_value_normal_matches_minimiser():
     assert np.all(v[::50] == 0.0)  # NaN coordinates -> 0
     g = interp.evaluate_gradient(pts)
     assert np.array_equal(g, ofd.evaluate_gradient(og, interp.c, pts))
+    # evaluation follows c after a solve: a new array, then an in-place change
+    interp.c = og.nodes() @ np.array([0.5, -1.0, 2.0])
+    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts))
+    interp.c[: og.n_nodes // 2] *= -3.0
+    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts))
+    assert np.array_equal(interp.evaluate_gradient(pts),
+                          ofd.evaluate_gradient(og, interp.c, pts))
 
 
 def test_fdi_all_data_families_match_minimiser():

@@ -5,9 +5,14 @@ observed values and normal directions).
 Each lane is checked against the single-system multigrid solve of the same
 right-hand side (both to rtol 1e-12: agreement 1e-9 relative) and against the
 exact minimiser of its realisation (spsolve on the oracle's normal equations,
-1e-8); lane counts 3, 8 and 11 cover the 4- and 8-lane kernels and a split
-into two groups.  The right-hand side of a realisation from its row values is
-checked against the system's own rhs after set_rows with those rows.
+1e-8).  Groups are 4, 8, 16 or 32 lanes wide (32 once at least 24 right-hand
+sides are left, else the smallest width that holds them): 3 -> 4, 8 -> 8,
+11 -> 16, 21 -> 16 + 8, 27 -> 32 (5 idle lanes), 37 -> 32 + 8, which covers
+every width, padded groups, and two widths in one call (a work per width is
+kept on the handle).  The right-hand side of a realisation from its row values
+is checked against the system's own rhs after set_rows with those rows.  A
+failure while the lane work is allocated leaves no half-built work behind: the
+next call builds it afresh and solves correctly.
 """
 import numpy as np
 import pytest
@@ -57,7 +62,7 @@ def test_rhs_for_matches_set_rows():
     assert float(torch.linalg.vector_norm(rhs[1] - ref) / torch.linalg.vector_norm(ref)) <= 1e-14
 
 
-@pytest.mark.parametrize("nreal", [3, 8, 11, 21, 27])
+@pytest.mark.parametrize("nreal", [3, 8, 11, 21, 27, 37])
 def test_multi_rhs_lanes_match_single_solves(nreal):
     s, rhs, reals, g, n = _setup(23, nreal)
     c, infos = s.solve_multi(rhs, rtol=1e-12)
@@ -90,3 +95,19 @@ def test_multi_rhs_after_new_rows_and_zero_rhs():
     c1, _ = s.solve(rtol=1e-12, multigrid=True)
     for r in range(2):
         assert float(torch.linalg.vector_norm(c2[r] - c1) / torch.linalg.vector_norm(c1)) <= 1e-9
+
+
+def test_multi_rhs_work_rebuilt_after_a_failed_allocation(monkeypatch):
+    from loopstructural_amd import _lib
+
+    s, rhs, reals, g, n = _setup(15, 5)
+    monkeypatch.setenv("LSAMD_FAULT_INJECT", "multi_rhs_work")
+    with pytest.raises(_lib.LsamdError) as e:
+        s.solve_multi(rhs, rtol=1e-12)
+    assert e.value.code == _lib.LSAMD_ERR_INJECTED
+    monkeypatch.delenv("LSAMD_FAULT_INJECT")
+    c, infos = s.solve_multi(rhs, rtol=1e-12)
+    assert all(i["converged"] for i in infos), infos
+    for r in range(5):
+        c1, _ = s.solve(rtol=1e-12, rhs=rhs[r], multigrid=True, check_every=8)
+        assert float(torch.linalg.vector_norm(c[r] - c1) / torch.linalg.vector_norm(c1)) <= 1e-9
