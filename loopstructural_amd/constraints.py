@@ -116,3 +116,52 @@ def gradient_array(points, dimensions: int = 3, is_normal: bool = False) -> np.n
 def interface_array(points, dimensions: int = 3) -> np.ndarray:
     """(N, d+1|d+2) -> validated (M, d+2) [xyz, interface id, weight]."""
     return value_array(points, dimensions)
+
+
+def _require_finite(arr: np.ndarray, name: str) -> None:
+    bad = ~np.isfinite(arr)
+    if np.any(bad):
+        raise FiniteValueError(f"{name} contains {int(np.sum(bad))} non-finite value(s)")
+
+
+def _weight_column(pts: np.ndarray, col: int, name: str) -> np.ndarray:
+    if pts.shape[1] <= col:
+        return np.ones(pts.shape[0])
+    return _fix_weights(np.array(pts[:, col], dtype=float))
+
+
+def inequality_array(points, dimensions: int = 3) -> np.ndarray:
+    """(N, d+2|d+3) -> (N, d+3) [xyz, lower, upper, weight]; positions and
+    bounds must be finite and lower < upper (LI/constraints.py:290-348)."""
+    pts = _to_float(points, "Inequality constraint array")
+    if pts.ndim != 2:
+        raise ShapeError("Inequality constraint array must be 2D")
+    if pts.shape[1] not in (dimensions + 2, dimensions + 3):
+        raise ShapeError(f"Inequality constraint array must have {dimensions + 2} or "
+                         f"{dimensions + 3} columns")
+    w = _weight_column(pts, dimensions + 2, "Inequality constraint")
+    _require_finite(pts[:, :dimensions], "Position (X, Y, Z)")
+    bounds = pts[:, dimensions:dimensions + 2]
+    _require_finite(bounds, "Bound values")
+    bad = bounds[:, 0] >= bounds[:, 1]
+    if np.any(bad):
+        raise ValidationError(
+            f"Found {int(np.sum(bad))} inequality constraints with lower_bound >= upper_bound. "
+            "For each constraint, lower_bound must be strictly less than upper_bound. "
+            f"Invalid constraints at indices: {np.nonzero(bad)[0]}")
+    return np.hstack([pts[:, :dimensions + 2], w[:, None]])
+
+
+def inequality_pair_array(points, dimensions: int = 3) -> np.ndarray:
+    """(N, d+1|d+2) -> (N, d+2) [xyz, pair id, weight]; positions and ids must
+    be finite (LI/constraints.py:351-400)."""
+    pts = _to_float(points, "Inequality pair constraint array")
+    if pts.ndim != 2:
+        raise ShapeError("Inequality pair constraint array must be 2D")
+    if pts.shape[1] not in (dimensions + 1, dimensions + 2):
+        raise ShapeError(f"Inequality pair constraint array must have {dimensions + 1} or "
+                         f"{dimensions + 2} columns")
+    w = _weight_column(pts, dimensions + 1, "Inequality pairs constraint")
+    _require_finite(pts[:, :dimensions], "Position (X, Y, Z)")
+    _require_finite(pts[:, dimensions], "Pair id column")
+    return np.hstack([pts[:, :dimensions + 1], w[:, None]])

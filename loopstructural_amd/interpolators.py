@@ -6,7 +6,9 @@ LI/_discrete_interpolator.py, LI/_finite_difference_interpolator.py); the
 numerical work (point location, row assembly and normalisation, the
 matrix-free regularised normal operator, the PCG solve and evaluation) runs in
 the HIP kernels of libloopstructural_amd.  Constraint rows live in HBM; the
-regulariser is never materialised.
+regulariser is never materialised by the solvers (``build_matrix``,
+``constraints[name]['matrix']`` and callable solvers get an explicit copy on
+request).
 """
 from __future__ import annotations
 
@@ -20,9 +22,11 @@ from time import perf_counter
 
 import numpy as np
 import torch
+from scipy import sparse
 
 from . import _lib
 from . import constraints as cons
+from . import explicit
 from . import fd_system as fs
 from .diagnostics import (ConstraintDiagnosticsReport, ConstraintFamilyDiagnostics,
                           RegionCoverageDiagnostics)
@@ -56,15 +60,36 @@ class InterpolatorType(Enum):
     FINITE_DIFFERENCE_CONSTANT_NORM = "FINITE_DIFFERENCE_CONSTANT_NORM"
 
 
+class _Lazy:
+    """A ``constraints`` entry computed when it is first read."""
+
+    def __init__(self, fn):
+        self.fn = fn
+
+
 class _Constraint(dict):
-    """``constraints[name]`` entry: {"matrix", "b", "w"} with device tensors
-    converted to numpy on access (LI/_discrete_interpolator.py:383-389)."""
+    """``constraints[name]`` entry: {"matrix", "b", "w"} as in
+    LI/_discrete_interpolator.py:383-389.  Device tensors are returned as
+    numpy arrays.  Row blocks held on the device and matrix-free families
+    return ``matrix`` as a scipy CSC matrix of their normalised rows, built on
+    first access and kept."""
 
     def __getitem__(self, key):
         v = dict.__getitem__(self, key)
+        if key == "matrix" and hasattr(v, "to_scipy"):
+            m = getattr(self, "_csc", None)
+            if m is None:
+                m = self._csc = v.to_scipy().tocsc()
+            return m
+        if isinstance(v, _Lazy):
+            v = v.fn()
         if isinstance(v, torch.Tensor):
             return v.detach().cpu().numpy()
         return v
+
+    def raw(self, key):
+        """The stored object (device block, stencil family) behind ``key``."""
+        return dict.__getitem__(self, key)
 
     def get(self, key, default=None):
         return self[key] if key in self else default
@@ -72,14 +97,24 @@ class _Constraint(dict):
 
 class StencilFamily:
     """A regulariser family kept matrix-free: rows of one normalised mask at a
-    set of centre nodes (interior nodes, or one border layer)."""
+    set of centre nodes, the interior nodes or one border layer
+    (``layer`` = (axis, upper))."""
 
-    def __init__(self, name, kind, weight, nrows, dof):
+    def __init__(self, name, kind, weight, nrows, dof, mask=None, nnodes=None, layer=None):
         self.name, self.kind, self.weight, self.nrows, self.dof = name, kind, weight, nrows, dof
+        self.mask, self.nnodes, self.layer = mask, nnodes, layer
 
     @property
     def shape(self):
         return (self.nrows, self.dof)
+
+    def centres(self):
+        if self.layer is None:
+            return explicit.interior_centres(self.nnodes)
+        return explicit.layer_centres(self.nnodes, *self.layer)
+
+    def to_scipy(self):
+        return explicit.stencil_matrix(self.nnodes, self.mask, self.centres())
 
 
 def _mask_kind(mask):
@@ -99,6 +134,83 @@ def _mask_kind(mask):
         if np.allclose(m / nm, r, atol=1e-12) or np.allclose(m / nm, -r, atol=1e-12):
             return kind
     return None
+
+
+_ADMM_OPTIONS = ("adaptive_rho", "adaptive_rho_mu", "adaptive_rho_tau", "adaptive_rho_min",
+                 "adaptive_rho_max", "admm_weight_final", "admm_weight_schedule", "admm_abs_tol",
+                 "admm_rel_tol", "min_iterations", "active_set", "active_set_padding",
+                 "active_set_min_size", "active_set_max_size", "active_set_hysteresis",
+                 "active_set_refresh_interval", "model_update_tol", "return_history")
+
+
+class _AdmmOps:
+    """ADMM x-updates on the GPU system of an interpolator.
+
+    The inequality rows act on the scaled unknown ``y = S^-1 c`` (S the
+    column scaling of the ridge-augmented equality system, or I), so in node
+    space they are ``Q S^-1``.  They join the system's data rows with a
+    per-row weight; the x-update is then the system's own least-squares
+    problem with right-hand side ``v`` on those rows, solved by PCG from the
+    previous iterate to a relative tolerance ``step_rtol``."""
+
+    def __init__(self, interp, system, Q, multigrid, step_rtol):
+        dev = _lib.device()
+        self.system = system
+        self.multigrid = multigrid
+        self.step_rtol = step_rtol
+        self.iterations = 0
+        self.base = list(interp._rows.values())
+        if interp.apply_scaling_matrix:
+            d = system.diag()
+            self.S = torch.where(d > 0, 1.0 / torch.sqrt(d), torch.ones_like(d))
+        else:
+            self.S = torch.ones(interp.dof, dtype=torch.float64, device=dev)
+        Q = sparse.csr_matrix(Q)
+        Q.sum_duplicates()
+        n_ie = Q.shape[0]
+        counts = np.diff(Q.indptr)
+        k = max(int(counts.max()) if n_ie else 1, 1)
+        cols = np.zeros((n_ie, k), dtype=np.int64)
+        coef = np.zeros((n_ie, k))
+        pos = np.arange(Q.nnz) - np.repeat(Q.indptr[:-1], counts)
+        rows = np.repeat(np.arange(n_ie), counts)
+        cols[rows, pos] = Q.indices
+        coef[rows, pos] = Q.data
+        self.qcols = torch.as_tensor(cols, device=dev)
+        Sinv = 1.0 / self.S[self.qcols]
+        self.qcoef = torch.as_tensor(coef, device=dev) * Sinv  # Q S^-1
+        self.n_ie, self.dof = n_ie, interp.dof
+        self.base_b = torch.cat([blk.b for blk in self.base]) if self.base else \
+            torch.zeros(0, dtype=torch.float64, device=dev)
+        norm = torch.ones(n_ie, dtype=torch.float64, device=dev)
+        self._qblk = fs.fixed_width_rows(self.qcols.to(torch.int32).contiguous(),
+                                         self.qcoef.contiguous(),
+                                         torch.zeros(n_ie, dtype=torch.float64, device=dev),
+                                         torch.zeros(n_ie, dtype=torch.float64, device=dev),
+                                         norm, interp.dof)
+
+    def q(self, c):
+        return (self.qcoef * c[self.qcols]).sum(dim=1)
+
+    def set_weights(self, w):
+        self._qblk.w = w.contiguous()
+        self.system.set_rows(fs.concat_rows(self.base + [self._qblk], self.dof))
+        d = self.system.diag()
+        self.Sn = torch.where(d > 0, 1.0 / torch.sqrt(d), torch.ones_like(d))
+
+    def step(self, v, c):
+        rhs = self.system.rhs_for(torch.cat([self.base_b, v]))
+        r = rhs - self.system.apply(c)
+        thr = self.step_rtol * float(torch.linalg.vector_norm(self.Sn * rhs))
+        if float(torch.linalg.vector_norm(self.Sn * r)) <= thr:
+            return c
+        dc, info = self.system.solve(rtol=0.0, atol=thr, rhs=r, multigrid=self.multigrid,
+                                     check_every=8 if self.multigrid else 64)
+        self.iterations += info["iterations"]
+        return c + dc
+
+    def y_norm(self, x):
+        return float(torch.linalg.vector_norm(x / self.S))
 
 
 class GeologicalInterpolator:
@@ -218,22 +330,23 @@ class GeologicalInterpolator:
             raise type(e)(f"Failed to set interface constraints: {e}") from e
 
     def set_value_inequality_constraints(self, points):
-        """Stored for API compatibility; the inequality (ADMM) solver is not
-        part of this package and such constraints are ignored with a warning."""
-        pts = np.asarray(points, dtype=float)
-        if pts.ndim != 2 or pts.shape[1] not in (5, 6):
-            raise cons.ShapeError("Inequality constraint array must have 5 or 6 columns")
-        if pts.shape[1] == 5:
-            pts = np.hstack([pts, np.ones((pts.shape[0], 1))])
-        self.data["inequality"] = pts.copy()
-        self.up_to_date = False
+        """(N,5|6) [x,y,z,lower,upper(,w)], lower < upper (:464-493)."""
+        try:
+            pts = self._project_constraint_array(cons.inequality_array(points, self.dimensions))
+            self.data["inequality"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set inequality value constraints: {e}") from e
 
     def set_inequality_pairs_constraints(self, points):
-        pts = np.asarray(points, dtype=float)
-        if pts.ndim != 2 or pts.shape[1] not in (4, 5):
-            raise cons.ShapeError("Inequality pair array must have 4 or 5 columns")
-        self.data["inequality_pairs"] = pts[:, :4].copy()
-        self.up_to_date = False
+        """(N,4|5) [x,y,z,rock id(,w)] (:495-522)."""
+        try:
+            pts = self._project_constraint_array(
+                cons.inequality_pair_array(points, self.dimensions))
+            self.data["inequality_pairs"] = pts.copy()
+            self.up_to_date = False
+        except ValidationError as e:
+            raise type(e)(f"Failed to set inequality pairs constraints: {e}") from e
 
     def get_value_constraints(self):
         return self.data["value"]
@@ -513,6 +626,106 @@ class DiscreteInterpolator(GeologicalInterpolator):
                                   norm, self.dof)
         return self._register_rows(blk, name)
 
+    # ------------------------------------------------------- inequalities --
+    def add_inequality_constraints_to_matrix(self, A, bounds, idc, name="undefined"):
+        """Rows ``l <= A c <= u`` over the node ids ``idc``, bounds (n, 2|3)
+        (LI/_discrete_interpolator.py:467-507).  Kept on the host as a CSC
+        block: these rows are few next to the grid, and the ADMM solver
+        uploads them once per solve."""
+        A = np.asarray(A, dtype=float)
+        idc = np.asarray(idc, dtype=int)
+        bounds = np.asarray(bounds, dtype=float)
+        if A.ndim != 2 or idc.ndim != 2:
+            raise ValueError("A and idc must be 2D arrays")
+        if A.shape != idc.shape:
+            raise ValueError("A and idc must have the same shape")
+        if bounds.ndim != 2 or bounds.shape[0] != A.shape[0] or bounds.shape[1] not in (2, 3):
+            raise ValueError("bounds must have shape (n_rows, 2) or (n_rows, 3)")
+        if np.any(idc < 0) or np.any(idc >= self.dof):
+            raise ValueError("Inequality constraint indices are out of range")
+        rows = np.repeat(np.arange(idc.shape[0]), A.shape[1])
+        self.ineq_constraints[name] = {
+            "matrix": sparse.coo_matrix((A.ravel(), (rows, idc.ravel())),
+                                        shape=(idc.shape[0], self.dof)).tocsc(),
+            "bounds": bounds}
+
+    def _element_rows(self, coords):
+        """Shape-function values and node ids of the element holding each
+        point, and the inside flags (get_element_for_location + elements)."""
+        _, a, element, inside = self.support.get_element_for_location(coords)
+        return a[inside], self._element_nodes(element[inside]), inside
+
+    def _element_nodes(self, element):
+        return self.support.elements[element]
+
+    def add_value_inequality_constraints(self, w: float = 1.0):
+        """``lower <= f(p) <= upper`` at each inequality point inside the
+        support (LI/_discrete_interpolator.py:509-518)."""
+        points = self.get_inequality_value_constraints()
+        if points.shape[0] > 0:
+            a, cols, inside = self._element_rows(points[:, :self.support.dimension])
+            d = self.support.dimension
+            self.add_inequality_constraints_to_matrix(a, points[inside, d:d + 2], cols,
+                                                      "inequality_value")
+
+    def add_inequality_pairs_constraints(self, w: float = 1.0, upper_bound=1.0,
+                                         lower_bound=-np.inf, pairs=None):
+        """``lower_bound <= f(p_i) - f(q_j) <= upper_bound`` for every point
+        p_i of one rock id and q_j of another, per pair of ids (all ordered id
+        pairs seen when ``pairs`` is not given)
+        (LI/_discrete_interpolator.py:520-593)."""
+        points = self.get_inequality_pairs_constraints()
+        if points.shape[0] == 0:
+            return
+        d = self.support.dimension
+        if not pairs:
+            ids = np.unique(points[:, d])
+            pairs = list(dict.fromkeys(tuple(sorted((i, j))) for i in ids for j in ids if i != j))
+        for pair in pairs:
+            up = points[points[:, d] == pair[0]]
+            lo = points[points[:, d] == pair[1]]
+            a_up, c_up, in_up = self._element_rows(up[:, :d])
+            a_lo, c_lo, in_lo = self._element_rows(lo[:, :d])
+            if (~in_up).sum() > 0:
+                logger.warning(f"Upper points not in mesh {up[~in_up]}")
+            if (~in_lo).sum() > 0:
+                logger.warning(f"Lower points not in mesh {lo[~in_lo]}")
+            # every (upper i, lower j), j major: the order of meshgrid(i, j)
+            jj, ii = np.divmod(np.arange(a_up.shape[0] * a_lo.shape[0]), a_up.shape[0])
+            a = np.hstack([a_up[ii], -a_lo[jj]])
+            cols = np.hstack([c_up[ii], c_lo[jj]])
+            bounds = np.zeros((a.shape[0], 2))
+            bounds[:, 0] = lower_bound
+            bounds[:, 1] = upper_bound
+            self.add_inequality_constraints_to_matrix(
+                a, bounds, cols, f"inequality_pairs_{pair[0]}_{pair[1]}")
+
+    def add_inequality_feature(self, feature, lower=True, mask=None):
+        """Bound every node by a feature evaluated there: from below
+        (``lower``) or above, on the nodes of ``mask`` where the feature is
+        defined (LI/_discrete_interpolator.py:595-636)."""
+        value = feature(self.support.nodes)
+        if mask is None:
+            mask = np.ones(value.shape[0], dtype=bool)
+        lo = np.full(value.shape[0], -np.inf)
+        up = np.full(value.shape[0], np.inf)
+        mask = np.logical_and(mask, ~np.isnan(value))
+        if lower:
+            lo[mask] = value[mask]
+        else:
+            up[mask] = value[mask]
+        self.add_inequality_constraints_to_matrix(
+            np.ones((value.shape[0], 1)), np.column_stack([lo, up]),
+            np.arange(0, self.dof, dtype=int)[:, None], name="inequality_feature")
+
+    def build_inequality_matrix(self):
+        """(Q, bounds) stacked over the inequality families
+        (LI/_discrete_interpolator.py:874-884)."""
+        mats = [c["matrix"] for c in self.ineq_constraints.values()]
+        if not mats:
+            return sparse.csr_matrix((0, self.dof), dtype=float), np.zeros((0, 3))
+        return sparse.vstack(mats), np.vstack([c["bounds"] for c in self.ineq_constraints.values()])
+
     # ------------------------------------------------------------ solve --
     def _normalise_solver_choice(self, solver):
         """LI/_solver_strategy.py:15-29."""
@@ -529,6 +742,42 @@ class DiscreteInterpolator(GeologicalInterpolator):
 
     def _system_rows(self):
         return fs.concat_rows(list(self._rows.values()), self.dof)
+
+    def build_matrix(self):
+        """The assembled system ``A = [w * a_hat]``, ``b = [w * b_hat]`` over
+        the families in insertion order, as a scipy CSR matrix on the host
+        (LI/_discrete_interpolator.py:684-755).  The GPU solvers do not use
+        it; it is for callers that inspect the system or solve it themselves."""
+        A, B = explicit.stack(((c["matrix"], c["b"], c["w"]) for c in self.constraints.values()),
+                              self.dof)
+        logger.info(f"Interpolation matrix is {A.shape[0]} x {A.shape[1]}")
+        return A, B
+
+    def _solve_callable(self, solver, timing, started) -> bool:
+        """solve_with_callable (LI/_solver_strategy.py:32-45) on the explicit
+        preprocessed system: ridge rows and column scaling
+        (LI/_solver_pipeline.py:86-124), ``y = solver(A S, b)``, ``c = S y``
+        (LI/_discrete_interpolator.py:1288-1313)."""
+        t0 = perf_counter()
+        A, b = self.build_matrix()
+        timing["assembly_seconds"] = perf_counter() - t0
+        timing["matrix_rows"] = int(A.shape[0])
+        timing["matrix_cols"] = int(A.shape[1])
+        timing["matrix_nnz"] = int(A.nnz)
+        t1 = perf_counter()
+        A, b, S = explicit.preprocess(A, b, self.ridge_factor, self.add_ridge_regulatisation,
+                                      self.apply_scaling_matrix)
+        timing["preprocess_seconds"] = perf_counter() - t1
+        logger.warning("Using custom solver")
+        t2 = perf_counter()
+        c = np.asarray(solver(A.tocsr(), b))
+        timing["solve_seconds"] = perf_counter() - t2
+        self.c = S @ c if S is not None else c
+        self.up_to_date = True
+        timing["total_seconds"] = perf_counter() - started
+        timing["up_to_date"] = True
+        self.latest_solve_timing = timing
+        return True
 
     def _build_system(self):
         raise NotImplementedError
@@ -548,18 +797,24 @@ class DiscreteInterpolator(GeologicalInterpolator):
         started = perf_counter()
         choice = self._normalise_solver_choice(solver)
         kw = dict(solver_kwargs or {})
-        for k in ("constant_norm_iterations", "constant_norm_weight", "constant_norm_target"):
-            if kw.pop(k, None):
-                logger.warning("%s is not supported by the GPU solver; ignored", k)
+        cn_iters = max(0, int(kw.pop("constant_norm_iterations", 0)))
+        cn_weight = float(kw.pop("constant_norm_weight", 0.0))
+        cn_target = kw.pop("constant_norm_target", None)
+        if cn_target is not None:
+            cn_target = float(cn_target)
+            if cn_target <= 0.0:
+                logger.warning("constant_norm_target must be > 0; disabling explicit target and "
+                               "using auto mode")
+                cn_target = None
         kw.pop("backend", None)
         timing = {"solver": "callable" if callable(choice) else choice, "backend": "hip"}
-        if self.get_inequality_value_constraints().shape[0] or \
-                self.get_inequality_pairs_constraints().shape[0]:
-            logger.warning("Inequality constraints are not supported by the GPU path; ignored")
-        timing["inequality_rows"] = 0
-        if choice == "admm":
-            logger.warning("No inequality constraints, using lsmr")
-            choice = "lsmr"
+        self.solver_history = None
+        t_ie = perf_counter()
+        Q, bounds = self.build_inequality_matrix()
+        timing["inequality_seconds"] = perf_counter() - t_ie
+        timing["inequality_rows"] = int(Q.shape[0])
+        if callable(choice):
+            return self._solve_callable(choice, timing, started)
         t0 = perf_counter()
         system, nrows_total, nnz = self._build_system()
         torch.cuda.synchronize()
@@ -568,11 +823,11 @@ class DiscreteInterpolator(GeologicalInterpolator):
         timing["matrix_cols"] = int(self.dof)
         timing["matrix_nnz"] = int(nnz)
         timing["preprocess_seconds"] = 0.0
-        if callable(choice):
-            raise NotImplementedError(
-                "a callable solver needs the explicit sparse system; use solver='cg'")
         if choice == "lsmr":
             return self._solve_lsmr(self._lsmr_system(system), tol, kw, timing, started)
+        if choice == "admm":
+            return self._solve_admm(system, Q, bounds, kw, timing, started,
+                                    (cn_iters, cn_weight, cn_target))
         if ("atol" not in kw or "rtol" not in kw) and tol is not None:
             kw["atol"] = tol
         for k in ("x0", "M", "callback"):
@@ -602,6 +857,158 @@ class DiscreteInterpolator(GeologicalInterpolator):
         timing["up_to_date"] = True
         self.latest_solve_timing = timing
         return self.up_to_date
+
+    def _solve_admm(self, system, Q, bounds, kw, timing, started, cn) -> bool:
+        """solve_with_admm (LI/_solver_strategy.py:213-247) and
+        _solve_with_admm (LI/_discrete_interpolator.py:1171-1219): the
+        reference's ADMM on the preprocessed system, every x-update solved
+        on the GPU operator, then the optional constant-norm polish.  Without
+        inequality rows the reference's ADMM set-up raises, which
+        _solve_with_admm logs and reports as a failed solve."""
+        logger.info("Solving using admm")
+        cn_kwargs = dict(kw)
+        x0 = kw.pop("x0", None)
+        kw.pop("linsys_solver", None)
+        opts = {k: kw.pop(k) for k in list(kw) if k in _ADMM_OPTIONS}
+        for k in ("batch_size", "batch_fraction", "random_seed"):
+            kw.pop(k, None)
+        admm_weight = kw.pop("admm_weight", 0.01)
+        nmajor = kw.pop("nmajor", 200)
+        step_rtol = float(kw.pop("step_rtol", 1e-10))
+        pre = str(kw.pop("preconditioner", "jacobi")).lower()
+        for k in kw:
+            logger.info("solver_kwargs[%r] sets the reference's inner LSMR; the GPU x-update "
+                        "is solved to step_rtol instead", k)
+        t1 = perf_counter()
+        try:
+            if Q.shape[0] == 0:
+                raise ValueError("nelements must be greater than 0")
+            ops = _AdmmOps(self, system, Q, pre in ("mg", "multigrid"), step_rtol)
+            if x0 is not None:
+                y0 = torch.as_tensor(np.asarray(x0(self.support), dtype=float),
+                                     device=_lib.device())
+                c0 = y0 * ops.S
+            else:
+                c0 = torch.zeros(self.dof, dtype=torch.float64, device=_lib.device())
+            from .admm import admm_solve
+            b = torch.as_tensor(np.asarray(bounds, dtype=float), device=_lib.device())
+            res = admm_solve(ops, b, c0, admm_weight=admm_weight, nmajor=nmajor, **opts)
+        except ValueError as e:
+            logger.error(f"ADMM solver failed: {e}")
+            timing["solve_seconds"] = perf_counter() - t1
+            return self._finish(False, timing, started)
+        torch.cuda.synchronize()
+        timing["solve_seconds"] = perf_counter() - t1
+        timing["step_iterations"] = ops.iterations
+        c, history = (res if isinstance(res, tuple) else (res, None))
+        self.c = c.cpu().numpy()
+        self.solver_history = history
+        self._finish(True, timing, started)
+        iters, weight, target = cn
+        if iters > 0 and weight > 0.0:
+            self._run_constant_norm_polish(cn_kwargs, iters, weight, target)
+        return True
+
+    def _finish(self, ok, timing, started) -> bool:
+        self.up_to_date = bool(ok)
+        timing["total_seconds"] = perf_counter() - started
+        timing["up_to_date"] = bool(ok)
+        self.latest_solve_timing = timing
+        return self.up_to_date
+
+    def _remove_constraints_with_prefix(self, prefix: str) -> None:
+        for name in [n for n in self.constraints if n.startswith(prefix)]:
+            self.constraints.pop(name, None)
+            self._rows.pop(name, None)
+
+    def _constant_norm_gradient_data(self):
+        """Gradient of the current solution at every element barycentre,
+        with the element gradient rows, node ids and volumes, keeping elements
+        whose gradient norm is finite and above 1e-10
+        (LI/_discrete_interpolator.py:886-936).  On the device."""
+        sup = self.support
+        try:
+            _, grad, elements, inside = sup.get_element_gradient_for_location(sup.barycentre)
+        except (AttributeError, TypeError, ValueError) as err:
+            logger.debug("Unable to build constant-norm gradient rows: %s", err)
+            return None
+        inside = np.asarray(inside, dtype=bool)
+        if not np.any(inside):
+            return None
+        dev = _lib.device()
+        element_ids = np.asarray(elements, dtype=int)[inside]
+        T = torch.as_tensor(np.asarray(grad, dtype=float)[inside], device=dev)
+        idc = torch.as_tensor(np.asarray(self._element_nodes(element_ids), dtype=np.int64),
+                              device=dev)
+        c = torch.as_tensor(np.asarray(self.c, dtype=float), device=dev)
+        gvec = torch.einsum("ijk,ik->ij", T, c[idc])
+        gnorm = torch.linalg.vector_norm(gvec, dim=1)
+        valid = torch.isfinite(gnorm) & (gnorm > 1e-10)
+        if not bool(valid.any()):
+            return None
+        size = np.asarray(sup.element_size, dtype=float)
+        if size.ndim == 0:
+            vol = torch.full((int(valid.sum()),), float(size), dtype=torch.float64, device=dev)
+        else:
+            vol = torch.as_tensor(size[element_ids], device=dev)[valid].clamp(min=1e-12)
+        return {"gradient": T[valid], "grad_vec": gvec[valid], "grad_norm": gnorm[valid],
+                "idc": idc[valid], "volume": vol}
+
+    def _run_constant_norm_polish(self, solver_kwargs, iterations, base_weight, target_norm):
+        """Pull the gradient norm towards a constant: rows ``g_hat . grad f =
+        target`` in every element, weight ``base_weight * (k + 1)`` in round k,
+        each round an ADMM solve from the current solution
+        (LI/_discrete_interpolator.py:938-1004)."""
+        if iterations <= 0 or base_weight <= 0.0:
+            return
+        if not np.all(np.isfinite(self.c)):
+            logger.warning("Skipping constant-norm polish because current solution is not finite")
+            return
+        prefix = "__constant_norm_polish__"
+        self._remove_constraints_with_prefix(prefix)
+        stable = dict(solver_kwargs or {})
+        stable.pop("x0", None)
+        for i in range(int(iterations)):
+            data = self._constant_norm_gradient_data()
+            if data is None:
+                logger.warning("Constant-norm polish stopped: gradient rows unavailable")
+                break
+            if target_norm is None:
+                target = float(np.median(data["grad_norm"].cpu().numpy()))
+            else:
+                target = float(target_norm)
+            unit = data["grad_vec"] / data["grad_norm"][:, None]
+            A = torch.einsum("ij,ijk->ik", unit, data["gradient"]) / data["volume"][:, None]
+            b = torch.full((A.shape[0],), target, dtype=torch.float64,
+                           device=A.device) / data["volume"]
+            self._add_device_rows(A, b, data["idc"], float(base_weight) * float(i + 1),
+                                  f"{prefix}{i}")
+            seed = np.asarray(self.c, dtype=float).copy()
+            kw = dict(stable)
+            kw["x0"] = lambda _support, x0=seed: np.array(x0, copy=True)
+            kw["constant_norm_iterations"] = 0
+            if not self.solve_system("admm", solver_kwargs=kw):
+                logger.warning("Constant-norm polish terminated because ADMM solve failed")
+                break
+            upd = self._constant_norm_gradient_data()
+            if upd is not None:
+                gn = upd["grad_norm"]
+                logger.info("Constant-norm iter %d: target=%.3e, grad_norm mean=%.3e, std=%.3e",
+                            i + 1, target, float(gn.mean()), float(gn.std(unbiased=False)))
+        self._remove_constraints_with_prefix(prefix)
+
+    def _add_device_rows(self, A, b, idc, w, name):
+        """add_constraints_to_least_squares for rows already on the device."""
+        A = A.contiguous()
+        b = b.contiguous().clone()
+        R, k = A.shape
+        norm = torch.empty(R, dtype=torch.float64, device=A.device)
+        if R:
+            _lib.call("lsamd_rows_normalise", R, k, _lib.ptr(A), _lib.ptr(b), _lib.ptr(norm),
+                      _lib.stream())
+        wv = torch.full((R,), float(w), dtype=torch.float64, device=A.device)
+        blk = fs.fixed_width_rows(idc.to(torch.int32).contiguous(), A, b, wv, norm, self.dof)
+        return self._register_rows(blk, name)
 
     def _lsmr_system(self, system):
         """The system LSMR iterates on; FD systems hold their regulariser
@@ -667,7 +1074,7 @@ class DiscreteInterpolator(GeologicalInterpolator):
 
     # ------------------------------------------------------ diagnostics --
     @staticmethod
-    def _family_of(name):
+    def _family_of(name, inequality=False):
         """LI/_discrete_interpolator.py:179-201."""
         n = name.lower()
         for pre, fam in (("value", "value"), ("gradient", "gradient"), ("norm", "normal"),
@@ -678,7 +1085,7 @@ class DiscreteInterpolator(GeologicalInterpolator):
                 return fam
         if n.startswith("d") and len(n) in (3, 4, 5, 6, 7, 8):
             return "regularisation"
-        return "other"
+        return "inequality" if inequality else "other"
 
     def _build_constraint_diagnostics_report(self):
         """LI/_discrete_interpolator.py:203-278."""
@@ -701,6 +1108,8 @@ class DiscreteInterpolator(GeologicalInterpolator):
                 w = np.atleast_1d(np.asarray(w, dtype=float))
                 if w.size:
                     wts[fam].append(w)
+        for name, con in self.ineq_constraints.items():
+            rows[self._family_of(name, inequality=True)] += int(con["matrix"].shape[0])
         fams = {}
         for fam in sorted(set(raw) | set(rows)):
             rc = int(rows.get(fam, 0))
@@ -821,10 +1230,22 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
         self.add_value_constraints(self.interpolation_weights["cpw"])
         self.add_tangent_constraints(self.interpolation_weights["tpw"])
         self.add_interface_constraints(self.interpolation_weights["ipw"])
+        self.add_value_inequality_constraints()
+        self.add_inequality_pairs_constraints(
+            pairs=kwargs.get("inequality_pairs", None),
+            upper_bound=kwargs.get("inequality_pair_upper_bound", np.finfo(float).eps),
+            lower_bound=kwargs.get("inequality_pair_lower_bound", -np.inf))
         for k, o in operators.items():
             self.assemble_inner(o[0], o[1], name=k)
         self.assemble_borders()
         return self.finalize_setup_diagnostics_report()
+
+    def _element_nodes(self, element):
+        """Corner node ids of the given cells, without building the table of
+        every cell."""
+        g = self.support
+        return g.global_node_indices(g.cell_corner_indexes(
+            g.global_index_to_cell_index(np.asarray(element))))
 
     # ------------------------------------------------------- data rows ----
     def _inside_points(self, points, ncols):
@@ -978,15 +1399,15 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
         n = self.support.nsteps
         nrows = int(np.prod(n - 2)) if np.all(n >= 3) else 0
         name = self._unique_name(name)
-        fam = StencilFamily(name, kind, float(w), nrows, self.dof)
-        self._stencils[name] = fam
-        self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(nrows),
-                                             w=np.full(nrows, float(w)))
+        fam = StencilFamily(name, kind, float(w), nrows, self.dof, np.asarray(operator, float),
+                            tuple(int(v) for v in n))
+        self._add_stencil_family(fam)
 
     def assemble_borders(self):
         """Border layers 1 and n-2 with the axial masks, weights dx/dy/dz
         (:1052-1144); skipped when the weight is 0 or the axis has < 3 nodes."""
         n = self.support.nsteps
+        masks = (Operator.Dxx_mask, Operator.Dyy_mask, Operator.Dzz_mask)
         for axis, key, kind in ((0, "dx", "bx"), (1, "dy", "by"), (2, "dz", "bz")):
             weight = self.interpolation_weights[key]
             if weight == 0 or n[axis] < 3:
@@ -994,10 +1415,25 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
             nrows = int(np.prod(n) // n[axis])
             for side in ("lower", "upper"):
                 name = self._unique_name(f"{key}_{side}")
-                fam = StencilFamily(name, kind, float(weight), nrows, self.dof)
-                self._stencils[name] = fam
-                self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(nrows),
-                                                     w=np.full(nrows, float(weight)))
+                fam = StencilFamily(name, kind, float(weight), nrows, self.dof,
+                                    np.asarray(masks[axis], float), tuple(int(v) for v in n),
+                                    (axis, side == "upper"))
+                self._add_stencil_family(fam)
+
+    def _add_stencil_family(self, fam):
+        """Record a matrix-free family; its row weights are the family weight,
+        times the per-node regularisation scale at each row's centre when
+        use_regularisation_weight_scale is set (:670-674)."""
+        self._stencils[fam.name] = fam
+
+        def weights():
+            w = np.full(fam.nrows, fam.weight)
+            if self.use_regularisation_weight_scale:
+                w = self.regularisation_scale[fam.centres()] * fam.weight
+            return w
+
+        self.constraints[fam.name] = _Constraint(matrix=fam, b=np.zeros(fam.nrows),
+                                                 w=_Lazy(weights))
 
     def _reg_struct(self):
         r = _lib.FdReg()
@@ -1047,12 +1483,17 @@ class EdgeJumpFamily:
     """The P1 edge-jump regulariser kept matrix-free: one normalised 8-entry
     row per face-sharing tet pair, uniform weight."""
 
-    def __init__(self, name, weight, nrows, dof):
+    def __init__(self, name, weight, nrows, dof, grid=None):
         self.name, self.weight, self.nrows, self.dof = name, weight, nrows, dof
+        self.grid = grid
 
     @property
     def shape(self):
         return (self.nrows, self.dof)
+
+    def to_scipy(self):
+        """The normalised rows from the GPU face table, in pair order."""
+        return fs.p1_edge_rows(self.grid, 1.0, self.dof).to_scipy()
 
 
 class P1Interpolator(DiscreteInterpolator):
@@ -1161,7 +1602,7 @@ class P1Interpolator(DiscreteInterpolator):
             self._register_rows(blk, name)
         else:
             name = self._unique_name(name)
-            fam = EdgeJumpFamily(name, float(w), P, self.dof)
+            fam = EdgeJumpFamily(name, float(w), P, self.dof, grid)
             self._edge_families[name] = fam
             self.constraints[name] = _Constraint(matrix=fam, b=np.zeros(P), w=np.full(P, float(w)))
         self.up_to_date = False
@@ -1202,6 +1643,8 @@ class P1Interpolator(DiscreteInterpolator):
         self.add_norm_constraints(self.interpolation_weights["npw"])
         self.add_value_constraints(self.interpolation_weights["cpw"])
         self.add_tangent_constraints(self.interpolation_weights["tpw"])
+        self.add_value_inequality_constraints()
+        self.add_inequality_pairs_constraints()
         return self.finalize_setup_diagnostics_report()
 
     def add_tangent_constraints(self, w=1.0):

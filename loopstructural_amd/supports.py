@@ -273,6 +273,13 @@ class BaseStructuredSupport:
                         [0, 1, 1], [1, 1, 1]])
         return ci[:, None, :] + off[None, :, :]
 
+    @property
+    def elements(self):
+        """(n_cells, 8) corner node ids of every cell, cells x fastest
+        (LC/supports/_3d_base_structured.py:216-221)."""
+        cells = self.global_index_to_cell_index(np.arange(self.n_elements))
+        return self.global_node_indices(self.cell_corner_indexes(cells))
+
     def node_indexes_to_position(self, node_indexes):
         ni = np.asarray(node_indexes)
         shape = ni.shape
