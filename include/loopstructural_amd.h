@@ -53,11 +53,20 @@ extern "C" {
 #define LSAMD_ROWS_GRADIENT 2 /* 2 rows / point: strike.T, dip.T, b = 0       */
 #define LSAMD_ROWS_TANGENT 3  /* 1 row / point: t_hat.T, b = 0                */
 
-/* Structured grid: nnodes = nsteps_cells + 1 per axis. */
+/* Structured grid: nnodes = nsteps_cells + 1 per axis, node i at
+ * origin + i * step.  With axis_nodes[0..2] all set the grid is rectilinear
+ * (LC/supports/_3d_rectilinear_grid.py): node i of axis a is the DEVICE value
+ * axis_nodes[a][i] (nnodes[a] strictly increasing values), origin and step
+ * are not used by point location, row building and evaluation, points are
+ * inside only strictly between the first and last node, local coordinates
+ * are not clipped, and the regulariser of an FD system on it comes as rows
+ * (its stencil weights zero, the border families excepted).  P1 meshes and
+ * batches take uniform grids only (axis_nodes all NULL). */
 typedef struct lsamd_grid {
   double origin[3];
   double step[3];
   int64_t nnodes[3];
+  const double* axis_nodes[3];
 } lsamd_grid;
 
 /* Regulariser weights, all already squared: w2_xx = dxx^2, w2_xy = (dxy/4)^2,

@@ -7,14 +7,14 @@ from .diagnostics import (ConstraintDiagnosticsReport, ConstraintFamilyDiagnosti
                           RegionCoverageDiagnostics)
 from .interpolators import (DiscreteInterpolator, FiniteDifferenceInterpolator,
                             GeologicalInterpolator, InterpolatorType, P1Interpolator)
-from .supports import Operator, StructuredGrid, SupportType, TetMesh
+from .supports import Operator, RectilinearGrid, StructuredGrid, SupportType, TetMesh
 from .factory import (InterpolatorFactory, SupportFactory, interpolator_map,
                       interpolator_string_map, support_interpolator_map, support_map)
 
 __all__ = [
     "ConstraintDiagnosticsReport", "ConstraintFamilyDiagnostics", "DiscreteInterpolator",
     "DtypeError", "FiniteDifferenceInterpolator", "FiniteValueError", "GeologicalInterpolator",
-    "InterpolatorFactory", "InterpolatorType", "Operator", "P1Interpolator", "RegionCoverageDiagnostics", "ShapeError", "StructuredGrid",
+    "InterpolatorFactory", "InterpolatorType", "Operator", "P1Interpolator", "RegionCoverageDiagnostics", "ShapeError", "StructuredGrid", "RectilinearGrid",
     "SupportFactory", "SupportType", "TetMesh", "UnsupportedCombinationError", "ValidationError", "VectorError",
     "WeightError", "interpolator_map", "interpolator_string_map", "support_interpolator_map",
     "support_map",

@@ -33,7 +33,7 @@ class LsamdError(RuntimeError):
 
 class Grid(C.Structure):
     _fields_ = [("origin", C.c_double * 3), ("step", C.c_double * 3),
-                ("nnodes", C.c_int64 * 3)]
+                ("nnodes", C.c_int64 * 3), ("axis_nodes", C.c_void_p * 3)]
 
 
 class FdReg(C.Structure):

@@ -46,12 +46,24 @@ int injected(const char* site) {
 }
 int check_grid(const lsamd_grid* g) {
   if (!g) return fail(LSAMD_ERR_ARG, "grid descriptor is NULL");
+  const int given = (g->axis_nodes[0] != nullptr) + (g->axis_nodes[1] != nullptr) +
+                    (g->axis_nodes[2] != nullptr);
+  if (given != 0 && given != 3)
+    return fail(LSAMD_ERR_ARG, "a rectilinear grid needs node arrays on all three axes");
   for (int a = 0; a < 3; ++a) {
     if (g->nnodes[a] < 2) return fail(LSAMD_ERR_ARG, "grid needs >= 2 nodes per axis");
-    if (!(g->step[a] != 0.0)) return fail(LSAMD_ERR_ARG, "grid step must be non-zero");
+    if (given == 0 && !(g->step[a] != 0.0))
+      return fail(LSAMD_ERR_ARG, "grid step must be non-zero");
   }
   if (g->nnodes[0] * g->nnodes[1] * g->nnodes[2] > (int64_t)1 << 40)
     return fail(LSAMD_ERR_ARG, "grid too large");
+  return LSAMD_OK;
+}
+
+int check_uniform_grid(const lsamd_grid* g, const char* fn) {
+  if (int rc = check_grid(g)) return rc;
+  if (g->axis_nodes[0])
+    return fail(LSAMD_ERR_ARG, std::string(fn) + ": needs a uniform grid (axis_nodes NULL)");
   return LSAMD_OK;
 }
 
@@ -329,6 +341,123 @@ __global__ __launch_bounds__(256) void k_p1_stencil(const T* __restrict__ x, T* 
   }
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// P1 edge-jump regulariser, plane-streaming.  A block covers a TXV x (512 /
+// TXV) xy tile and kP1Zc output planes; the planes from two below to two
+// above the chunk stream through a double-buffered LDS tile with a 2-node xy
+// halo (D planes of global loads in flight in registers, out-of-grid values
+// zeroed).  Each plane, as it arrives, adds to the five output planes of the
+// column that it touches: its in-plane taps (+-1, +-2 along x and y, and the
+// centre) to the output in the same plane, its centre and four diagonal
+// nodes to the outputs one plane above and below, its centre to those two
+// planes away.  The output two planes below is then complete and stored.
+// Coefficients come from the operator table row of each output node; the
+// table is small and stays cache resident.  One x.y partial per block.
+constexpr int kP1Zc = 16;
+
+template <typename T, int TXV>
+__global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __restrict__ y,
+                                                  GridDev g, const T* __restrict__ tab, int nvx,
+                                                  int nvy, T ridge2, int ntx, int nzc,
+                                                  double* __restrict__ part, const CgCtl* ctl) {
+  if (ctl->done) return;
+  constexpr int TYV = 512 / TXV, QX = TXV + 4, QY = TYV + 4, QS = QX * QY, D = 2;
+  __shared__ T pl[2][QS];
+  __shared__ double shm[8];
+  const int bz = (int)(blockIdx.x % (unsigned)nzc);
+  const int tt = (int)(blockIdx.x / (unsigned)nzc);
+  const int x0 = (tt % ntx) * TXV, y0 = (tt / ntx) * TYV;
+  const int z0 = bz * kP1Zc, z1 = min(z0 + kP1Zc, g.nz);
+  const int lx = threadIdx.x % TXV, ly = threadIdx.x / TXV;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool col = i < g.nx && j < g.ny;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  // two load slots per thread in the (TXV+4) x (TYV+4) plane tile
+  const int e1 = threadIdx.x + 512;
+  const bool has1 = e1 < QS;
+  int64_t go0, go1;
+  bool ok0, ok1;
+  {
+    const int ey = threadIdx.x / QX, ex = threadIdx.x - ey * QX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok0 = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go0 = ok0 ? (int64_t)gx + (int64_t)gy * g.nx : 0;
+  }
+  {
+    const int ee = has1 ? e1 : 0;
+    const int ey = ee / QX, ex = ee - ey * QX;
+    const int gx = x0 - 2 + ex, gy = y0 - 2 + ey;
+    ok1 = has1 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    go1 = ok1 ? (int64_t)gx + (int64_t)gy * g.nx : 0;
+  }
+  // table row of node (i, j, z): rowxy + virtual(z) * zstride
+  const int vx = p1_virtual(min(i, g.nx - 1), g.nx), vy = p1_virtual(min(j, g.ny - 1), g.ny);
+  const T* rowxy = tab + ((int64_t)vy * nvx + vx) * kP1Taps;
+  const int64_t zstride = (int64_t)nvy * nvx * kP1Taps;
+  const int pb = z0 - 2, NS = z1 - z0 + 4;  // planes pb .. pb + NS - 1
+  T qa[D], qb[D];
+  auto ld = [&](int p, T& va, T& vb) {
+    const bool pin = p >= 0 && p < g.nz;
+    const int64_t po = (int64_t)(pin ? p : 0) * nxy;
+    const T a = x[go0 + po], b = x[go1 + po];
+    va = (pin && ok0) ? a : (T)0;
+    vb = (pin && ok1) ? b : (T)0;
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) ld(pb + d, qa[d], qb[d]);
+  // pending outputs z = p-2 .. p+2 and the centre values of planes p-2, p-1
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, h2 = 0, h1 = 0;
+  double dot = 0.0;
+  pl[0][threadIdx.x] = qa[0];
+  if (has1) pl[0][e1] = qb[0];
+  __syncthreads();
+  for (int s = 0; s < NS; ++s) {
+    const int p = pb + s;
+    if (s + D < NS) ld(p + D, qa[s % D], qb[s % D]);
+    const T* bb = pl[s & 1] + (ly + 2) * QX + lx + 2;
+    const T c = bb[0];
+    if (col && p >= 0 && p < g.nz) {
+      if (p >= z0 && p < z1) {
+        const T* C = rowxy + (int64_t)p1_virtual(p, g.nz) * zstride;
+        a2 += C[0] * c + C[1] * bb[1] + C[2] * bb[-1] + C[3] * bb[QX] + C[4] * bb[-QX] +
+              C[7] * bb[2] + C[8] * bb[-2] + C[9] * bb[2 * QX] + C[10] * bb[-2 * QX] + ridge2 * c;
+      }
+      if (p - 1 >= z0 && p - 1 < z1) {  // this plane is dz = +1 of output p-1
+        const T* C = rowxy + (int64_t)p1_virtual(p - 1, g.nz) * zstride;
+        a1 += C[5] * c + C[17] * bb[-1 - QX] + C[18] * bb[1 - QX] + C[19] * bb[-1 + QX] +
+              C[20] * bb[1 + QX];
+      }
+      if (p + 1 >= z0 && p + 1 < z1) {  // dz = -1 of output p+1
+        const T* C = rowxy + (int64_t)p1_virtual(p + 1, g.nz) * zstride;
+        a3 += C[6] * c + C[13] * bb[-1 - QX] + C[14] * bb[1 - QX] + C[15] * bb[-1 + QX] +
+              C[16] * bb[1 + QX];
+      }
+      if (p - 2 >= z0 && p - 2 < z1)
+        a0 += rowxy[(int64_t)p1_virtual(p - 2, g.nz) * zstride + 11] * c;
+      if (p + 2 >= z0 && p + 2 < z1)
+        a4 += rowxy[(int64_t)p1_virtual(p + 2, g.nz) * zstride + 12] * c;
+    }
+    const int zo = p - 2;  // complete now
+    if (col && zo >= z0 && zo < z1) {
+      __builtin_nontemporal_store(a0, &y[(int64_t)i + (int64_t)j * g.nx + (int64_t)zo * nxy]);
+      dot += (double)h2 * (double)a0;
+    }
+    a0 = a1;
+    a1 = a2;
+    a2 = a3;
+    a3 = a4;
+    a4 = 0;
+    h2 = h1;
+    h1 = (p >= 0 && p < g.nz) ? c : (T)0;
+    if (s + 1 < NS) {
+      pl[(s + 1) & 1][threadIdx.x] = qa[(s + 1) % D];
+      if (has1) pl[(s + 1) & 1][e1] = qb[(s + 1) % D];
+    }
+    __syncthreads();
+  }
+  const double bsum = block_sum(dot, shm);
+  if (threadIdx.x == 0) part[blockIdx.x] = bsum;
 }
 
 __global__ void k_p1_diag(GridDev g, const double* __restrict__ tab, int nvx, int nvy,
@@ -669,8 +798,10 @@ __global__ __launch_bounds__(256) void k_nearest_d2(GridDev g, const double* __r
     const double st = (mx - o) / (double)(nn - 1);
     return (double)i * st + o;
   };
-  const double x = lin(i, g.nx, g.ox, g.hx), y = lin(j, g.ny, g.oy, g.hy),
-               z = lin(k, g.nz, g.oz, g.hz);
+  const bool rect = g.xn != nullptr;
+  const double x = rect ? g.xn[i < g.nx ? i : g.nx - 1] : lin(i, g.nx, g.ox, g.hx);
+  const double y = rect ? g.yn[j < g.ny ? j : g.ny - 1] : lin(j, g.ny, g.oy, g.hy);
+  const double z = rect ? g.zn[k < g.nz ? k : g.nz - 1] : lin(k, g.nz, g.oz, g.hz);
   double best = __builtin_inf();
   for (int64_t t0 = 0; t0 < m; t0 += 256) {
     __syncthreads();
@@ -760,6 +891,7 @@ struct lsamd_fd_system {
   Coef25<float> ku32;
   int op = 0;                  // regulariser: 0 FD second derivatives, 1 P1 edge jumps
   int p1nv[3] = {0, 0, 0};     // P1 virtual axis lengths of the table
+  int p1tx = 0;                // P1 stencil tile width (64 or 32; 0: one node per thread)
   double* p1tab64 = nullptr;
   float* p1tab32 = nullptr;
   double p1_ridge2 = 0.0;
@@ -889,8 +1021,36 @@ constexpr int kZcCand[3] = {8, 16, 32};
 
 int zc_index(int zc) { return zc == 8 ? 0 : (zc == 16 ? 1 : 2); }
 
+// P1 plane-streaming stencil: tiles per axis and z-chunks of the launch
+void p1_tiling(const lsamd_fd_system* s, int tx, int* ntx, int* nty, int* nzc) {
+  *ntx = (int)ceil_div(s->gd.nx, tx);
+  *nty = (int)ceil_div(s->gd.ny, 512 / tx);
+  *nzc = (int)ceil_div(s->gd.nz, kP1Zc);
+}
+
+// Tile width of the P1 stencil: the one of 64x8 and 32x16 that covers the xy
+// plane with fewer padding nodes (64 on a tie); LSAMD_P1_STENCIL=node selects
+// the one-node-per-thread kernel.
+int p1_choose_tile(const lsamd_fd_system* s) {
+  if (const char* e = getenv("LSAMD_P1_STENCIL"))
+    if (strcmp(e, "node") == 0) return 0;
+  int64_t area[2];
+  const int cand[2] = {64, 32};
+  for (int c = 0; c < 2; ++c) {
+    int ntx, nty, nzc;
+    p1_tiling(s, cand[c], &ntx, &nty, &nzc);
+    area[c] = (int64_t)ntx * cand[c] * nty * (512 / cand[c]);
+  }
+  return area[1] < area[0] ? 32 : 64;
+}
+
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
   if (s->op == 2) return std::max(s->nblk, s->nbtile);
+  if (s->op == 1 && s->p1tx) {
+    int ntx, nty, nzc;
+    p1_tiling(s, s->p1tx, &ntx, &nty, &nzc);
+    return (int64_t)ntx * nty * nzc;
+  }
   if (!s->regular) return ceil_div(s->n, 256);
   return s->nstile[zc_index(zc)];
 }
@@ -927,9 +1087,21 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
   }
   if (s->op == 1) {
     const bool f32 = sizeof(T) == 4;
+    const T* tab = f32 ? (const T*)s->p1tab32 : (const T*)s->p1tab64;
+    if (s->p1tx) {
+      int ntx, nty, nzc;
+      p1_tiling(s, s->p1tx, &ntx, &nty, &nzc);
+      const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
+      if (s->p1tx == 64)
+        hipLaunchKernelGGL((k_p1_tiles<T, 64>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, ntx, nzc, s->part, ctl);
+      else
+        hipLaunchKernelGGL((k_p1_tiles<T, 32>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, ntx, nzc, s->part, ctl);
+      return nb;
+    }
     hipLaunchKernelGGL(k_p1_stencil<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
-                       y, s->gd, f32 ? (const T*)s->p1tab32 : (const T*)s->p1tab64, s->p1nv[0],
-                       s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
+                       y, s->gd, tab, s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
     return ceil_div(s->n, 256);
   }
   if (!s->regular) {
@@ -2859,7 +3031,7 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
   if (int rc = injected("p1_system_create")) return rc;
   if (!out) return fail(LSAMD_ERR_ARG, "lsamd_p1_system_create: NULL argument");
   *out = nullptr;
-  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_uniform_grid(g, "lsamd_p1_system_create")) return rc;
   if (!(cgw >= 0.0) || !(ridge2 >= 0.0))
     return fail(LSAMD_ERR_ARG, "lsamd_p1_system_create: cgw and ridge2 must be >= 0");
   lsamd_fd_system* s = new lsamd_fd_system();
@@ -2869,6 +3041,7 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
   s->n = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
   s->op = 1;
   s->p1_ridge2 = ridge2;
+  s->p1tx = p1_choose_tile(s);
   auto bail = [&](int rc) {
     lsamd_fd_system_destroy(s);
     return rc;
@@ -3395,7 +3568,7 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   int64_t off = 0;
   int64_t nb = 0;
   for (int i = 0; i < nsys; ++i) {
-    if (int rc = check_grid(&grids[i])) return rc;
+    if (int rc = check_uniform_grid(&grids[i], "lsamd_batch_create")) return rc;
     for (int a = 0; a < 3; ++a)
       if (grids[i].nnodes[a] < 7)
         return fail(LSAMD_ERR_ARG, "lsamd_batch_create: every grid needs >= 7 nodes per axis");

@@ -34,6 +34,7 @@ struct Located {
   bool inside;
   int cx, cy, cz;
   double lx, ly, lz;
+  double hx, hy, hz;  // size of the cell along each axis
 };
 
 __device__ inline bool inside_axis(double p, double o, double h, int nn) {
@@ -53,8 +54,45 @@ __device__ inline int cell_axis(double p, double o, double h, int nn) {
   return (int)c;
 }
 
+// Rectilinear axis (LC/supports/_3d_rectilinear_grid.py:115-159): cell =
+// clip(searchsorted(nodes, p, 'right') - 1, 0, ncells - 1), local coordinate
+// (p - x0) / h unclipped with h = x1 - x0 (1 where 0), inside strictly
+// between the first and last node.
+__device__ inline bool rect_axis(const double* nodes, int nn, double p, int* cell, double* loc,
+                                 double* h) {
+  int lo = 0, hi = nn;
+  while (lo < hi) {  // number of nodes <= p
+    const int mid = (lo + hi) >> 1;
+    if (nodes[mid] <= p)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  int c = lo - 1;
+  if (c < 0) c = 0;
+  if (c > nn - 2) c = nn - 2;
+  const double x0 = nodes[c];
+  const double x1 = nodes[c + 1 < nn ? c + 1 : nn - 1];
+  double w = x1 - x0;
+  if (w == 0.0) w = 1.0;
+  *cell = c;
+  *loc = (p - x0) / w;
+  *h = w;
+  return (p > nodes[0]) && (p < nodes[nn - 1]);
+}
+
 __device__ inline Located locate(const GridDev& g, double px, double py, double pz) {
   Located L;
+  if (g.xn) {
+    const bool ix = rect_axis(g.xn, g.nx, px, &L.cx, &L.lx, &L.hx);
+    const bool iy = rect_axis(g.yn, g.ny, py, &L.cy, &L.ly, &L.hy);
+    const bool iz = rect_axis(g.zn, g.nz, pz, &L.cz, &L.lz, &L.hz);
+    L.inside = ix && iy && iz;
+    return L;
+  }
+  L.hx = g.hx;
+  L.hy = g.hy;
+  L.hz = g.hz;
   L.inside = inside_axis(px, g.ox, g.hx, g.nx) && inside_axis(py, g.oy, g.hy, g.ny) &&
              inside_axis(pz, g.oz, g.hz, g.nz);
   L.cx = L.cy = L.cz = 0;
@@ -151,7 +189,7 @@ __global__ void k_locate(GridDev g, const double* __restrict__ pts, int64_t n,
   }
   if (T) {
     double t[24];
-    shape_grad(L.lx, L.ly, L.lz, g.hx, g.hy, g.hz, t);
+    shape_grad(L.lx, L.ly, L.lz, L.hx, L.hy, L.hz, t);
 #pragma unroll
     for (int k = 0; k < 24; ++k) T[24 * i + k] = L.inside ? t[k] : 0.0;
   }
@@ -249,7 +287,7 @@ __global__ void k_build_rows(GridDev g, int kind, double* __restrict__ pts, int6
     return;
   }
   double T[24];
-  shape_grad(L.lx, L.ly, L.lz, g.hx, g.hy, g.hz, T);
+  shape_grad(L.lx, L.ly, L.lz, L.hx, L.hy, L.hz, T);
   if (kind == LSAMD_ROWS_NORMAL) {
     for (int d = 0; d < 3; ++d) {
       double a[8];
@@ -328,7 +366,7 @@ __global__ void k_evaluate(GridDev g, const double* __restrict__ c, const double
     double gx = 0.0, gy = 0.0, gz = 0.0;
     if (L.inside) {
       double T[24];
-      shape_grad(L.lx, L.ly, L.lz, g.hx, g.hy, g.hz, T);
+      shape_grad(L.lx, L.ly, L.lz, L.hx, L.hy, L.hz, T);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         T[k] *= cv[k];

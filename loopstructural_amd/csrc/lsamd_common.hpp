@@ -43,6 +43,10 @@ struct GridDev {
   double ox, oy, oz;
   double hx, hy, hz;
   int nx, ny, nz;  // nodes per axis
+  // node coordinates per axis of a rectilinear grid (all NULL: uniform)
+  const double* xn;
+  const double* yn;
+  const double* zn;
 };
 
 inline GridDev to_dev(const lsamd_grid& g) {
@@ -56,10 +60,15 @@ inline GridDev to_dev(const lsamd_grid& g) {
   d.nx = (int)g.nnodes[0];
   d.ny = (int)g.nnodes[1];
   d.nz = (int)g.nnodes[2];
+  d.xn = g.axis_nodes[0];
+  d.yn = g.axis_nodes[1];
+  d.zn = g.axis_nodes[2];
   return d;
 }
 
 int check_grid(const lsamd_grid* g);
+// check_grid, and the grid must be uniform (P1 meshes, batches)
+int check_uniform_grid(const lsamd_grid* g, const char* fn);
 
 // ------------------------------------------------ deterministic reductions --
 // Fixed-order wave64 butterfly sum.

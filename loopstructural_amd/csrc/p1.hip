@@ -315,7 +315,7 @@ __global__ void k_p1_evaluate(P1Mesh m, const double* __restrict__ c,
 }
 
 int check_mesh(const lsamd_grid* g, const char* fn) {
-  if (int rc = check_grid(g)) return rc;
+  if (int rc = check_uniform_grid(g, fn)) return rc;
   const int64_t ntet = 5 * (g->nnodes[0] - 1) * (g->nnodes[1] - 1) * (g->nnodes[2] - 1);
   if (ntet > ((int64_t)1 << 40)) return fail(LSAMD_ERR_ARG, std::string(fn) + ": mesh too large");
   return LSAMD_OK;

@@ -998,15 +998,19 @@ class DiscreteInterpolator(GeologicalInterpolator):
         self._remove_constraints_with_prefix(prefix)
 
     def _add_device_rows(self, A, b, idc, w, name):
-        """add_constraints_to_least_squares for rows already on the device."""
-        A = A.contiguous()
+        """add_constraints_to_least_squares for rows already on the device;
+        ``w`` one weight or one per row."""
+        A = A.contiguous().clone()
         b = b.contiguous().clone()
         R, k = A.shape
         norm = torch.empty(R, dtype=torch.float64, device=A.device)
         if R:
             _lib.call("lsamd_rows_normalise", R, k, _lib.ptr(A), _lib.ptr(b), _lib.ptr(norm),
                       _lib.stream())
-        wv = torch.full((R,), float(w), dtype=torch.float64, device=A.device)
+        if isinstance(w, torch.Tensor):
+            wv = w.to(A.device, torch.float64).contiguous()
+        else:
+            wv = torch.full((R,), float(w), dtype=torch.float64, device=A.device)
         blk = fs.fixed_width_rows(idc.to(torch.int32).contiguous(), A, b, wv, norm, self.dof)
         return self._register_rows(blk, name)
 
@@ -1389,9 +1393,11 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
 
     # ------------------------------------------------------ regulariser ----
     def assemble_inner(self, operator, w, name="regularisation"):
-        """Record an interior stencil family (:1146-1164); matrix-free."""
+        """Record an interior stencil family (:1146-1164); matrix-free.  A
+        ``None`` operator (rectilinear grids) builds spacing-scaled rows."""
         if operator is None:
-            raise NotImplementedError("rectilinear grids are not part of this package")
+            self._assemble_rectilinear_operator(name, w)
+            return
         kind = _mask_kind(operator)
         if kind is None:
             raise NotImplementedError(
@@ -1402,6 +1408,26 @@ class FiniteDifferenceInterpolator(DiscreteInterpolator):
         fam = StencilFamily(name, kind, float(w), nrows, self.dof, np.asarray(operator, float),
                             tuple(int(v) for v in n))
         self._add_stencil_family(fam)
+
+    def _assemble_rectilinear_operator(self, name, w):
+        """Second-derivative rows scaled by the local node spacing at every
+        node interior along the axes of the family, as data rows on the GPU
+        (LI/_finite_difference_interpolator.py:1166-1205)."""
+        axes = {"dxx": (0, -1), "dyy": (1, -1), "dzz": (2, -1), "dxy": (0, 1), "dxz": (0, 2),
+                "dyz": (1, 2)}
+        if name not in axes:
+            logger.warning(f"Unknown rectilinear operator name '{name}', skipping.")
+            return
+        from .supports import scaled_operator_rows
+
+        nodes = (self.support.xnodes, self.support.ynodes, self.support.znodes)
+        A, cols, centres = scaled_operator_rows(nodes, *axes[name])
+        w_rows = torch.full((A.shape[0],), float(w), dtype=torch.float64, device=A.device)
+        if self.use_regularisation_weight_scale:
+            self.regularisation_scale  # materialise on the device
+            w_rows = self._regularisation_scale[centres] * float(w)
+        self._add_device_rows(A, torch.zeros(A.shape[0], dtype=torch.float64, device=A.device),
+                              cols, w_rows, name)
 
     def assemble_borders(self):
         """Border layers 1 and n-2 with the axial masks, weights dx/dy/dz

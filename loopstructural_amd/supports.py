@@ -33,6 +33,9 @@ class SupportType(IntEnum):
     BaseStructured = 6
     TetMesh = 10
     P2UnstructuredTetMesh = 11
+    P2StructuredTetMesh = 12
+    DataSupported = 13
+    RectilinearGrid = 14
 
 
 class LoopException(Exception):
@@ -448,6 +451,161 @@ class StructuredGrid(BaseStructuredSupport):
         except ImportError:
             raise ImportError("pyvista is required for vtk support")
         raise NotImplementedError("vtk export is outside this package's scope")
+
+
+class RectilinearGrid(StructuredGrid):
+    """Structured grid with its own node coordinates per axis
+    (LC/supports/_3d_rectilinear_grid.py:20-315).
+
+    Point location (binary search per axis, strict inside test, unclipped
+    local coordinates), shape functions and their gradients scaled by each
+    cell's size, and evaluation run in the same HIP kernels as the uniform
+    grid, reading the node arrays from device memory.  The FD regulariser on
+    this support is built as explicit rows scaled by the local spacing
+    (``build_scaled_operator_rows``)."""
+
+    def __init__(self, xnodes, ynodes, znodes):
+        xnodes = np.asarray(xnodes, dtype=float)
+        ynodes = np.asarray(ynodes, dtype=float)
+        znodes = np.asarray(znodes, dtype=float)
+        if xnodes.ndim != 1 or ynodes.ndim != 1 or znodes.ndim != 1:
+            raise ValueError("Node arrays must be 1-D.")
+        self._axis = (xnodes, ynodes, znodes)
+        self._axis_dev = None
+        nsteps = np.array([len(a) - 1 for a in self._axis], dtype=int)
+        origin = np.array([a[0] for a in self._axis])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            extent = np.array([a[-1] - a[0] for a in self._axis])
+            step_vector = np.where(nsteps > 0, extent / nsteps, 1.0)
+        BaseStructuredSupport.__init__(self, origin=origin, nsteps=nsteps,
+                                       step_vector=step_vector)
+        self.type = SupportType.RectilinearGrid
+        self.name = "RectilinearGrid"
+        self.regions = {"everywhere": np.ones(self.n_nodes).astype(bool)}
+
+    @property
+    def xnodes(self):
+        return self._axis[0]
+
+    @property
+    def ynodes(self):
+        return self._axis[1]
+
+    @property
+    def znodes(self):
+        return self._axis[2]
+
+    @property
+    def nodes(self):
+        xx, yy, zz = np.meshgrid(*self._axis, indexing="ij")
+        return np.column_stack([xx.ravel(order="F"), yy.ravel(order="F"), zz.ravel(order="F")])
+
+    @property
+    def maximum(self):
+        return np.array([a[-1] for a in self._axis])
+
+    def cell_centres(self, global_index):
+        ci = self.global_index_to_cell_index(np.asarray(global_index))
+        return np.column_stack([0.5 * (a[ci[:, d]] + a[ci[:, d] + 1])
+                                for d, a in enumerate(self._axis)])
+
+    def node_indexes_to_position(self, node_indexes):
+        ni = np.asarray(node_indexes)
+        shape = ni.shape
+        ni = ni.reshape(-1, 3)
+        xyz = np.column_stack([a[np.clip(ni[:, d], 0, len(a) - 1)]
+                               for d, a in enumerate(self._axis)])
+        return xyz.reshape(shape)
+
+    def position_to_cell_index(self, pos):
+        """(N,3) cell ijk by binary search on each axis, and the strict inside
+        flags (:115-130); cells are returned for outside points too."""
+        pts = _as_points(pos)
+        inside = self.inside(pts)
+        ci = np.column_stack([np.clip(np.searchsorted(a, pts[:, d], side="right") - 1, 0,
+                                      len(a) - 2) for d, a in enumerate(self._axis)])
+        return ci, inside
+
+    def position_to_local_coordinates(self, pos):
+        """(p - x0) / h in the cell, not clipped (:145-159)."""
+        pts = _as_points(pos)
+        ci, _ = self.position_to_cell_index(pts)
+        out = np.zeros(pts.shape)
+        for d, a in enumerate(self._axis):
+            x0 = a[ci[:, d]]
+            h = a[np.minimum(ci[:, d] + 1, len(a) - 1)] - x0
+            out[:, d] = (pts[:, d] - x0) / np.where(h == 0, 1.0, h)
+        return out
+
+    def _lsgrid(self) -> _lib.Grid:
+        g = _lib.make_grid(self.origin, self.step_vector, self.nsteps)
+        if self._axis_dev is None:
+            self._axis_dev = [torch.as_tensor(np.ascontiguousarray(a), device=_lib.device())
+                              for a in self._axis]
+        for d in range(3):
+            g.axis_nodes[d] = self._axis_dev[d].data_ptr()
+        return g
+
+    def get_operators(self, weights):
+        """No fixed masks: the interpolator builds spacing-scaled rows
+        (:217-230)."""
+        return {"dxx": (None, weights["dxx"]), "dyy": (None, weights["dyy"]),
+                "dzz": (None, weights["dzz"]), "dxy": (None, weights["dxy"] / 4),
+                "dyz": (None, weights["dyz"] / 4), "dxz": (None, weights["dxz"] / 4)}
+
+    def build_scaled_operator_rows(self, axis, cross_axis=-1):
+        """(coefficients (n,3|4), node ids (n,3|4), centre ids (n,)) of the
+        second derivative along ``axis`` (or the mixed one with
+        ``cross_axis``) at every node interior along the axes involved
+        (:232-315)."""
+        A, cols, rows = scaled_operator_rows(self._axis, axis, cross_axis)
+        return A.cpu().numpy(), cols.cpu().numpy(), rows.cpu().numpy()
+
+    def to_dict(self):
+        return {"type": int(self.type), "xnodes": self._axis[0], "ynodes": self._axis[1],
+                "znodes": self._axis[2]}
+
+
+def scaled_operator_rows(axis_nodes, axis, cross_axis=-1):
+    """Spacing-scaled second-derivative rows on a rectilinear grid, built on
+    the device: the three-point second derivative
+    2[f_(i-1)/(hL (hL+hR)) - f_i/(hL hR) + f_(i+1)/(hR (hL+hR))] at nodes
+    interior along ``axis``, or the four-point mixed derivative
+    [f(+,+) - f(+,-) - f(-,+) + f(-,-)] / ((h_a) (h_c)) (h over two cells) at
+    nodes interior along both axes; centres in ascending node order
+    (LC/supports/_3d_rectilinear_grid.py:232-315)."""
+    dev = _lib.device()
+    ax = [torch.as_tensor(np.ascontiguousarray(a, dtype=float), device=dev) for a in axis_nodes]
+    n = [int(a.shape[0]) for a in ax]
+    ijk = torch.stack(torch.meshgrid(*[torch.arange(v, device=dev) for v in n], indexing="ij"),
+                      dim=-1).permute(2, 1, 0, 3).reshape(-1, 3)
+    pure = cross_axis < 0 or cross_axis == axis
+    involved = (axis,) if pure else (axis, cross_axis)
+    keep = torch.ones(ijk.shape[0], dtype=torch.bool, device=dev)
+    for a in involved:
+        keep &= (ijk[:, a] > 0) & (ijk[:, a] < n[a] - 1)
+    c = ijk[keep]
+    stride = (1, n[0], n[0] * n[1])
+    rows = c[:, 0] + stride[1] * c[:, 1] + stride[2] * c[:, 2]
+    if pure:
+        i = c[:, axis]
+        hL = ax[axis][i] - ax[axis][i - 1]
+        hR = ax[axis][i + 1] - ax[axis][i]
+        cm = 2.0 / (hL * (hL + hR))
+        cp = 2.0 / (hR * (hL + hR))
+        A = torch.stack([cm, -(cm + cp), cp], dim=1)
+        cols = torch.stack([rows - stride[axis], rows, rows + stride[axis]], dim=1)
+    else:
+        ia, ic = c[:, axis], c[:, cross_axis]
+        hx = ax[axis][ia + 1] - ax[axis][ia - 1]
+        hy = ax[cross_axis][ic + 1] - ax[cross_axis][ic - 1]
+        den = hx * hy
+        signs = torch.tensor([1.0, -1.0, -1.0, 1.0], dtype=torch.float64, device=dev)
+        A = signs[None, :] / den[:, None]
+        sa, sc = stride[axis], stride[cross_axis]
+        cols = torch.stack([rows + sa + sc, rows + sa - sc, rows - sa + sc, rows - sa - sc],
+                           dim=1)
+    return A.contiguous(), cols.contiguous(), rows
 
 
 def _dev_points(pos) -> torch.Tensor:

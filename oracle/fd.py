@@ -113,7 +113,15 @@ def trilinear(lc):
 
 def element_gradient(grid: Grid, pos):
     """_3d_structured_grid.py:393-454: T[:, d, k] = dN_k/dx_d / h_d."""
-    lc = local_coords(grid, pos)
+    T = element_gradient_from_local(local_coords(grid, pos))
+    T[:, 0, :] /= grid.step[None, 0]
+    T[:, 1, :] /= grid.step[None, 1]
+    T[:, 2, :] /= grid.step[None, 2]
+    return T
+
+
+def element_gradient_from_local(lc):
+    """dN_k/d(local)_d of the trilinear shape functions, _3d_structured_grid.py:425-453."""
     x, y, z = lc[:, 0], lc[:, 1], lc[:, 2]
     T = np.zeros((lc.shape[0], 3, 8))
     T[:, 0, 0] = (1 - z) * (y - 1)
@@ -140,9 +148,6 @@ def element_gradient(grid: Grid, pos):
     T[:, 2, 6] = (1 - x) * y
     T[:, 2, 3] = -x * y
     T[:, 2, 7] = x * y
-    T[:, 0, :] /= grid.step[None, 0]
-    T[:, 1, :] /= grid.step[None, 1]
-    T[:, 2, :] /= grid.step[None, 2]
     return T
 
 

@@ -71,11 +71,13 @@ def test_fdi_value_normal_matches_minimiser():
     assert np.array_equal(g, ofd.evaluate_gradient(og, interp.c, pts))
     # evaluation follows c after a solve: a new array, then an in-place change
     interp.c = og.nodes() @ np.array([0.5, -1.0, 2.0])
-    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts))
+    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts),
+                          equal_nan=True)
     interp.c[: og.n_nodes // 2] *= -3.0
-    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts))
+    assert np.array_equal(interp.evaluate_value(pts), ofd.evaluate_value(og, interp.c, pts),
+                          equal_nan=True)
     assert np.array_equal(interp.evaluate_gradient(pts),
-                          ofd.evaluate_gradient(og, interp.c, pts))
+                          ofd.evaluate_gradient(og, interp.c, pts), equal_nan=True)
 
 
 def test_fdi_all_data_families_match_minimiser():
