@@ -224,14 +224,16 @@ def main():
     extras = {}
     if not args.no_extras and c_ref is not None:
         extras["fd_evaluation"] = fd_evaluation(torch, _lib, fs, grid, c_ref, n)
-        extras["ensemble"] = ensemble_c4(args, torch, _lib, fs, synthetic)
         extras["p1"] = p1_c3(torch, _lib, fs, synthetic)
         extras["adjoint"] = adjoint_c5(torch, synthetic)
+        extras["ensemble"] = ensemble_c4(args, torch, _lib, fs, synthetic)
 
     cpu = None
     if not args.no_cpu_baseline and info_ref["iterations"] is not None:
         cpu = cpu_baseline(cells, values, normals, forms["plain"]["iterations"]
                            if "plain" in forms else info_ref["iterations"])
+        if not args.no_extras:
+            extras["c1_phases"] = c1_phases()
 
     out = {
         "metric": METRIC,
@@ -258,6 +260,7 @@ def main():
                              if "batched" in forms and "plain" in forms else None),
         "fp64_reference_iterations": info_ref["iterations"],
         "roofline": roofline,
+        "roofline_batched": forms.get("batched", {}).get("roofline_batched"),
         "roofline_hbm_512": hbm,
         "extras": extras,
         "cpu_baseline": cpu,
@@ -318,6 +321,21 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
            "dtype": "f32+f64"}
     log(f"batched x{B}: {dt * 1e3:.0f} ms/batch = {B / dt:.2f} solves/s, max relerr "
         f"{max(errs):.2e}")
+    if B == 32:
+        # fine-level kernels of the 32-lane group, HIP-event timed on the
+        # launch stream (after the timed region: the group's work is reused)
+        res["roofline_batched"] = {}
+        for kname, kfull in (("cheb", "k_mr_cheb (Chebyshev step, nodes without Schwarz "
+                                      "blocks, 32 lanes)"),
+                             ("update", "k_mr_upd<32> (FP64 x, r update, 32 lanes)"),
+                             ("stencil", "k_b_tiles<float, zc> over 32 lanes (regulariser)")):
+            ms, nbytes = sysb.time_multi(kname, iters=20)
+            ach = nbytes / (ms * 1e-3) / 1e9
+            res["roofline_batched"][kname] = {
+                "kernel": kfull, "bound": "hbm", "avg_us": round(ms * 1e3, 2),
+                "algorithmic_bytes": int(nbytes), "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+            log(f"batched {kname}: {ms * 1e3:.1f} us, {ach:.0f} GB/s")
     sysb.close()
     del sysb
     torch.cuda.empty_cache()
@@ -420,6 +438,29 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
         single.close()
     sb_ms = bs.time_stencil("fp32", iters=10)
     tot = sum(ns)
+    bs.close()
+    del bs
+    torch.cuda.empty_cache()
+    log(f"ensemble (batched Jacobi PCG): {dt:.2f} s")
+    # the same ensemble by per-system multigrid, shape groups on several streams
+    blocks = [fd_rows(fs, _lib, g, v, m, k) for g, (v, m), k in zip(grids, vd, ns)]
+    es = fs.EnsembleSolver(grids, reg, workers=4)
+    t0 = time.perf_counter()
+    es.prepare(blocks, args.rtol)
+    t_prep = time.perf_counter() - t0
+    log(f"ensemble multigrid set-up ({len(es.shapes)} shapes): {t_prep:.2f} s")
+    t0 = time.perf_counter()
+    cm, infm = es.solve(blocks, rtol=args.rtol)
+    dt_mg = time.perf_counter() - t0
+    log(f"ensemble multigrid solves: {dt_mg:.2f} s")
+    errs_mg = []
+    for s_ in np.linspace(0, len(ens) - 1, 8).astype(int):
+        single = fs.FdSystem(grids[s_], fs.reg_struct(W, (ens[s_][0],) * 3, 1e-8))
+        single.set_rows(blocks[s_])
+        r, _ = single.solve(rtol=1e-13, precision="fp64", check_every=256)
+        errs_mg.append(float(torch.linalg.vector_norm(cm[s_] - r) / torch.linalg.vector_norm(r)))
+        single.close()
+    es.close()
     res = {"systems": len(ens), "total_dof": tot, "seconds": round(dt, 3),
            "solves_per_s": round(len(ens) / dt, 2),
            "converged": sum(i["converged"] for i in infos),
@@ -427,10 +468,16 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
            "rel_err_vs_fp64_max_of_8": max(errs),
            "stencil_fp32_us": round(sb_ms * 1e3, 1),
            "stencil_fp32_GBs": round(tot * 8 / (sb_ms * 1e-3) / 1e9, 1),
-           "stencil_frac": round(tot * 8 / (sb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+           "stencil_frac": round(tot * 8 / (sb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "multigrid": {"seconds": round(dt_mg, 3), "solves_per_s": round(len(ens) / dt_mg, 2),
+                         "setup_seconds": round(t_prep, 3), "shapes": len(es.shapes),
+                         "converged": sum(i["converged"] for i in infm),
+                         "iterations_max": max(i["iterations"] for i in infm),
+                         "rel_err_vs_fp64_max_of_8": max(errs_mg),
+                         "thread_seconds": {k: round(v, 3) for k, v in es.last_timing.items()},
+                         "what": "per-system multigrid, one handle per grid shape (hierarchy "
+                                 "built in setup_seconds), shape groups on 4 streams"}}
     log(f"ensemble: {res}")
-    bs.close()
-    del bs
     torch.cuda.empty_cache()
     return res
 
@@ -527,6 +574,74 @@ def adjoint_c5(torch, synthetic):
     return res
 
 
+def host_env():
+    """CPU model, visible cores and the BLAS/OMP thread settings of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    keys = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "BLIS_NUM_THREADS")
+    return {"cpu_model": model, "cores_visible": len(os.sched_getaffinity(0)),
+            "thread_env": {k: os.environ.get(k) for k in keys},
+            "note": "scipy's CSR SpMV is single-threaded: CG runs on one core"}
+
+
+def c1_phases():
+    """The 46^3-node FD workload (200 values, 50 normals of
+    f = z + 0.1 sin 2 pi x cos 2 pi y, regularisation 0.1, cg at rtol 1e-10):
+    a full CPU solve by the oracle's restatement of the reference path, timed
+    phase by phase under the reference's timing keys, beside the GPU facade's
+    own timing dict for the same solve (second call, kernels loaded)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from scipy.sparse import linalg as spla
+
+    from loopstructural_amd import FiniteDifferenceInterpolator, StructuredGrid, synthetic
+    from oracle import fd as ofd
+
+    values, normals = synthetic.constraints(200, 50, 1001)
+    gpu = None
+    for _ in range(2):
+        it = FiniteDifferenceInterpolator(StructuredGrid(origin=(0, 0, 0), nsteps=(45, 45, 45),
+                                                         step_vector=(1 / 45,) * 3))
+        it.set_value_constraints(values)
+        it.set_normal_constraints(normals)
+        it.setup_interpolator(regularisation=0.1)
+        t0 = time.perf_counter()
+        it.solve_system("cg", solver_kwargs={"rtol": 1e-10, "atol": 0})
+        wall = time.perf_counter() - t0
+        gpu = {k: v for k, v in it.get_last_solve_timing().items()
+               if isinstance(v, (int, float, bool))}
+        gpu["wall_seconds"] = wall
+    og = ofd.Grid((0, 0, 0), (45, 45, 45), (1 / 45,) * 3)
+    t0 = time.perf_counter()
+    A, b = ofd.assemble_fd(og, {"value": values, "normal": normals},
+                           regularisation=0.1).build_matrix()
+    t1 = time.perf_counter()
+    A, b, S = ofd.preprocess(A, b)
+    t2 = time.perf_counter()
+    count = [0]
+
+    def cb(_):
+        count[0] += 1
+
+    y, info = spla.cg(A.T @ A, A.T @ b, rtol=1e-10, atol=0.0, callback=cb)
+    c = S @ y
+    t3 = time.perf_counter()
+    cpu = {"assembly_seconds": t1 - t0, "preprocess_seconds": t2 - t1,
+           "solve_seconds": t3 - t2, "total_seconds": t3 - t0, "cg_iterations": count[0],
+           "cg_info": int(info), "matrix_rows": int(A.shape[0]), "matrix_nnz": int(A.nnz)}
+    rel = float(np.linalg.norm(it.c - c) / np.linalg.norm(c))
+    log(f"C1 phases: cpu total {cpu['total_seconds']:.1f}s ({count[0]} it), gpu wall "
+        f"{gpu['wall_seconds']:.3f}s, rel diff {rel:.1e}")
+    return {"workload": "46^3 nodes, 200 values, 50 normals, cg rtol 1e-10", "cpu": cpu,
+            "gpu": gpu, "rel_l2_gpu_vs_cpu": rel, "host": host_env(), "cpu_kind": "port"}
+
+
 def cpu_baseline(cells, values, normals, gpu_iterations):
     """The reference's CPU path (oracle restatement: explicit scipy assembly,
     ridge + column scaling, scipy cg on A.T@A) on a bounded sample: full
@@ -557,7 +672,7 @@ def cpu_baseline(cells, values, normals, gpu_iterations):
             "sample": f"{cells + 1}^3 nodes: full scipy assembly+preprocess+A^T A "
                       f"({t_setup:.1f}s) + {k} CG iterations ({t_it * 1e3:.1f} ms/it), "
                       f"extrapolated to {gpu_iterations} iterations (GPU FP64 count)",
-            "host_cpus_visible": len(os.sched_getaffinity(0))}
+            "host_cpus_visible": len(os.sched_getaffinity(0)), "host": host_env()}
 
 
 if __name__ == "__main__":

@@ -228,10 +228,25 @@ int lsamd_fd_system_lsmr(lsamd_fd_system* s, const lsamd_lsmr_opts* opts, double
 int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
                                  double* avg_ms, void* stream);
 
-/* Autotuner result: z-chunk of the plane-streaming stencil chosen for the
- * precision (0 = exact per-node kernel: an axis has < 7 nodes or a per-node
- * weight scale is set; also 0 before the first apply/solve), and the timed
- * candidates 8/16/32 in microseconds (tune_us[3], nullable, -1 = skipped). */
+/* Times one kernel of the 32-lane multi-right-hand-side multigrid on the fine
+ * level, reusing the work of this handle's last 32-lane
+ * lsamd_fd_system_solve_multi group (LSAMD_ERR_ARG without one) and leaving
+ * its contents undefined: kernel 0 = the Chebyshev smoothing step of the
+ * nodes without Schwarz blocks (k_mr_cheb), 1 = the FP64 update of x and r
+ * (k_mr_upd), 2 = the 32-lane regulariser stencil.  One warm launch, then
+ * `iters` launches between two HIP events on `stream`: *avg_ms per launch,
+ * *bytes = algorithmic bytes per launch (every input and output element
+ * once).  A measurement hook with no reference counterpart. */
+int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters,
+                               double* avg_ms, double* bytes, void* stream);
+
+/* Autotuner result: form of the plane-streaming stencil chosen for the
+ * precision: the z-chunk 8, 16 or 32 with planes staged through registers, or
+ * 1008 / 1016 / 1032 for z-chunk 8 / 16 / 32 with planes copied straight into
+ * LDS (0 = exact per-node kernel: an axis has < 7 nodes or a per-node weight
+ * scale is set; also 0 before the first apply/solve), and the timed forms 8,
+ * 16, 32, 1008, 1016, 1032 in microseconds (tune_us[6], nullable, -1 =
+ * skipped). */
 int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,
                                  float* tune_us);
 

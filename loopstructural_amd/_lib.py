@@ -92,6 +92,7 @@ SIGNATURES = {
     "lsamd_fd_nearest_d2": [_P, _P, _I64, _P, _P],
     "lsamd_fd_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
+    "lsamd_fd_system_time_multi": [_P, _I32, _I32, _P, _P, _P],
     "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],
     "lsamd_p1_locate": [_P, _P, _I64, _P, _P, _P, _P, _P, _P],
     "lsamd_p1_elements": [_P, _P, _P, _P],

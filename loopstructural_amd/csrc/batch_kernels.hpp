@@ -63,8 +63,10 @@ __constant__ signed char kBTap[kTaps][3] = {
 
 // Plane-streaming stencil over a table of work items (one launch for a
 // whole system or a whole ensemble); per-block partials of x.y.  The
-// single-system plan is the same table with one system.
-template <typename T, int ZC>
+// single-system plan is the same table with one system.  DMA: bulk planes
+// staged by direct global -> LDS copies (st_bulk_dma) instead of through
+// registers (st_bulk_body); same outputs bit for bit.
+template <typename T, int ZC, bool DMA = false>
 __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __restrict__ y,
                                                 const BSys* __restrict__ sys,
                                                 const int4* __restrict__ tiles,
@@ -76,14 +78,23 @@ __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __re
   if (ctl[t.x].done) return;
   const BSys S = sys[t.x];
   double dot;
-  if (t.w >= kUniTile)
-    dot = st_bulk_body<T, ZC, 4, true>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
-                                       t.w - kUniTile);
-  else if (t.w >= 0)
-    dot = st_bulk_body<T, ZC, 4, false>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
-                                        t.w);
+  if (t.w >= kUniTile) {
+    if constexpr (DMA)
+      dot = st_bulk_dma<T, ZC, true>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
+                                     t.w - kUniTile);
+    else
+      dot = st_bulk_body<T, ZC, 4, true>(x + S.off, y + S.off, S.g, tab, ku,
+                                         make_int2(t.y, t.z), t.w - kUniTile);
+  } else if (t.w >= 0) {
+    if constexpr (DMA)
+      dot = st_bulk_dma<T, ZC, false>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
+                                      t.w);
+    else
+      dot = st_bulk_body<T, ZC, 4, false>(x + S.off, y + S.off, S.g, tab, ku,
+                                          make_int2(t.y, t.z), t.w);
+  }
   else
-    dot = st_zb_body<T>(x + S.off, y + S.off, S.g, tab, t.y, t.z, t.w == kZbLow ? 0 : 1);
+    dot = st_zb_body<T, DMA>(x + S.off, y + S.off, S.g, tab, t.y, t.z, t.w == kZbLow ? 0 : 1);
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }

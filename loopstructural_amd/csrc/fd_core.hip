@@ -16,6 +16,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fd_operator.hpp"
@@ -205,13 +206,197 @@ __device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __res
   return dot;
 }
 
+// ---- planes staged by direct global -> LDS loads (LDS DMA) ----
+//
+// The same work item and arithmetic as st_bulk_body, with the (64+4) x (8+4)
+// plane tiles copied by global_load_lds_dword straight into a ring of DD LDS
+// buffers: no VGPRs hold planes in flight, so DD - 1 planes per workgroup are
+// in flight at once.  Each wave copies SL slots of 64 dwords per plane (a
+// double takes two dwords), slots past the tile re-read its first dword.
+// Out-of-grid halo positions read the clamped in-grid node instead of zero:
+// the class coefficients of every tap leaving the grid are exactly zero, so
+// each output is bit-identical to st_bulk_body's.  Outputs are written by
+// buffer stores that every wave issues once per output plane (lanes without
+// an output get an offset past the buffer, which the hardware drops), so
+// the count of memory operations issued after each plane copy is known at
+// compile time and vmcnt(N) proves that the plane has landed (vector memory
+// operations complete in issue order).  The ring buffers are distinct LDS
+// arrays so the compiler's own wait insertion can tell the copy into one
+// buffer from reads of another.
+#define LSAMD_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
+
+template <typename T>
+constexpr int kDmaWords = PSZ * (int)sizeof(T) / 4;  // dwords of one plane tile
+template <typename T>
+constexpr int kDmaSlots = (kDmaWords<T> + NT - 1) / NT;  // 64-dword slots per wave
+template <typename T>
+constexpr int kDmaBuf = kDmaSlots<T> * NT;  // dwords of one ring buffer
+template <typename T>
+constexpr int kDmaDepth = sizeof(T) == 4 ? 7 : 6;  // ring buffers (LDS: 28 KB / 48 KB)
+
+// buffer resource of one plane (bytes in num_records; raw, no stride)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, 0x00020000);
+}
+
+// non-temporal buffer store of one element (dropped past num_records)
+template <typename T>
+__device__ __forceinline__ void buffer_store_nt(T v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 2);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__attribute__((__vector_size__(8))) unsigned, v), r, (int)off, 0, 2);
+}
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (expcnt untouched)
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// ring buffer I of element type T: one LDS array per (T, I) in a kernel,
+// shared by every body of the kernel that uses it
+template <typename T, int I>
+__device__ __forceinline__ unsigned* dma_ring() {
+  __shared__ __align__(16) unsigned buf[kDmaBuf<T>];
+  return buf;
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+template <typename T, int ZC, bool UNI>
+__device__ __forceinline__ double st_bulk_dma(const T* __restrict__ x, T* __restrict__ y,
+                                              const GridDev& g, const T* __restrict__ tab,
+                                              const Coef25<T>& ku, int2 tb, int bz) {
+  constexpr int NS = ZC + 4;
+  constexpr int DD = kDmaDepth<T>;
+  constexpr int SL = kDmaSlots<T>;
+  constexpr int WPE = (int)sizeof(T) / 4;  // dwords per element
+  static_assert(DD >= 3 && DD <= 8, "ring depth");
+  auto ring = [](auto I) -> unsigned* { return dma_ring<T, decltype(I)::value % DD>(); };
+  const int x0 = tb.x * TX, y0 = tb.y * TY, z0 = 3 + bz * ZC;
+  const int zend = min(z0 + ZC, g.nz - 3);
+  const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
+  const int ix = x0 + lx, iy = y0 + ly;
+  const bool colok = ix < g.nx && iy < g.ny;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  T k[kTaps];
+  if constexpr (UNI) {
+#pragma unroll
+    for (int q = 0; q < kTaps; ++q) k[q] = ku.k[q];
+  } else {
+    const int cx = cls7(min(ix, g.nx - 1), g.nx), cy = cls7(min(iy, g.ny - 1), g.ny);
+    const T* tc = tab + (int64_t)((3 * 7 + cy) * 7 + cx) * kTaps;
+#pragma unroll
+    for (int q = 0; q < kTaps; ++q) k[q] = tc[q];
+  }
+  // copy slots of this lane: byte offset inside a plane of the tile (a plane
+  // is at most 4 GiB), added to the plane's wave-uniform base address
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ln = threadIdx.x & 63;
+  unsigned goff[SL];
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int e = (j * (NT / 64) + wv) * 64 + ln;
+    const int ee = e < kDmaWords<T> ? e : 0;
+    const int el = ee / WPE, half = ee - el * WPE;
+    const int ey = el / PX, ex = el - ey * PX;
+    const int gx = min(max(x0 - 2 + ex, 0), g.nx - 1), gy = min(max(y0 - 2 + ey, 0), g.ny - 1);
+    goff[j] = (unsigned)((((int64_t)gx + (int64_t)gy * g.nx) * WPE + half) * 4);
+  }
+  const char* xb = (const char*)x;
+  const int pbeg = z0 - 2;  // >= 1
+  auto copy_plane = [&](auto I) {
+    constexpr int q = decltype(I)::value;
+    unsigned* buf = ring(I);
+    const char* plane = xb + (int64_t)min(pbeg + q, g.nz - 1) * nxy * (int64_t)sizeof(T);
+#pragma unroll
+    for (int j = 0; j < SL; ++j)
+      __builtin_amdgcn_global_load_lds((const void*)(plane + goff[j]),
+                                       LSAMD_LDS_PTR(buf + (j * (NT / 64) + wv) * 64), 4, 0, 0);
+  };
+  // the coefficient loads complete before the copies start
+  wait_vm_lgkm<0>();
+  asm volatile("" ::: "memory");
+  static_for<0, (DD - 1 < NS ? DD - 1 : NS)>(copy_plane);
+  T r4[5] = {0, 0, 0, 0, 0}, r3[5] = {0, 0, 0, 0, 0}, r2[5] = {0, 0, 0, 0, 0},
+    r1[5] = {0, 0, 0, 0, 0};
+  T v2 = 0, v1 = 0;
+  double dot = 0.0;
+  const unsigned pbytes = (unsigned)(nxy * (int64_t)sizeof(T));
+  // lanes without an output store at kDropOffset: past any plane (planes are
+  // below 2 GiB) and far from wrapping, so the hardware drops the store
+  constexpr unsigned kDropOffset = 0x80000000u;
+  const unsigned yoff = colok ? (unsigned)(((int64_t)ix + (int64_t)iy * g.nx) * sizeof(T))
+                              : kDropOffset;
+  static_for<0, NS>([&](auto I) {
+    constexpr int s = decltype(I)::value;
+    // Plane s has landed once at most the operations issued after its copy
+    // are outstanding: the copies of the later planes issued so far and the
+    // output stores of the iterations since (one per iteration from s = 4).
+    // The barrier also ends every wave's reads of plane s - 1, whose buffer
+    // the next copy overwrites.
+    constexpr int later = (DD - 2 < NS - 1 - s ? DD - 2 : NS - 1 - s);
+    constexpr int t0 = s - DD + 1 > 0 ? s - DD + 1 : 0;  // iteration that copied plane s
+    constexpr int st0 = t0 > 4 ? t0 : 4;
+    constexpr int stores = s > st0 ? s - st0 : 0;
+    wait_vm_lgkm<SL * later + stores>();
+    lds_barrier();
+    if constexpr (s + DD - 1 < NS) copy_plane(std::integral_constant<int, s + DD - 1>{});
+    const T* b = (const T*)ring(I) + (ly + 2) * PX + lx + 2;
+    const T c = b[0], xp2 = b[2], xm2 = b[-2], yp2 = b[2 * PX], ym2 = b[-2 * PX];
+    const T v0 = inplane(k, b);
+    if constexpr (s >= 4) {
+      const int z = pbeg + s - 2;
+      const T yv = v2 + k[13] * r3[0] + k[14] * r1[0] + k[15] * r4[0] + k[16] * r4[1] +
+                   k[17] * r4[2] + k[18] * r4[3] + k[19] * r4[4] + k[20] * c + k[21] * xp2 +
+                   k[22] * xm2 + k[23] * yp2 + k[24] * ym2;
+      // branch-free, so that every wave issues exactly one store here
+      const bool out = colok && z < zend;
+      buffer_store_nt<T>(yv, plane_rsrc(y + (int64_t)z * nxy, pbytes), out ? yoff : kDropOffset);
+      dot += (double)(out ? r2[0] : (T)0) * (double)yv;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      r4[q] = r3[q];
+      r3[q] = r2[q];
+      r2[q] = r1[q];
+    }
+    r1[0] = c;
+    r1[1] = xp2;
+    r1[2] = xm2;
+    r1[3] = yp2;
+    r1[4] = ym2;
+    v2 = v1;
+    v1 = v0;
+  });
+  return dot;
+}
+
 // Output planes [0,3) (bz = 0) or [nz-3, nz) (bz = 1) of xy tile (tx, ty);
 // coefficients re-read per plane.  Returns this thread's share of x.y.
-template <typename T>
+template <typename T, bool RING = false>
 __device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restrict__ y,
                                              const GridDev& g, const T* __restrict__ tab, int tx,
                                              int ty, int bz) {
-  __shared__ T pl[2][PSZ];
+  // RING: the two plane buffers are the first two LDS-DMA ring buffers of
+  // the kernel (a block runs one body, so they are free here)
+  __shared__ T own[RING ? 1 : 2][RING ? 1 : PSZ];
+  T* pl[2] = {RING ? (T*)dma_ring<T, 0>() : own[0], RING ? (T*)dma_ring<T, 1>() : own[RING ? 0 : 1]};
   const int x0 = tx * TX, y0 = ty * TY, z0 = bz == 0 ? 0 : g.nz - 3;
   const int lx = threadIdx.x & (TX - 1), ly = threadIdx.x / TX;
   const int ix = x0 + lx, iy = y0 + ly;
@@ -344,31 +529,40 @@ __global__ __launch_bounds__(256) void k_p1_stencil(const T* __restrict__ x, T* 
 }
 
 // P1 edge-jump regulariser, plane-streaming.  A block covers a TXV x (512 /
-// TXV) xy tile and kP1Zc output planes; the planes from two below to two
-// above the chunk stream through a double-buffered LDS tile with a 2-node xy
-// halo (D planes of global loads in flight in registers, out-of-grid values
+// TXV) xy tile and ZC output planes; the planes from two below to two above
+// the chunk stream through a double-buffered LDS tile with a 2-node xy halo
+// (D planes of global loads in flight in registers, out-of-grid values
 // zeroed).  Each plane, as it arrives, adds to the five output planes of the
 // column that it touches: its in-plane taps (+-1, +-2 along x and y, and the
 // centre) to the output in the same plane, its centre and four diagonal
 // nodes to the outputs one plane above and below, its centre to those two
 // planes away.  The output two planes below is then complete and stored.
-// Coefficients come from the operator table row of each output node; the
-// table is small and stays cache resident.  One x.y partial per block.
-constexpr int kP1Zc = 16;
+//
+// Coefficients.  A node's table row is fixed by its virtual (x, y) index and
+// its virtual z index, which is 2 + (z & 1) for every z at least two nodes
+// from a z face.  A z chunk whose outputs all lie there (ZIN) takes its rows
+// from the table slab of virtual z 2 and 3, staged in LDS for every virtual
+// (x, y): each thread holds the two rows of its column (even and odd z), in
+// registers for FP32 and read from LDS for FP64, and the plane loop runs in
+// even/odd pairs.  The first and last chunks read rows from the table in
+// global memory (cache resident).  One x.y partial per block.
+template <typename T>
+__device__ __forceinline__ const T* p1_grow(const T* __restrict__ tab, int vx, int vy, int vz,
+                                            int nvx, int nvy) {
+  return tab + (((int64_t)vz * nvy + vy) * nvx + vx) * kP1Taps;
+}
 
-template <typename T, int TXV>
-__global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __restrict__ y,
-                                                  GridDev g, const T* __restrict__ tab, int nvx,
-                                                  int nvy, T ridge2, int ntx, int nzc,
-                                                  double* __restrict__ part, const CgCtl* ctl) {
-  if (ctl->done) return;
-  constexpr int TYV = 512 / TXV, QX = TXV + 4, QY = TYV + 4, QS = QX * QY, D = 2;
-  __shared__ T pl[2][QS];
-  __shared__ double shm[8];
-  const int bz = (int)(blockIdx.x % (unsigned)nzc);
-  const int tt = (int)(blockIdx.x / (unsigned)nzc);
-  const int x0 = (tt % ntx) * TXV, y0 = (tt / ntx) * TYV;
-  const int z0 = bz * kP1Zc, z1 = min(z0 + kP1Zc, g.nz);
+constexpr int kP1SlabMax = 2 * 7 * 7 * kP1Taps;  // rows of virtual z 2, 3
+
+template <typename T, int TXV, int ZC, bool ZIN>
+__device__ __forceinline__ double p1_body(const T* __restrict__ x, T* __restrict__ y,
+                                          const GridDev& g, const T* __restrict__ tab,
+                                          const T* slab, T* plbuf, int nvx, int nvy, T ridge2,
+                                          int x0, int y0, int z0, int z1) {
+  constexpr int TYV = 512 / TXV, QX = TXV + 4, QY = TYV + 4, QS = QX * QY;
+  constexpr int D = ZIN ? 4 : 2;
+  constexpr bool CREG = ZIN && sizeof(T) == 4;  // FP32 rows in registers
+  T* const pl[2] = {plbuf, plbuf + QS};
   const int lx = threadIdx.x % TXV, ly = threadIdx.x / TXV;
   const int i = x0 + lx, j = y0 + ly;
   const bool col = i < g.nx && j < g.ny;
@@ -391,11 +585,20 @@ __global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __
     ok1 = has1 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
     go1 = ok1 ? (int64_t)gx + (int64_t)gy * g.nx : 0;
   }
-  // table row of node (i, j, z): rowxy + virtual(z) * zstride
   const int vx = p1_virtual(min(i, g.nx - 1), g.nx), vy = p1_virtual(min(j, g.ny - 1), g.ny);
-  const T* rowxy = tab + ((int64_t)vy * nvx + vx) * kP1Taps;
-  const int64_t zstride = (int64_t)nvy * nvx * kP1Taps;
-  const int pb = z0 - 2, NS = z1 - z0 + 4;  // planes pb .. pb + NS - 1
+  // ZIN: the rows of this column for even and odd z
+  const T* rowE = slab + ((int64_t)(0 * nvy + vy) * nvx + vx) * kP1Taps;
+  const T* rowO = slab + ((int64_t)(1 * nvy + vy) * nvx + vx) * kP1Taps;
+  T ce[CREG ? kP1Taps : 1], co[CREG ? kP1Taps : 1];
+  if constexpr (CREG) {
+#pragma unroll
+    for (int q = 0; q < kP1Taps; ++q) {
+      ce[q] = rowE[q];
+      co[q] = rowO[q];
+    }
+  }
+  // planes pb .. pb + NS - 1 (a ZIN chunk is a full chunk)
+  const int pb = z0 - 2, NS = ZIN ? ZC + 4 : z1 - z0 + 4;
   T qa[D], qb[D];
   auto ld = [&](int p, T& va, T& vb) {
     const bool pin = p >= 0 && p < g.nz;
@@ -405,38 +608,47 @@ __global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __
     vb = (pin && ok1) ? b : (T)0;
   };
 #pragma unroll
-  for (int d = 0; d < D; ++d) ld(pb + d, qa[d], qb[d]);
+  for (int d = 0; d < D; ++d)
+    if (d < NS) ld(pb + d, qa[d], qb[d]);
   // pending outputs z = p-2 .. p+2 and the centre values of planes p-2, p-1
   T a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, h2 = 0, h1 = 0;
   double dot = 0.0;
   pl[0][threadIdx.x] = qa[0];
   if (has1) pl[0][e1] = qb[0];
   __syncthreads();
-  for (int s = 0; s < NS; ++s) {
+  // one plane; ODD: parity of p (ZIN only)
+  auto plane = [&](int s, auto odd) {
+    constexpr bool ODD = decltype(odd)::value;
     const int p = pb + s;
-    if (s + D < NS) ld(p + D, qa[s % D], qb[s % D]);
+    if (s + D < NS) ld(p + D, qa[(s + D) % D], qb[(s + D) % D]);
     const T* bb = pl[s & 1] + (ly + 2) * QX + lx + 2;
     const T c = bb[0];
-    if (col && p >= 0 && p < g.nz) {
-      if (p >= z0 && p < z1) {
-        const T* C = rowxy + (int64_t)p1_virtual(p, g.nz) * zstride;
-        a2 += C[0] * c + C[1] * bb[1] + C[2] * bb[-1] + C[3] * bb[QX] + C[4] * bb[-QX] +
-              C[7] * bb[2] + C[8] * bb[-2] + C[9] * bb[2 * QX] + C[10] * bb[-2 * QX] + ridge2 * c;
-      }
-      if (p - 1 >= z0 && p - 1 < z1) {  // this plane is dz = +1 of output p-1
-        const T* C = rowxy + (int64_t)p1_virtual(p - 1, g.nz) * zstride;
-        a1 += C[5] * c + C[17] * bb[-1 - QX] + C[18] * bb[1 - QX] + C[19] * bb[-1 + QX] +
-              C[20] * bb[1 + QX];
-      }
-      if (p + 1 >= z0 && p + 1 < z1) {  // dz = -1 of output p+1
-        const T* C = rowxy + (int64_t)p1_virtual(p + 1, g.nz) * zstride;
-        a3 += C[6] * c + C[13] * bb[-1 - QX] + C[14] * bb[1 - QX] + C[15] * bb[-1 + QX] +
-              C[16] * bb[1 + QX];
-      }
-      if (p - 2 >= z0 && p - 2 < z1)
-        a0 += rowxy[(int64_t)p1_virtual(p - 2, g.nz) * zstride + 11] * c;
-      if (p + 2 >= z0 && p + 2 < z1)
-        a4 += rowxy[(int64_t)p1_virtual(p + 2, g.nz) * zstride + 12] * c;
+    const bool pin = ZIN || (p >= 0 && p < g.nz);
+    if (col && pin) {
+      // Rs: row of an output in this plane (or two away), Rn: one plane away
+      auto Rs = [&](int q, int z) -> T {
+        if constexpr (CREG) return ODD ? co[q] : ce[q];
+        else if constexpr (ZIN) return (ODD ? rowO : rowE)[q];
+        else return p1_grow(tab, vx, vy, p1_virtual(z, g.nz), nvx, nvy)[q];
+      };
+      auto Rn = [&](int q, int z) -> T {
+        if constexpr (CREG) return ODD ? ce[q] : co[q];
+        else if constexpr (ZIN) return (ODD ? rowE : rowO)[q];
+        else return p1_grow(tab, vx, vy, p1_virtual(z, g.nz), nvx, nvy)[q];
+      };
+      if (p >= z0 && p < z1)
+        a2 += Rs(0, p) * c + Rs(1, p) * bb[1] + Rs(2, p) * bb[-1] + Rs(3, p) * bb[QX] +
+              Rs(4, p) * bb[-QX] + Rs(7, p) * bb[2] + Rs(8, p) * bb[-2] + Rs(9, p) * bb[2 * QX] +
+              Rs(10, p) * bb[-2 * QX] + ridge2 * c;
+      const T dmm = bb[-1 - QX], dpm = bb[1 - QX], dmp = bb[-1 + QX], dpp = bb[1 + QX];
+      if (p - 1 >= z0 && p - 1 < z1)  // this plane is dz = +1 of output p-1
+        a1 += Rn(5, p - 1) * c + Rn(17, p - 1) * dmm + Rn(18, p - 1) * dpm +
+              Rn(19, p - 1) * dmp + Rn(20, p - 1) * dpp;
+      if (p + 1 >= z0 && p + 1 < z1)  // dz = -1 of output p+1
+        a3 += Rn(6, p + 1) * c + Rn(13, p + 1) * dmm + Rn(14, p + 1) * dpm +
+              Rn(15, p + 1) * dmp + Rn(16, p + 1) * dpp;
+      if (p - 2 >= z0 && p - 2 < z1) a0 += Rs(11, p - 2) * c;
+      if (p + 2 >= z0 && p + 2 < z1) a4 += Rs(12, p + 2) * c;
     }
     const int zo = p - 2;  // complete now
     if (col && zo >= z0 && zo < z1) {
@@ -449,12 +661,55 @@ __global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __
     a3 = a4;
     a4 = 0;
     h2 = h1;
-    h1 = (p >= 0 && p < g.nz) ? c : (T)0;
+    h1 = pin ? c : (T)0;
     if (s + 1 < NS) {
       pl[(s + 1) & 1][threadIdx.x] = qa[(s + 1) % D];
       if (has1) pl[(s + 1) & 1][e1] = qb[(s + 1) % D];
     }
     __syncthreads();
+  };
+  if constexpr (ZIN) {
+    // z0 even (ZC even) so plane s has the parity of s; NS = ZC + 4 is even
+#pragma unroll
+    for (int s = 0; s < NS; s += 2) {
+      plane(s, std::false_type{});
+      plane(s + 1, std::true_type{});
+    }
+  } else {
+    for (int s = 0; s < NS; ++s) plane(s, std::false_type{});
+  }
+  return dot;
+}
+
+template <typename T, int TXV, int ZC>
+__global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __restrict__ y,
+                                                  GridDev g, const T* __restrict__ tab, int nvx,
+                                                  int nvy, int nvz, T ridge2, int ntx, int nzc,
+                                                  double* __restrict__ part, const CgCtl* ctl) {
+  static_assert(ZC % 2 == 0, "even z chunks keep the plane parity");
+  if (ctl->done) return;
+  constexpr int TYV = 512 / TXV;
+  __shared__ T slab[kP1SlabMax];
+  __shared__ T plbuf[2 * (TXV + 4) * (TYV + 4)];
+  __shared__ double shm[8];
+  const int bz = (int)(blockIdx.x % (unsigned)nzc);
+  const int tt = (int)(blockIdx.x / (unsigned)nzc);
+  const int x0 = (tt % ntx) * TXV, y0 = (tt / ntx) * TYV;
+  const int z0 = bz * ZC, z1 = min(z0 + ZC, g.nz);
+  // outputs at least two nodes from both z faces, on a z axis long enough
+  // for the parity rule (virtual z 2 and 3 exist)
+  const bool zin = nvz > 3 && g.nz > 7 && z0 >= 2 && z1 <= g.nz - 2;
+  double dot;
+  if (zin) {
+    const int ns = 2 * nvy * nvx * kP1Taps;
+    const T* src = tab + (int64_t)2 * nvy * nvx * kP1Taps;
+    for (int e = threadIdx.x; e < ns; e += 512) slab[e] = src[e];
+    __syncthreads();
+    dot = p1_body<T, TXV, ZC, true>(x, y, g, tab, slab, plbuf, nvx, nvy, ridge2, x0, y0, z0,
+                                    z1);
+  } else {
+    dot = p1_body<T, TXV, ZC, false>(x, y, g, tab, slab, plbuf, nvx, nvy, ridge2, x0, y0, z0,
+                                     z1);
   }
   const double bsum = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bsum;
@@ -874,6 +1129,7 @@ struct lsamd_fd_system {
   lsamd::Ls* ls = nullptr;  // LSMR work (built on the first LSMR solve after set_rows)
   // multi-rhs multigrid work per group width 4/8/16/32 (kept across set_rows)
   lsamd::MrWork* mr[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* tzero = nullptr;  // zeroed control blocks of the timing entry points
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -891,7 +1147,6 @@ struct lsamd_fd_system {
   Coef25<float> ku32;
   int op = 0;                  // regulariser: 0 FD second derivatives, 1 P1 edge jumps
   int p1nv[3] = {0, 0, 0};     // P1 virtual axis lengths of the table
-  int p1tx = 0;                // P1 stencil tile width (64 or 32; 0: one node per thread)
   double* p1tab64 = nullptr;
   float* p1tab32 = nullptr;
   double p1_ridge2 = 0.0;
@@ -915,7 +1170,7 @@ struct lsamd_fd_system {
   hipGraphExec_t bgraph[2] = {nullptr, nullptr};
   int bgraph_chunk[2] = {0, 0};
   int zc[2] = {0, 0};          // tuned z-chunk per precision (0: not tuned yet)
-  float tune_us[2][3] = {{0}};  // timings of the candidates 8, 16, 32
+  float tune_us[2][6] = {{0}};  // timings of the stencil forms kStCand
   // data rows
   int64_t nrows = 0, nnz = 0, ndn = 0;
   int64_t* row_ptr = nullptr;
@@ -953,6 +1208,24 @@ struct lsamd_fd_system {
 };
 
 namespace {
+
+// Calls that run on the legacy null stream (synchronous hipMemcpy and
+// hipMemset, device synchronisation, launches on stream 0) would make that
+// stream depend on a stream another host thread is capturing into a graph,
+// which HIP rejects.  Handles driven from several host threads at once (one
+// stream each) therefore take this lock around every such call and around
+// every capture region, so the two never overlap.
+std::recursive_mutex g_legacy_mu;
+
+hipError_t legacy_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  std::lock_guard<std::recursive_mutex> lock(g_legacy_mu);
+  return hipMemcpy(dst, src, bytes, kind);
+}
+
+hipError_t legacy_memset(void* dst, int value, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lock(g_legacy_mu);
+  return hipMemset(dst, value, bytes);
+}
 
 int dmalloc(lsamd_fd_system* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -1019,52 +1292,68 @@ int vec_blocks(int64_t n) {
 
 constexpr int kZcCand[3] = {8, 16, 32};
 
-int zc_index(int zc) { return zc == 8 ? 0 : (zc == 16 ? 1 : 2); }
+// Stencil forms of a regular system: the z-chunk, plus kDmaZc when the bulk
+// planes are staged by direct global -> LDS copies (planes below 2 GiB).
+constexpr int kDmaZc = 1000;
+constexpr int kNStCand = 6;
+constexpr int kStCand[kNStCand] = {8, 16, 32, kDmaZc + 8, kDmaZc + 16, kDmaZc + 32};
 
-// P1 plane-streaming stencil: tiles per axis and z-chunks of the launch
-void p1_tiling(const lsamd_fd_system* s, int tx, int* ntx, int* nty, int* nzc) {
-  *ntx = (int)ceil_div(s->gd.nx, tx);
-  *nty = (int)ceil_div(s->gd.ny, 512 / tx);
-  *nzc = (int)ceil_div(s->gd.nz, kP1Zc);
+bool dma_fits(const GridDev& g, size_t elem) {
+  return (int64_t)g.nx * g.ny * (int64_t)elem < ((int64_t)1 << 31);
 }
 
-// Tile width of the P1 stencil: the one of 64x8 and 32x16 that covers the xy
-// plane with fewer padding nodes (64 on a tie); LSAMD_P1_STENCIL=node selects
-// the one-node-per-thread kernel.
-int p1_choose_tile(const lsamd_fd_system* s) {
-  if (const char* e = getenv("LSAMD_P1_STENCIL"))
-    if (strcmp(e, "node") == 0) return 0;
-  int64_t area[2];
-  const int cand[2] = {64, 32};
-  for (int c = 0; c < 2; ++c) {
-    int ntx, nty, nzc;
-    p1_tiling(s, cand[c], &ntx, &nty, &nzc);
-    area[c] = (int64_t)ntx * cand[c] * nty * (512 / cand[c]);
-  }
-  return area[1] < area[0] ? 32 : 64;
+int zc_index(int zc) {
+  zc %= kDmaZc;
+  return zc == 8 ? 0 : (zc == 16 ? 1 : 2);
+}
+
+// P1 plane-streaming stencil configurations (tile width, z-chunk); the
+// stencil's `zc` slot of a P1 system holds 1 + the configuration index, or
+// kP1Node for the one-node-per-thread kernel (LSAMD_P1_STENCIL=node).
+constexpr int kP1Cfg[4][2] = {{64, 8}, {64, 16}, {32, 8}, {32, 16}};
+constexpr int kP1Node = 100;
+constexpr int kP1Default = 2;  // (64, 16) before tuning
+
+void p1_tiling(const lsamd_fd_system* s, int cfg, int* ntx, int* nty, int* nzc) {
+  const int tx = kP1Cfg[cfg][0], zc = kP1Cfg[cfg][1];
+  *ntx = (int)ceil_div(s->gd.nx, tx);
+  *nty = (int)ceil_div(s->gd.ny, 512 / tx);
+  *nzc = (int)ceil_div(s->gd.nz, zc);
+}
+
+int64_t p1_blocks(const lsamd_fd_system* s, int cfg) {
+  int ntx, nty, nzc;
+  p1_tiling(s, cfg, &ntx, &nty, &nzc);
+  return (int64_t)ntx * nty * nzc;
+}
+
+// largest partial count over the P1 stencil forms (allocation)
+int64_t p1_max_partials(const lsamd_fd_system* s) {
+  int64_t m = ceil_div(s->n, 256);
+  for (int c = 0; c < 4; ++c) m = std::max(m, p1_blocks(s, c));
+  return m;
 }
 
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
   if (s->op == 2) return std::max(s->nblk, s->nbtile);
-  if (s->op == 1 && s->p1tx) {
-    int ntx, nty, nzc;
-    p1_tiling(s, s->p1tx, &ntx, &nty, &nzc);
-    return (int64_t)ntx * nty * nzc;
+  if (s->op == 1) {
+    if (zc == kP1Node) return ceil_div(s->n, 256);
+    return p1_blocks(s, (zc >= 1 && zc <= 4) ? zc - 1 : kP1Default - 1);
   }
   if (!s->regular) return ceil_div(s->n, 256);
   return s->nstile[zc_index(zc)];
 }
 
 // the regular single-system stencil: one launch over the system's work items
-template <typename T, int ZC>
+template <typename T, int ZC, bool DMA = false>
 void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
   const bool f32 = sizeof(T) == 4;
   const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
   const int zi = zc_index(ZC);
-  hipLaunchKernelGGL((k_b_tiles<T, ZC>), dim3((unsigned)s->nstile[zi]), dim3(NT), 0, st, x, y,
-                     (const BSys*)s->sys1, (const int4*)s->stiles[zi], tab, ku, s->part, ctl);
+  hipLaunchKernelGGL((k_b_tiles<T, ZC, DMA>), dim3((unsigned)s->nstile[zi]), dim3(NT), 0, st, x,
+                     y, (const BSys*)s->sys1, (const int4*)s->stiles[zi], tab, ku, s->part, ctl);
 }
 
 template <typename T>
@@ -1088,21 +1377,36 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
   if (s->op == 1) {
     const bool f32 = sizeof(T) == 4;
     const T* tab = f32 ? (const T*)s->p1tab32 : (const T*)s->p1tab64;
-    if (s->p1tx) {
-      int ntx, nty, nzc;
-      p1_tiling(s, s->p1tx, &ntx, &nty, &nzc);
-      const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
-      if (s->p1tx == 64)
-        hipLaunchKernelGGL((k_p1_tiles<T, 64>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
-                           s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, ntx, nzc, s->part, ctl);
-      else
-        hipLaunchKernelGGL((k_p1_tiles<T, 32>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
-                           s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, ntx, nzc, s->part, ctl);
-      return nb;
+    if (zc == kP1Node) {
+      hipLaunchKernelGGL(k_p1_stencil<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st,
+                         x, y, s->gd, tab, s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
+      return ceil_div(s->n, 256);
     }
-    hipLaunchKernelGGL(k_p1_stencil<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
-                       y, s->gd, tab, s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
-    return ceil_div(s->n, 256);
+    const int cfg = (zc >= 1 && zc <= 4) ? zc - 1 : kP1Default - 1;
+    int ntx, nty, nzc;
+    p1_tiling(s, cfg, &ntx, &nty, &nzc);
+    const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
+    const T r2 = (T)s->p1_ridge2;
+    const int* nv = s->p1nv;
+    switch (cfg) {
+      case 0:
+        hipLaunchKernelGGL((k_p1_tiles<T, 64, 8>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_p1_tiles<T, 64, 16>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_p1_tiles<T, 32, 8>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+      default:
+        hipLaunchKernelGGL((k_p1_tiles<T, 32, 16>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+    }
+    return nb;
   }
   if (!s->regular) {
     hipLaunchKernelGGL(k_st_general<T>, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st, x,
@@ -1112,27 +1416,70 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
   switch (zc) {
     case 8: launch_bulk<T, 8>(s, x, y, ctl, st); break;
     case 16: launch_bulk<T, 16>(s, x, y, ctl, st); break;
+    case kDmaZc + 8: launch_bulk<T, 8, true>(s, x, y, ctl, st); break;
+    case kDmaZc + 16: launch_bulk<T, 16, true>(s, x, y, ctl, st); break;
+    case kDmaZc + 32: launch_bulk<T, 32, true>(s, x, y, ctl, st); break;
     default: launch_bulk<T, 32>(s, x, y, ctl, st); break;
   }
   return stencil_partials(s, zc);
 }
 
-// Autotuner: time each z-chunk candidate (HIP events) and keep the fastest.
-// LSAMD_STENCIL_ZC in the environment pins the choice.
+// P1 autotuner: the tile shape (64 x 8 or 32 x 16) and z-chunk (8 or 16
+// planes) of the plane-streaming stencil, timed like the FD candidates.
+template <typename T>
+int tune_p1(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (const char* e = getenv("LSAMD_P1_STENCIL")) {
+    if (strcmp(e, "node") == 0) {
+      s->zc[pi] = kP1Node;
+      return LSAMD_OK;
+    }
+  }
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestc = kP1Default;
+  for (int round = 0; round < 3; ++round)
+    for (int c = 1; c <= 4; ++c) {
+      launch_stencil<T>(s, x, y, s->ctl_apply, st, c);
+      LSAMD_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 5; ++r) launch_stencil<T>(s, x, y, s->ctl_apply, st, c);
+      LSAMD_HIP(hipEventRecord(e1, st));
+      LSAMD_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bestc = c;
+      }
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("P1 stencil autotune");
+  s->zc[pi] = bestc;
+  return LSAMD_OK;
+}
+
+// Autotuner: time each stencil form (z-chunk 8/16/32 with register-staged
+// planes, 8/16 with LDS-DMA staged planes; HIP events) and keep the fastest.
+// LSAMD_STENCIL_ZC in the environment pins the choice (a kStCand value).
 template <typename T>
 int ensure_tuned(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
   const int pi = sizeof(T) == 4 ? 1 : 0;
   if (s->zc[pi]) return LSAMD_OK;
+  if (s->op == 1) return tune_p1<T>(s, x, y, st);
   if (!s->regular) {
     s->zc[pi] = 1;
     return LSAMD_OK;
   }
   if (const char* e = getenv("LSAMD_STENCIL_ZC")) {
     const int v = atoi(e);
-    if (v == 8 || v == 16 || v == 32) {
-      s->zc[pi] = v;
-      return LSAMD_OK;
-    }
+    for (int c = 0; c < kNStCand; ++c)
+      if (v == kStCand[c] && (v < kDmaZc || dma_fits(s->gd, sizeof(T)))) {
+        s->zc[pi] = v;
+        return LSAMD_OK;
+      }
   }
   hipEvent_t e0, e1;
   LSAMD_HIP(hipEventCreate(&e0));
@@ -1142,11 +1489,14 @@ int ensure_tuned(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
   // perturb single rounds)
   float best = 1e30f;
   int bestzc = 16;
-  for (int c = 0; c < 3; ++c) s->tune_us[pi][c] = -1.f;
+  for (int c = 0; c < kNStCand; ++c) s->tune_us[pi][c] = -1.f;
   for (int round = 0; round < 3; ++round)
-    for (int c = 0; c < 3; ++c) {
-      const int zc = kZcCand[c];
-      if (zc > 8 && s->gd.nz - 6 < zc) continue;  // a chunk longer than the bulk gains nothing
+    for (int c = 0; c < kNStCand; ++c) {
+      const int zc = kStCand[c];
+      // a chunk longer than the bulk gains nothing; LDS-DMA forms address a
+      // plane with 32-bit byte offsets
+      if (zc % kDmaZc > 8 && s->gd.nz - 6 < zc % kDmaZc) continue;
+      if (zc > kDmaZc && !dma_fits(s->gd, sizeof(T))) continue;
       launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
       LSAMD_HIP(hipEventRecord(e0, st));
       for (int r = 0; r < 5; ++r) launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
@@ -1218,14 +1568,14 @@ int ensure_work(lsamd_fd_system* s, bool f32) {
     if (int rc = dalloc(s, &s->r64, n)) return rc;
     if (int rc = dalloc(s, &s->p64, n)) return rc;
     if (int rc = dalloc(s, &s->q64, n)) return rc;
-    for (double* v : {s->x64, s->r64, s->p64, s->q64}) LSAMD_HIP(hipMemset(v, 0, n * 8));
+    for (double* v : {s->x64, s->r64, s->p64, s->q64}) LSAMD_HIP(legacy_memset(v, 0, n * 8));
   }
   if (f32 && !s->x32) {
     if (int rc = dalloc(s, &s->x32, n)) return rc;
     if (int rc = dalloc(s, &s->r32, n)) return rc;
     if (int rc = dalloc(s, &s->p32, n)) return rc;
     if (int rc = dalloc(s, &s->q32, n)) return rc;
-    for (float* v : {s->x32, s->r32, s->p32, s->q32}) LSAMD_HIP(hipMemset(v, 0, n * 4));
+    for (float* v : {s->x32, s->r32, s->p32, s->q32}) LSAMD_HIP(legacy_memset(v, 0, n * 4));
   }
   return LSAMD_OK;
 }
@@ -1241,6 +1591,7 @@ int run_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const T
     s->graph[gi] = nullptr;
   }
   if (!s->graph[gi]) {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
@@ -1269,12 +1620,12 @@ int run_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const T
 // Work buffers, control blocks and the regulariser-only diagonal shared by
 // every system kind.
 int init_common(lsamd_fd_system* s) {
-  s->part_n = std::max<int64_t>(stencil_partials(s, 8), 8192);
+  s->part_n = std::max<int64_t>(s->op == 1 ? p1_max_partials(s) : stencil_partials(s, 8), 8192);
   if (int rc = dalloc(s, &s->part, s->part_n)) return rc;
   if (int rc = dalloc(s, &s->ctl, 1)) return rc;
   if (int rc = dalloc(s, &s->ctl_apply, 1)) return rc;
-  LSAMD_HIP(hipMemset(s->ctl, 0, sizeof(CgCtl)));
-  LSAMD_HIP(hipMemset(s->ctl_apply, 0, sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memset(s->ctl, 0, sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memset(s->ctl_apply, 0, sizeof(CgCtl)));
   if (hipHostMalloc((void**)&s->ctl_host, sizeof(CgCtl), hipHostMallocDefault) != hipSuccess)
     return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed");
   if (int rc = dalloc(s, &s->rhs, s->n)) return rc;
@@ -1282,6 +1633,7 @@ int init_common(lsamd_fd_system* s) {
   if (int rc = dalloc(s, &s->minv64, s->n)) return rc;
   if (int rc = dalloc(s, &s->minv32, s->n)) return rc;
   // regulariser-only system until rows are set
+  std::lock_guard<std::recursive_mutex> lock(g_legacy_mu);
   launch_reg_diag(s, 0);
   hipLaunchKernelGGL(k_minv, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, 0, s->diag,
                      s->minv64, s->minv32, s->n);
@@ -1353,6 +1705,7 @@ int run_b_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const
     s->bgraph[gi] = nullptr;
   }
   if (!s->bgraph[gi]) {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
@@ -1971,7 +2324,7 @@ int mg_level_vectors(Mg* m, MgLevel& L) {
   if (int rc = mg_alloc(m, &L.x, L.n)) return rc;
   if (int rc = mg_alloc(m, &L.q, L.n)) return rc;
   if (int rc = mg_alloc(m, &L.d, L.n)) return rc;
-  for (float* v : {L.b, L.x, L.q, L.d}) LSAMD_HIP(hipMemset(v, 0, L.n * 4));
+  for (float* v : {L.b, L.x, L.q, L.d}) LSAMD_HIP(legacy_memset(v, 0, L.n * 4));
   return LSAMD_OK;
 }
 
@@ -2040,7 +2393,7 @@ int mg_build_static(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   m->nc = (int)nc;
   if (int rc = mg_alloc(m, &m->cinv, nc * nc)) return rc;
   if (int rc = mg_alloc(m, &m->rp64, s->n)) return rc;
-  LSAMD_HIP(hipMemset(m->rp64, 0, s->n * 8));
+  LSAMD_HIP(legacy_memset(m->rp64, 0, s->n * 8));
   return LSAMD_OK;
 }
 
@@ -2298,6 +2651,7 @@ int run_fcg(lsamd_fd_system* s, int chunk, hipStream_t st) {
     m->graph = nullptr;
   }
   if (!m->graph) {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
@@ -2390,6 +2744,12 @@ struct MrWork {
   int zcR[2] = {0, 0};
   CgCtl* ctlR = nullptr;
   double* partR = nullptr;
+  // coarse levels on regular grids (index = level; entries of level 0 and of
+  // irregular levels unused): the same kernel over the R lanes with the
+  // level's class table, the Gram rows added for all lanes after it
+  std::vector<BSys*> lsys;
+  std::vector<int4*> ltiles;
+  std::vector<int> lnt, lzc;
   std::vector<void*> mem;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
@@ -2424,6 +2784,10 @@ int mr_alloc(MrWork* w, P** p, int64_t count) {
 
 // lane vectors, controls and the level-0 stencil plan of R lanes for the
 // system's hierarchy
+template <typename T>
+void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st,
+              const CgCtl* ctl = nullptr);
+
 int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
   const Mg* m = s->mg;
   const int64_t n = s->n;
@@ -2432,7 +2796,7 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
     float *b, *x, *q, *d;
     for (float** v : {&b, &x, &q, &d}) {
       if (int rc = mr_alloc(w, v, L.n * R)) return rc;
-      LSAMD_HIP(hipMemset(*v, 0, L.n * R * 4));
+      LSAMD_HIP(legacy_memset(*v, 0, L.n * R * 4));
     }
     w->b.push_back(b);
     w->x.push_back(x);
@@ -2442,7 +2806,7 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
   if (int rc = injected("multi_rhs_work")) return rc;
   for (double** v : {&w->rhs, &w->x64, &w->r64, &w->p64, &w->q64}) {
     if (int rc = mr_alloc(w, v, n * R)) return rc;
-    LSAMD_HIP(hipMemset(*v, 0, n * R * 8));
+    LSAMD_HIP(legacy_memset(*v, 0, n * R * 8));
   }
   if (int rc = mr_alloc(w, &w->part, (int64_t)vec_blocks(n) * 2 * R)) return rc;
   if (int rc = mr_alloc(w, &w->ctl, R)) return rc;
@@ -2461,21 +2825,100 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
     hs[r].off = (int64_t)r * n;
   }
   if (int rc = mr_alloc(w, &w->sysR, R)) return rc;
-  LSAMD_HIP(hipMemcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
+  // work-item tables of every z-chunk candidate; the R lanes make this level
+  // HBM-bound (unlike one cache-resident system), so the chunk is timed here
+  // per precision rather than taken from the single-system tuning
   int64_t maxt = 1;
-  for (int p = 0; p < 2; ++p) {
-    const int zc = s->zc[p] == 8 || s->zc[p] == 16 || s->zc[p] == 32 ? s->zc[p] : 16;
+  int4* tc[3] = {nullptr, nullptr, nullptr};
+  int ntc[3] = {0, 0, 0};
+  for (int c = 0; c < 3; ++c) {
     std::vector<int4> t;
-    for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, zc, t);
-    if (int rc = mr_alloc(w, &w->tilesR[p], (int64_t)t.size())) return rc;
-    LSAMD_HIP(hipMemcpy(w->tilesR[p], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
-    w->ntR[p] = (int)t.size();
-    w->zcR[p] = zc;
+    for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, kZcCand[c], t);
+    if (int rc = mr_alloc(w, &tc[c], (int64_t)t.size())) return rc;
+    LSAMD_HIP(legacy_memcpy(tc[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    ntc[c] = (int)t.size();
     maxt = std::max<int64_t>(maxt, (int64_t)t.size());
   }
   if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
-  LSAMD_HIP(hipMemset(w->ctlR, 0, R * sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memset(w->ctlR, 0, R * sizeof(CgCtl)));
   if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
+  // coarse-level plans (the coarsest level is solved densely)
+  w->lsys.assign(m->lv.size(), nullptr);
+  w->ltiles.assign(m->lv.size(), nullptr);
+  w->lnt.assign(m->lv.size(), 0);
+  w->lzc.assign(m->lv.size(), 0);
+  for (size_t l = 1; l + 1 < m->lv.size(); ++l) {
+    const MgLevel& L = m->lv[l];
+    if (!L.sys->regular) continue;
+    std::vector<BSys> ls(R);
+    memset(ls.data(), 0, R * sizeof(BSys));
+    for (int r = 0; r < R; ++r) {
+      ls[r].g = L.g;
+      ls[r].n = L.n;
+      ls[r].off = (int64_t)r * L.n;
+    }
+    if (int rc = mr_alloc(w, &w->lsys[l], R)) return rc;
+    LSAMD_HIP(legacy_memcpy(w->lsys[l], ls.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
+    const int zc = L.g.nz - 6 >= 16 ? 16 : 8;
+    std::vector<int4> t;
+    for (int r = 0; r < R; ++r) stencil_tiles(r, L.g, zc, t);
+    if (int rc = mr_alloc(w, &w->ltiles[l], (int64_t)t.size())) return rc;
+    LSAMD_HIP(legacy_memcpy(w->ltiles[l], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    w->lnt[l] = (int)t.size();
+    w->lzc[l] = zc;
+    if ((int64_t)t.size() > maxt) {
+      LSAMD_HIP(hipFree(w->partR));
+      for (size_t i = 0; i < w->mem.size(); ++i)
+        if (w->mem[i] == (void*)w->partR) w->mem.erase(w->mem.begin() + i);
+      w->partR = nullptr;
+      maxt = (int64_t)t.size();
+      if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
+    }
+  }
+  const char* pin = getenv("LSAMD_STENCIL_ZC");
+  const int pinned = pin ? atoi(pin) : 0;
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  for (int p = 0; p < 2; ++p) {
+    float best = 1e30f;
+    int bc = 1;
+    for (int round = 0; round < 2; ++round)
+      for (int c = 0; c < 3; ++c) {
+        if (pinned == 8 || pinned == 16 || pinned == 32) {
+          if (kZcCand[c] != pinned) continue;
+        } else if (kZcCand[c] > 8 && s->gd.nz - 6 < kZcCand[c]) {
+          continue;
+        }
+        w->tilesR[p] = tc[c];
+        w->ntR[p] = ntc[c];
+        w->zcR[p] = kZcCand[c];
+        auto run = [&]() {
+          if (p)
+            mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], 0);
+          else
+            mr_tiles<double>(s, w, (const double*)w->p64, w->q64, 0);
+        };
+        run();
+        LSAMD_HIP(hipEventRecord(e0, 0));
+        for (int k = 0; k < 3; ++k) run();
+        LSAMD_HIP(hipEventRecord(e1, 0));
+        LSAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bc = c;
+        }
+      }
+    w->tilesR[p] = tc[bc];
+    w->ntR[p] = ntc[bc];
+    w->zcR[p] = kZcCand[bc];
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("multi-rhs stencil tuning");
   return LSAMD_OK;
 }
 
@@ -2527,10 +2970,13 @@ int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
   return LSAMD_OK;
 }
 
-// level-0 regulariser over the R lanes with the plane-streaming kernel
+// level-0 regulariser over the R lanes with the plane-streaming kernel (ctl:
+// per-lane control blocks, the group's own by default)
 template <typename T>
-void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st) {
+void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st,
+              const CgCtl* ctl) {
   const int p = sizeof(T) == 4 ? 1 : 0;
+  if (!ctl) ctl = (const CgCtl*)w->ctlR;
   const T* tab = p ? (const T*)s->tab32 : (const T*)s->tab64;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = p ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
@@ -2538,18 +2984,22 @@ void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st) {
   switch (w->zcR[p]) {
     case 8:
       hipLaunchKernelGGL((k_b_tiles<T, 8>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
       break;
     case 32:
       hipLaunchKernelGGL((k_b_tiles<T, 32>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
       break;
     default:
       hipLaunchKernelGGL((k_b_tiles<T, 16>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, (const CgCtl*)w->ctlR);
+                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
       break;
   }
 }
+
+// coarse levels of at least this many nodes add their Gram term four lanes
+// per thread, smaller ones one lane per thread
+constexpr int64_t kMrGramGroup = 65536;
 
 template <int R>
 void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hipStream_t st) {
@@ -2567,6 +3017,29 @@ void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hi
       hipLaunchKernelGGL(k_mr_rows_scatter<float>,
                          dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0,
                          st, Rw, (const float*)w->t32, y, L.n, R, ad);
+    return;
+  }
+  if (l < (int)w->ltiles.size() && w->ltiles[l]) {
+    const lsamd_fd_system* ls = L.sys;
+    const dim3 g((unsigned)w->lnt[l]), b(NT);
+    if (w->lzc[l] == 16)
+      hipLaunchKernelGGL((k_b_tiles<float, 16>), g, b, 0, st, x, y, (const BSys*)w->lsys[l],
+                         (const int4*)w->ltiles[l], (const float*)ls->tab32, ls->ku32, w->partR,
+                         (const CgCtl*)w->ctlR);
+    else
+      hipLaunchKernelGGL((k_b_tiles<float, 8>), g, b, 0, st, x, y, (const BSys*)w->lsys[l],
+                         (const int4*)w->ltiles[l], (const float*)ls->tab32, ls->ku32, w->partR,
+                         (const CgCtl*)w->ctlR);
+    if (L.gptr) {
+      if (L.n >= kMrGramGroup)
+        hipLaunchKernelGGL(k_mr_gram_add<4>, dim3((unsigned)ceil_div(L.n, 256), R / 4), b256, 0,
+                           st, (const int64_t*)L.gptr, (const int32_t*)L.gcol,
+                           (const float*)L.gval, L.n, x, y, ad);
+      else
+        hipLaunchKernelGGL(k_mr_gram_add<1>, dim3((unsigned)ceil_div(L.n, 256), R), b256, 0, st,
+                           (const int64_t*)L.gptr, (const int32_t*)L.gcol, (const float*)L.gval,
+                           L.n, x, y, ad);
+    }
     return;
   }
   if (L.n >= kMrNodeThreads)
@@ -2702,6 +3175,7 @@ int mr_solve_group(lsamd_fd_system* s, MrWork* w, const lsamd_solve_opts* o, int
   LSAMD_CHECK_LAUNCH("multi-rhs init");
   if (w->graph && w->graph_chunk != chunk) mr_invalidate(w);
   if (!w->graph) {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
@@ -2829,10 +3303,10 @@ int ls_build(lsamd_fd_system* s, Ls* l) {
   if (hipHostMalloc((void**)&l->host, sizeof(LsCtl), hipHostMallocDefault) != hipSuccess)
     return fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed (LSMR)");
   for (double* p : {l->v, l->h, l->hbar, l->x, l->xs, l->y, l->urid})
-    LSAMD_HIP(hipMemset(p, 0, n * 8));
-  LSAMD_HIP(hipMemset(l->ureg, 0, 6 * n * 8));
-  LSAMD_HIP(hipMemset(l->ud, 0, m * 8));
-  LSAMD_HIP(hipMemset(l->tr, 0, m * 8));
+    LSAMD_HIP(legacy_memset(p, 0, n * 8));
+  LSAMD_HIP(legacy_memset(l->ureg, 0, 6 * n * 8));
+  LSAMD_HIP(legacy_memset(l->ud, 0, m * 8));
+  LSAMD_HIP(legacy_memset(l->tr, 0, m * 8));
   return LSAMD_OK;
 }
 
@@ -2925,6 +3399,7 @@ int ls_solve(lsamd_fd_system* s, const lsamd_lsmr_opts* o, double* c, lsamd_lsmr
     l->graph = nullptr;
   }
   if (!l->graph) {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
@@ -2989,7 +3464,7 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
   };
   if (scale2) {
     if (int rc = dalloc(s, &s->scale2, s->n)) return bail(rc);
-    LSAMD_HIP(hipMemcpy(s->scale2, scale2, s->n * sizeof(double), hipMemcpyDefault));
+    LSAMD_HIP(legacy_memcpy(s->scale2, scale2, s->n * sizeof(double), hipMemcpyDefault));
   }
   s->regular = scale2 == nullptr && s->gd.nx >= 7 && s->gd.ny >= 7 && s->gd.nz >= 7;
   if (s->regular) {
@@ -3000,21 +3475,21 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
     one.g = s->gd;
     one.n = s->n;
     if (int rc = dalloc(s, &s->sys1, 1)) return bail(rc);
-    LSAMD_HIP(hipMemcpy(s->sys1, &one, sizeof(BSys), hipMemcpyHostToDevice));
+    LSAMD_HIP(legacy_memcpy(s->sys1, &one, sizeof(BSys), hipMemcpyHostToDevice));
     for (int c = 0; c < 3; ++c) {
       std::vector<int4> t;
       stencil_tiles(0, s->gd, kZcCand[c], t);
       s->nstile[c] = (int)t.size();
       if (int rc = dalloc(s, &s->stiles[c], t.size())) return bail(rc);
-      LSAMD_HIP(hipMemcpy(s->stiles[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+      LSAMD_HIP(legacy_memcpy(s->stiles[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     std::vector<double> tab(kClasses * kTaps);
     build_class_table(s->reg, tab.data());
     std::vector<float> tabf(tab.begin(), tab.end());
     if (int rc = dalloc(s, &s->tab64, kClasses * kTaps)) return bail(rc);
     if (int rc = dalloc(s, &s->tab32, kClasses * kTaps)) return bail(rc);
-    LSAMD_HIP(hipMemcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
-    LSAMD_HIP(hipMemcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
+    LSAMD_HIP(legacy_memcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    LSAMD_HIP(legacy_memcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
     const int bulk = ((3 * 7 + 3) * 7 + 3) * kTaps;
     for (int q = 0; q < kTaps; ++q) {
       s->ku64.k[q] = tab[bulk + q];
@@ -3041,7 +3516,6 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
   s->n = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
   s->op = 1;
   s->p1_ridge2 = ridge2;
-  s->p1tx = p1_choose_tile(s);
   auto bail = [&](int rc) {
     lsamd_fd_system_destroy(s);
     return rc;
@@ -3052,8 +3526,8 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
   std::vector<float> tabf(tab.begin(), tab.begin() + nt);
   if (int rc = dalloc(s, &s->p1tab64, nt)) return bail(rc);
   if (int rc = dalloc(s, &s->p1tab32, nt)) return bail(rc);
-  LSAMD_HIP(hipMemcpy(s->p1tab64, tab.data(), nt * 8, hipMemcpyHostToDevice));
-  LSAMD_HIP(hipMemcpy(s->p1tab32, tabf.data(), nt * 4, hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->p1tab64, tab.data(), nt * 8, hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->p1tab32, tabf.data(), nt * 4, hipMemcpyHostToDevice));
   if (int rc = init_common(s)) return bail(rc);
   *out = s;
   return LSAMD_OK;
@@ -3215,7 +3689,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     LSAMD_HIP(hipStreamSynchronize(st));
     hoff[nr] = (int32_t)nnz;
     std::vector<int64_t> hoff64(hoff.begin(), hoff.end());
-    LSAMD_HIP(hipMemcpy(s->seg_ptr, hoff64.data(), (nr + 1) * 8, hipMemcpyHostToDevice));
+    LSAMD_HIP(legacy_memcpy(s->seg_ptr, hoff64.data(), (nr + 1) * 8, hipMemcpyHostToDevice));
     LSAMD_HIP(hipMemcpyAsync(s->dnode, uniq, nr * 4, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_gather_i32, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, st, erow,
                        s->seg_entry, s->seg_row, nnz);
@@ -3435,6 +3909,58 @@ int lsamd_fd_system_rhs_rows(lsamd_fd_system* s, const double* b, double* rhs, v
   return LSAMD_OK;
 }
 
+int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters,
+                               double* avg_ms, double* bytes, void* stream) {
+  if (!s || !avg_ms || !bytes || iters < 1 || kernel < 0 || kernel > 2)
+    return fail(LSAMD_ERR_ARG, "time_multi: bad arguments");
+  MrWork* w = s->mr[3];
+  if (!w || w->R != 32 || !s->mg || !s->regular)
+    return fail(LSAMD_ERR_ARG, "time_multi: needs a 32-lane solve_multi on this handle first");
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int R = 32;
+  if (!s->tzero) {  // no lane done, all-done flag clear
+    if (int rc = dmalloc(s, &s->tzero, R * sizeof(CgCtl))) return rc;
+    LSAMD_HIP(legacy_memset(s->tzero, 0, R * sizeof(CgCtl)));
+  }
+  const int* zflag = (const int*)s->tzero;
+  const MgLevel& L = s->mg->lv[0];
+  const int64_t n = s->n;
+  const int split = L.cov ? 1 : 0;
+  const int64_t nu = split ? n - L.ncov : n;  // nodes of the Chebyshev step
+  auto launch = [&]() {
+    if (kernel == 0)
+      hipLaunchKernelGGL(k_mr_cheb, dim3((unsigned)ceil_div(n, 256), R), dim3(256), 0, st,
+                         (const float*)w->b[0], (const float*)w->q[0], mg_sm(L), w->x[0],
+                         w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
+    else if (kernel == 1)
+      hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n)), dim3(256), 0, st, w->x64, w->r64,
+                         (const double*)w->p64, (const double*)w->q64,
+                         (const double*)s->minv64, w->b[0], n, w->part, (const MrCtl*)w->ctl,
+                         zflag);
+    else
+      mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], st, (const CgCtl*)s->tzero);
+  };
+  // algorithmic bytes: every input and output element once
+  *bytes = kernel == 0   ? 24.0 * R * (double)nu + 4.0 * (double)n
+           : kernel == 1 ? 52.0 * R * (double)n + 8.0 * (double)n
+                         : 8.0 * R * (double)n;
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  launch();
+  LSAMD_HIP(hipEventRecord(e0, st));
+  for (int it = 0; it < iters; ++it) launch();
+  LSAMD_HIP(hipEventRecord(e1, st));
+  LSAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("time_multi");
+  *avg_ms = ms / iters;
+  return LSAMD_OK;
+}
+
 int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
                                  double* avg_ms, void* stream) {
   if (!s || !avg_ms || iters < 1) return fail(LSAMD_ERR_ARG, "time_stencil: bad arguments");
@@ -3528,7 +4054,7 @@ int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t*
   const int pi = precision == LSAMD_FP32 ? 1 : 0;
   *zc = s->regular ? s->zc[pi] : 0;
   if (tune_us)
-    for (int c = 0; c < 3; ++c) tune_us[c] = s->tune_us[pi][c];
+    for (int c = 0; c < kNStCand; ++c) tune_us[c] = s->tune_us[pi][c];
   return LSAMD_OK;
 }
 
@@ -3621,12 +4147,12 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   s->bsys_h = hs;  // with the tile ranges
   s->nbtile = (int)tiles.size();
   if (int rc = dalloc(s, &s->btiles, tiles.size())) return bail(rc);
-  LSAMD_HIP(hipMemcpy(s->btiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-  LSAMD_HIP(hipMemcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
-  LSAMD_HIP(hipMemcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-  LSAMD_HIP(hipMemset(s->bctl, 0, nsys * sizeof(CgCtl)));
-  LSAMD_HIP(hipMemset(s->bctl_apply, 0, nsys * sizeof(CgCtl)));
-  LSAMD_HIP(hipMemset(s->bctl64, 0, nsys * sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memcpy(s->btiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memset(s->bctl, 0, nsys * sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memset(s->bctl_apply, 0, nsys * sizeof(CgCtl)));
+  LSAMD_HIP(legacy_memset(s->bctl64, 0, nsys * sizeof(CgCtl)));
   if (hipHostMalloc((void**)&s->bctl_host, nsys * sizeof(CgCtl), hipHostMallocDefault) !=
       hipSuccess)
     return bail(fail(LSAMD_ERR_ALLOC, "hipHostMalloc failed"));
@@ -3635,8 +4161,8 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   std::vector<float> tabf(tab.begin(), tab.end());
   if (int rc = dalloc(s, &s->tab64, kClasses * kTaps)) return bail(rc);
   if (int rc = dalloc(s, &s->tab32, kClasses * kTaps)) return bail(rc);
-  LSAMD_HIP(hipMemcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
-  LSAMD_HIP(hipMemcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->tab64, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->tab32, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice));
   const int bulk = ((3 * 7 + 3) * 7 + 3) * kTaps;
   for (int q = 0; q < kTaps; ++q) {
     s->bku64.k[q] = tab[bulk + q];
@@ -3670,7 +4196,7 @@ int lsamd_batch_set_rows(lsamd_fd_system* s, const int64_t* sys_rows, int64_t nr
     s->bsys_h[i].r0 = sys_rows[i];
     s->bsys_h[i].r1 = sys_rows[i + 1];
   }
-  LSAMD_HIP(hipMemcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys), hipMemcpyHostToDevice));
+  LSAMD_HIP(legacy_memcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys), hipMemcpyHostToDevice));
   if (nrows > 0) {
     hipStream_t st = (hipStream_t)stream;
     int* bad = nullptr;
@@ -3685,7 +4211,7 @@ int lsamd_batch_set_rows(lsamd_fd_system* s, const int64_t* sys_rows, int64_t nr
     if (hb) {
       lsamd_fd_system_set_rows(s, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, stream);
       for (auto& q : s->bsys_h) q.r0 = q.r1 = 0;
-      LSAMD_HIP(hipMemcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys),
+      LSAMD_HIP(legacy_memcpy(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys),
                           hipMemcpyHostToDevice));
       return fail(LSAMD_ERR_ARG, "lsamd_batch_set_rows: a row addresses another system's nodes");
     }

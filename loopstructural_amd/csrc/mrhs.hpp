@@ -181,6 +181,36 @@ __global__ __launch_bounds__(256) void k_mr_apply_coarse_all(MgDev L,
   for (int r = 0; r < R; ++r) y[r * n + id] = acc[r];
 }
 
+// y += G x for the lanes of a coarse level (G its Gram rows), LG lanes per
+// thread (lane group in blockIdx.y): the row is read once for the LG lanes;
+// small levels take LG = 1 so that every lane has threads of its own.  The
+// regulariser part of y is already there.
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_gram_add(const int64_t* __restrict__ gptr,
+                                                     const int32_t* __restrict__ gcol,
+                                                     const float* __restrict__ gval, int64_t n,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ y,
+                                                     const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int64_t e0 = gptr[id], e1 = gptr[id + 1];
+  if (e0 == e1) return;
+  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int64_t e = e0; e < e1; ++e) {
+    const float a = gval[e];
+    const int64_t o = gcol[e];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) acc[l] += a * x[(r0 + l) * n + o];
+  }
+#pragma unroll
+  for (int l = 0; l < LG; ++l) y[(r0 + l) * n + id] += acc[l];
+}
+
 // ----------------------------------------------------------- smoother ----
 // Chebyshev step with the Schwarz preconditioner M: mode 0 first step from
 // zero (d = c0 M b, x = d), 1 first step (d = c0 M (b - q), x += d), 2 later

@@ -9,6 +9,7 @@ its Jacobi diagonal, rhs and the PCG driver.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -241,13 +242,27 @@ class FdSystem:
                   _lib.stream())
         return float(ms.value)
 
+    def time_multi(self, kernel: str, iters=20):
+        """Average launch time (ms) and algorithmic bytes of one fine-level
+        kernel of the 32-lane multi-rhs multigrid ('cheb', 'update' or
+        'stencil'), after a solve_multi with a 32-lane group; the group's work
+        is left undefined."""
+        k = {"cheb": 0, "update": 1, "stencil": 2}[kernel]
+        ms, nb = C.c_double(0.0), C.c_double(0.0)
+        _lib.call("lsamd_fd_system_time_multi", self._h, k, int(iters), C.byref(ms),
+                  C.byref(nb), _lib.stream())
+        return float(ms.value), float(nb.value)
+
     def stencil_info(self, precision="fp64"):
-        """Autotuned z-chunk (0: per-node kernel) and candidate timings (us)."""
+        """Autotuned stencil form (z-chunk 8/16/32, or 1008/1016/1032 for
+        z-chunk 8/16/32 with LDS-DMA staged planes; 0: per-node kernel) and
+        the timings of every form (us, -1 = skipped)."""
         zc = C.c_int32(0)
-        us = (C.c_float * 3)()
+        us = (C.c_float * 6)()
         _lib.call("lsamd_fd_system_stencil_info", self._h,
                   _lib.FP32 if precision == "fp32" else _lib.FP64, C.byref(zc), us)
-        return {"zc": int(zc.value), "candidates_us": {8: us[0], 16: us[1], 32: us[2]}}
+        return {"zc": int(zc.value),
+                "candidates_us": dict(zip((8, 16, 32, 1008, 1016, 1032), list(us)))}
 
     def adjoint_rows(self, c: torch.Tensor, lam: torch.Tensor):
         R = self.rows.nrows if self.rows is not None else 0
@@ -522,3 +537,99 @@ class BatchSystem:
                   _lib.FP32 if precision == "fp32" else _lib.FP64, int(iters), C.byref(ms),
                   _lib.stream())
         return float(ms.value)
+
+
+class EnsembleSolver:
+    """Multigrid solves of the independent FD systems of an uneven ensemble,
+    several at a time.
+
+    Systems are grouped by grid shape; each shape gets one handle, whose
+    coarse grids, tuned stencils and level vectors are built once (a set-up
+    solve of one iteration) and reused for every system of that shape (only
+    the row-dependent parts are rebuilt per system).  Shape groups are spread
+    over ``workers`` host threads, largest first, each thread driving its own
+    stream, so solves of different shapes overlap on the GPU.  The solves
+    are the single-system multigrid solves: same stopping test and minimiser
+    as ``FdSystem.solve(multigrid=True)``."""
+
+    def __init__(self, grids, reg: _lib.FdReg, workers: int = 4):
+        self.dev = _lib.device()
+        self.grids = list(grids)
+        self.reg = reg
+        self.workers = max(1, int(workers))
+        self.shapes = {}
+        for s, g in enumerate(self.grids):
+            key = (tuple(float(v) for v in g.origin), tuple(float(v) for v in g.step),
+                   tuple(int(v) for v in g.nnodes))
+            self.shapes.setdefault(key, []).append(s)
+        self.handles = {}
+        self.last_timing = {}
+
+    def close(self):
+        for h in self.handles.values():
+            h.close()
+        self.handles = {}
+
+    def prepare(self, blocks, rtol=1e-10):
+        """Create one handle per shape and build its hierarchy (one-iteration
+        solve with the rows of the shape's first system)."""
+        for key, members in self.shapes.items():
+            if key in self.handles:
+                continue
+            h = FdSystem(self.grids[members[0]], self.reg)
+            h.set_rows(blocks[members[0]])
+            h.solve(rtol=rtol, maxiter=1, multigrid=True, check_every=8)
+            self.handles[key] = h
+        torch.cuda.synchronize()
+
+    def solve(self, blocks, rtol=1e-5, atol=0.0, maxiter=None):
+        """``blocks[s]``: FdRows of system s.  Returns (list of c, list of infos)."""
+        import threading
+
+        self.prepare(blocks, rtol)
+        out = [None] * len(self.grids)
+        infos = [None] * len(self.grids)
+        order = sorted(self.shapes, key=lambda k: -sum(int(np.prod(self.grids[s].nnodes))
+                                                        for s in self.shapes[k]))
+        lock = threading.Lock()
+        queue = list(order)
+        errors = []
+        spent = [0.0, 0.0]  # host-thread seconds in set_rows, in solve
+
+        def work():
+            stream = torch.cuda.Stream(device=self.dev)
+            with torch.cuda.stream(stream):
+                while True:
+                    with lock:
+                        if not queue or errors:
+                            break
+                        key = queue.pop(0)
+                    try:
+                        h = self.handles[key]
+                        for s in self.shapes[key]:
+                            t0 = time.perf_counter()
+                            h.set_rows(blocks[s])
+                            t1 = time.perf_counter()
+                            c, info = h.solve(rtol=rtol, atol=atol, maxiter=maxiter,
+                                              multigrid=True, check_every=8)
+                            t2 = time.perf_counter()
+                            out[s], infos[s] = c, info
+                            with lock:
+                                spent[0] += t1 - t0
+                                spent[1] += t2 - t1
+                    except Exception as e:  # surfaced on the calling thread
+                        with lock:
+                            errors.append(e)
+                        break
+            stream.synchronize()
+
+        threads = [threading.Thread(target=work) for _ in range(min(self.workers, len(order)))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        torch.cuda.synchronize()
+        self.last_timing = {"set_rows_thread_s": spent[0], "solve_thread_s": spent[1]}
+        return out, infos
