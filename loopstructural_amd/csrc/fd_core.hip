@@ -597,7 +597,8 @@ __device__ __forceinline__ double p1_body(const T* __restrict__ x, T* __restrict
       co[q] = rowO[q];
     }
   }
-  // planes pb .. pb + NS - 1 (a ZIN chunk is a full chunk)
+  // planes pb .. pb + NS - 1; a ZIN chunk runs ZC + 4 planes (a short last
+  // chunk reads zeroed planes past the grid and writes no output there)
   const int pb = z0 - 2, NS = ZIN ? ZC + 4 : z1 - z0 + 4;
   T qa[D], qb[D];
   auto ld = [&](int p, T& va, T& vb) {
@@ -623,7 +624,7 @@ __device__ __forceinline__ double p1_body(const T* __restrict__ x, T* __restrict
     if (s + D < NS) ld(p + D, qa[(s + D) % D], qb[(s + D) % D]);
     const T* bb = pl[s & 1] + (ly + 2) * QX + lx + 2;
     const T c = bb[0];
-    const bool pin = ZIN || (p >= 0 && p < g.nz);
+    const bool pin = p >= 0 && p < g.nz;
     if (col && pin) {
       // Rs: row of an output in this plane (or two away), Rn: one plane away
       auto Rs = [&](int q, int z) -> T {
@@ -669,7 +670,7 @@ __device__ __forceinline__ double p1_body(const T* __restrict__ x, T* __restrict
     __syncthreads();
   };
   if constexpr (ZIN) {
-    // z0 even (ZC even) so plane s has the parity of s; NS = ZC + 4 is even
+    // z0 even so plane s has the parity of s; NS = ZC + 4 is even
 #pragma unroll
     for (int s = 0; s < NS; s += 2) {
       plane(s, std::false_type{});
@@ -679,6 +680,32 @@ __device__ __forceinline__ double p1_body(const T* __restrict__ x, T* __restrict
     for (int s = 0; s < NS; ++s) plane(s, std::false_type{});
   }
   return dot;
+}
+
+// z chunk bz of a P1 grid: on a z axis of more than 7 nodes, [0, 2), then
+// chunks of zc output planes over [2, nz - 2) (the last one shorter), then
+// [nz - 2, nz); otherwise chunks of zc from 0.  Every chunk that is neither
+// first nor last starts on an even plane.
+LSAMD_HD int p1_nzchunks(int nz, int zc) {
+  return nz > 7 ? 2 + (int)((nz - 4 + zc - 1) / zc) : (nz + zc - 1) / zc;
+}
+LSAMD_HD void p1_zchunk(int nz, int zc, int bz, int* z0, int* z1) {
+  if (nz <= 7) {
+    *z0 = bz * zc;
+    *z1 = min(*z0 + zc, nz);
+    return;
+  }
+  const int last = p1_nzchunks(nz, zc) - 1;
+  if (bz == 0) {
+    *z0 = 0;
+    *z1 = 2;
+  } else if (bz == last) {
+    *z0 = nz - 2;
+    *z1 = nz;
+  } else {
+    *z0 = 2 + (bz - 1) * zc;
+    *z1 = min(*z0 + zc, nz - 2);
+  }
 }
 
 template <typename T, int TXV, int ZC>
@@ -695,7 +722,8 @@ __global__ __launch_bounds__(512) void k_p1_tiles(const T* __restrict__ x, T* __
   const int bz = (int)(blockIdx.x % (unsigned)nzc);
   const int tt = (int)(blockIdx.x / (unsigned)nzc);
   const int x0 = (tt % ntx) * TXV, y0 = (tt / ntx) * TYV;
-  const int z0 = bz * ZC, z1 = min(z0 + ZC, g.nz);
+  int z0, z1;
+  p1_zchunk(g.nz, ZC, bz, &z0, &z1);
   // outputs at least two nodes from both z faces, on a z axis long enough
   // for the parity rule (virtual z 2 and 3 exist)
   const bool zin = nvz > 3 && g.nz > 7 && z0 >= 2 && z1 <= g.nz - 2;
@@ -1130,6 +1158,8 @@ struct lsamd_fd_system {
   // multi-rhs multigrid work per group width 4/8/16/32 (kept across set_rows)
   lsamd::MrWork* mr[4] = {nullptr, nullptr, nullptr, nullptr};
   void* tzero = nullptr;  // zeroed control blocks of the timing entry points
+  // buffers of the row set (set_rows), kept across calls: (pointer, bytes)
+  std::vector<std::pair<void*, size_t>> rpool;
   lsamd_grid grid;
   GridDev gd;
   lsamd_fd_reg reg;
@@ -1254,6 +1284,32 @@ void dfree(lsamd_fd_system* s, void* p) {
     }
 }
 
+// Buffer `slot` of the row set: kept across set_rows calls and replaced by a
+// larger one (25% headroom) only when the new row set does not fit, so that a
+// handle reused for many row sets of similar size stops allocating.
+enum RowSlot {
+  kRsRowPtr, kRsCols, kRsCoef64, kRsCoef32, kRsB, kRsW, kRsW2_64, kRsW2_32, kRsT64, kRsT32,
+  kRsPartRows, kRsURows, kRsBad, kRsErow, kRsKeys, kRsVals, kRsSegEntry, kRsSegRow, kRsUniq,
+  kRsCounts, kRsNrun, kRsTmp, kRsOff32, kRsSegPtr, kRsDnode, kRsSlots
+};
+
+template <typename P>
+int rbuf(lsamd_fd_system* s, int slot, P** p, int64_t count) {
+  if (s->rpool.empty()) s->rpool.assign(kRsSlots, {nullptr, 0});
+  const size_t bytes = std::max<size_t>(8, (size_t)std::max<int64_t>(count, 1) * sizeof(P));
+  auto& e = s->rpool[slot];
+  if (e.second < bytes) {
+    dfree(s, e.first);
+    e = {nullptr, 0};
+    void* q = nullptr;
+    const size_t want = bytes + bytes / 4;
+    if (int rc = dmalloc(s, &q, want)) return rc;
+    e = {q, want};
+  }
+  *p = (P*)e.first;
+  return LSAMD_OK;
+}
+
 RowsDev rows_dev(const lsamd_fd_system* s, bool fp32) {
   RowsDev R;
   R.nrows = s->nrows;
@@ -1310,7 +1366,8 @@ int zc_index(int zc) {
 // P1 plane-streaming stencil configurations (tile width, z-chunk); the
 // stencil's `zc` slot of a P1 system holds 1 + the configuration index, or
 // kP1Node for the one-node-per-thread kernel (LSAMD_P1_STENCIL=node).
-constexpr int kP1Cfg[4][2] = {{64, 8}, {64, 16}, {32, 8}, {32, 16}};
+constexpr int kP1NCfg = 6;
+constexpr int kP1Cfg[kP1NCfg][2] = {{64, 8}, {64, 16}, {32, 8}, {32, 16}, {64, 32}, {32, 32}};
 constexpr int kP1Node = 100;
 constexpr int kP1Default = 2;  // (64, 16) before tuning
 
@@ -1318,7 +1375,7 @@ void p1_tiling(const lsamd_fd_system* s, int cfg, int* ntx, int* nty, int* nzc) 
   const int tx = kP1Cfg[cfg][0], zc = kP1Cfg[cfg][1];
   *ntx = (int)ceil_div(s->gd.nx, tx);
   *nty = (int)ceil_div(s->gd.ny, 512 / tx);
-  *nzc = (int)ceil_div(s->gd.nz, zc);
+  *nzc = p1_nzchunks(s->gd.nz, zc);
 }
 
 int64_t p1_blocks(const lsamd_fd_system* s, int cfg) {
@@ -1330,7 +1387,7 @@ int64_t p1_blocks(const lsamd_fd_system* s, int cfg) {
 // largest partial count over the P1 stencil forms (allocation)
 int64_t p1_max_partials(const lsamd_fd_system* s) {
   int64_t m = ceil_div(s->n, 256);
-  for (int c = 0; c < 4; ++c) m = std::max(m, p1_blocks(s, c));
+  for (int c = 0; c < kP1NCfg; ++c) m = std::max(m, p1_blocks(s, c));
   return m;
 }
 
@@ -1338,7 +1395,7 @@ int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
   if (s->op == 2) return std::max(s->nblk, s->nbtile);
   if (s->op == 1) {
     if (zc == kP1Node) return ceil_div(s->n, 256);
-    return p1_blocks(s, (zc >= 1 && zc <= 4) ? zc - 1 : kP1Default - 1);
+    return p1_blocks(s, (zc >= 1 && zc <= kP1NCfg) ? zc - 1 : kP1Default - 1);
   }
   if (!s->regular) return ceil_div(s->n, 256);
   return s->nstile[zc_index(zc)];
@@ -1382,7 +1439,7 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
                          x, y, s->gd, tab, s->p1nv[0], s->p1nv[1], (T)s->p1_ridge2, s->part, ctl);
       return ceil_div(s->n, 256);
     }
-    const int cfg = (zc >= 1 && zc <= 4) ? zc - 1 : kP1Default - 1;
+    const int cfg = (zc >= 1 && zc <= kP1NCfg) ? zc - 1 : kP1Default - 1;
     int ntx, nty, nzc;
     p1_tiling(s, cfg, &ntx, &nty, &nzc);
     const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
@@ -1401,8 +1458,16 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
         hipLaunchKernelGGL((k_p1_tiles<T, 32, 8>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
                            nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
         break;
-      default:
+      case 3:
         hipLaunchKernelGGL((k_p1_tiles<T, 32, 16>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+      case 4:
+        hipLaunchKernelGGL((k_p1_tiles<T, 64, 32>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
+                           nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
+        break;
+      default:
+        hipLaunchKernelGGL((k_p1_tiles<T, 32, 32>), dim3(nb), dim3(512), 0, st, x, y, s->gd, tab,
                            nv[0], nv[1], nv[2], r2, ntx, nzc, s->part, ctl);
         break;
     }
@@ -1441,7 +1506,7 @@ int tune_p1(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
   float best = 1e30f;
   int bestc = kP1Default;
   for (int round = 0; round < 3; ++round)
-    for (int c = 1; c <= 4; ++c) {
+    for (int c = 1; c <= kP1NCfg; ++c) {
       launch_stencil<T>(s, x, y, s->ctl_apply, st, c);
       LSAMD_HIP(hipEventRecord(e0, st));
       for (int r = 0; r < 5; ++r) launch_stencil<T>(s, x, y, s->ctl_apply, st, c);
@@ -3577,10 +3642,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
       s->bgraph[i] = nullptr;
     }
   }
-  void* olds[] = {s->row_ptr, s->cols,  s->coef64,   s->coef32,  s->b,     s->w,
-                  s->w2_64,   s->w2_32, s->seg_ptr,  s->seg_entry, s->seg_row, s->dnode,
-                  s->t64,     s->t32,   s->part_rows, s->u_rows};
-  for (void* p : olds) dfree(s, p);
+  // the previous row set's buffers stay in s->rpool for this one
   s->u_rows = nullptr;
   s->row_ptr = nullptr;
   s->cols = nullptr;
@@ -3603,18 +3665,18 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
   launch_reg_diag(s, st);
   hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s->rhs, n);
   if (nrows > 0) {
-    if (int rc = dalloc(s, &s->row_ptr, nrows + 1)) return rc;
-    if (int rc = dalloc(s, &s->cols, nnz)) return rc;
-    if (int rc = dalloc(s, &s->coef64, nnz)) return rc;
-    if (int rc = dalloc(s, &s->coef32, nnz)) return rc;
-    if (int rc = dalloc(s, &s->b, nrows)) return rc;
-    if (int rc = dalloc(s, &s->w, nrows)) return rc;
-    if (int rc = dalloc(s, &s->w2_64, nrows)) return rc;
-    if (int rc = dalloc(s, &s->w2_32, nrows)) return rc;
-    if (int rc = dalloc(s, &s->t64, nrows)) return rc;
-    if (int rc = dalloc(s, &s->t32, nrows)) return rc;
-    if (int rc = dalloc(s, &s->part_rows, ceil_div(nrows, 256))) return rc;
-    if (int rc = dalloc(s, &s->u_rows, nrows)) return rc;
+    if (int rc = rbuf(s, kRsRowPtr, &s->row_ptr, nrows + 1)) return rc;
+    if (int rc = rbuf(s, kRsCols, &s->cols, nnz)) return rc;
+    if (int rc = rbuf(s, kRsCoef64, &s->coef64, nnz)) return rc;
+    if (int rc = rbuf(s, kRsCoef32, &s->coef32, nnz)) return rc;
+    if (int rc = rbuf(s, kRsB, &s->b, nrows)) return rc;
+    if (int rc = rbuf(s, kRsW, &s->w, nrows)) return rc;
+    if (int rc = rbuf(s, kRsW2_64, &s->w2_64, nrows)) return rc;
+    if (int rc = rbuf(s, kRsW2_32, &s->w2_32, nrows)) return rc;
+    if (int rc = rbuf(s, kRsT64, &s->t64, nrows)) return rc;
+    if (int rc = rbuf(s, kRsT32, &s->t32, nrows)) return rc;
+    if (int rc = rbuf(s, kRsPartRows, &s->part_rows, ceil_div(nrows, 256))) return rc;
+    if (int rc = rbuf(s, kRsURows, &s->u_rows, nrows)) return rc;
     LSAMD_HIP(hipMemcpyAsync(s->row_ptr, row_ptr, (nrows + 1) * 8, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->cols, cols, nnz * 4, hipMemcpyDefault, st));
     LSAMD_HIP(hipMemcpyAsync(s->coef64, coef, nnz * 8, hipMemcpyDefault, st));
@@ -3622,14 +3684,13 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     LSAMD_HIP(hipMemcpyAsync(s->w, w, nrows * 8, hipMemcpyDefault, st));
     {
       int* bad = nullptr;
-      if (int rc = dalloc(s, &bad, 1)) return rc;
+      if (int rc = rbuf(s, kRsBad, &bad, 1)) return rc;
       LSAMD_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
       hipLaunchKernelGGL(k_check_cols, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, st,
                          (const int32_t*)s->cols, nnz, (int64_t)0, s->n, bad);
       int hb = 0;
       LSAMD_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, st));
       LSAMD_HIP(hipStreamSynchronize(st));
-      dfree(s, bad);
       if (hb) {
         s->nrows = s->nnz = 0;
         return fail(LSAMD_ERR_ARG, "lsamd_fd_system_set_rows: a column id is outside the grid");
@@ -3643,14 +3704,14 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     int32_t* uniq = nullptr;
     int32_t* counts = nullptr;
     int32_t* nrun = nullptr;
-    if (int rc = dalloc(s, &erow, nnz)) return rc;
-    if (int rc = dalloc(s, &keys_out, nnz)) return rc;
-    if (int rc = dalloc(s, &vals_in, nnz)) return rc;
-    if (int rc = dalloc(s, &s->seg_entry, nnz)) return rc;
-    if (int rc = dalloc(s, &s->seg_row, nnz)) return rc;
-    if (int rc = dalloc(s, &uniq, nnz)) return rc;
-    if (int rc = dalloc(s, &counts, nnz)) return rc;
-    if (int rc = dalloc(s, &nrun, 1)) return rc;
+    if (int rc = rbuf(s, kRsErow, &erow, nnz)) return rc;
+    if (int rc = rbuf(s, kRsKeys, &keys_out, nnz)) return rc;
+    if (int rc = rbuf(s, kRsVals, &vals_in, nnz)) return rc;
+    if (int rc = rbuf(s, kRsSegEntry, &s->seg_entry, nnz)) return rc;
+    if (int rc = rbuf(s, kRsSegRow, &s->seg_row, nnz)) return rc;
+    if (int rc = rbuf(s, kRsUniq, &uniq, nnz)) return rc;
+    if (int rc = rbuf(s, kRsCounts, &counts, nnz)) return rc;
+    if (int rc = rbuf(s, kRsNrun, &nrun, 1)) return rc;
     hipLaunchKernelGGL(k_entry_rows, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, st,
                        s->row_ptr, nrows, erow);
     hipLaunchKernelGGL(k_iota, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, st, vals_in, nnz);
@@ -3666,7 +3727,7 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     size_t tb3 = 0;
     LSAMD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, counts, s->seg_row, (int)nnz, st));
     const size_t tbytes = std::max(tb, std::max(tb2, tb3));
-    if (int rc = dmalloc(s, &tmp, tbytes)) return rc;
+    if (int rc = rbuf(s, kRsTmp, (unsigned char**)&tmp, (int64_t)tbytes)) return rc;
     tb = tbytes;
     LSAMD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, s->cols, keys_out, vals_in,
                                                  s->seg_entry, (int)nnz, 0, bits, st));
@@ -3677,11 +3738,11 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
     LSAMD_HIP(hipMemcpyAsync(&nr, nrun, 4, hipMemcpyDeviceToHost, st));
     LSAMD_HIP(hipStreamSynchronize(st));
     s->ndn = nr;
-    if (int rc = dalloc(s, &s->seg_ptr, nr + 1)) return rc;
-    if (int rc = dalloc(s, &s->dnode, nr)) return rc;
+    if (int rc = rbuf(s, kRsSegPtr, &s->seg_ptr, nr + 1)) return rc;
+    if (int rc = rbuf(s, kRsDnode, &s->dnode, nr)) return rc;
     // segment offsets (int32 scan, then widened)
     int32_t* off32 = nullptr;
-    if (int rc = dalloc(s, &off32, nr + 1)) return rc;
+    if (int rc = rbuf(s, kRsOff32, &off32, nr + 1)) return rc;
     tb = tbytes;
     LSAMD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, off32, (int)nr, st));
     std::vector<int32_t> hoff(nr + 1);
@@ -3695,8 +3756,6 @@ int lsamd_fd_system_set_rows(lsamd_fd_system* s, int64_t nrows, int64_t nnz,
                        s->seg_entry, s->seg_row, nnz);
     LSAMD_CHECK_LAUNCH("transpose");
     LSAMD_HIP(hipStreamSynchronize(st));
-    void* tmps[] = {erow, keys_out, vals_in, uniq, counts, nrun, tmp, off32};
-    for (void* p : tmps) dfree(s, p);
     // rhs = A^T W^2 b, data diagonal
     RowsDev R = rows_dev(s, false);
     hipLaunchKernelGGL(k_w2b, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, st, s->w2_64,
