@@ -266,6 +266,18 @@ int lsamd_fd_nearest_d2(const lsamd_grid* g, const double* pts, int64_t m, doubl
 int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
                               const double* gv, double* out, void* stream);
 
+/* Allocation-free variant of lsamd_fd_evaluate_adjoint (same output, bit for
+ * bit).  No reference counterpart: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree and never synchronises:
+ * its entry arrays and sort storage live in one static 24 MiB scratch array
+ * per GPU, held until the process ends.  At most LSAMD_NOALLOC_MAX_POINTS
+ * points (LSAMD_ERR_ARG above that).  Calls on one GPU share the array, so
+ * calls that overlap on one GPU (other streams, other threads) race. */
+#define LSAMD_NOALLOC_MAX_POINTS 65536
+int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, int64_t n,
+                                      const double* gv, double* out, void* stream);
+
 /* A20: per data row r of the system (set_rows order), with u = lambda . a_r
  * and the row's normalised coefficients a_r, rhs b_r, weight w_r:
  *   g_b[r] = w_r^2 u            (d loss / d b_hat_r)

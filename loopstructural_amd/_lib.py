@@ -64,6 +64,9 @@ class LsmrInfo(C.Structure):
                 ("condA", C.c_double), ("normx", C.c_double)]
 
 
+# largest point count of lsamd_fd_evaluate_adjoint_noalloc
+NOALLOC_MAX_POINTS = 65536
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _I32 = C.c_int32
@@ -91,6 +94,7 @@ SIGNATURES = {
     "lsamd_rows_normalise": [_I64, _I32, _P, _P, _P, _P],
     "lsamd_fd_nearest_d2": [_P, _P, _I64, _P, _P],
     "lsamd_fd_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
+    "lsamd_fd_evaluate_adjoint_noalloc": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
     "lsamd_fd_system_time_multi": [_P, _I32, _I32, _P, _P, _P],
     "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],

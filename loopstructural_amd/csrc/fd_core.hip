@@ -3500,6 +3500,38 @@ int ls_solve(lsamd_fd_system* s, const lsamd_lsmr_opts* o, double* c, lsamd_lsmr
   return LSAMD_OK;
 }
 
+// Scratch of lsamd_fd_evaluate_adjoint_noalloc: the per-entry arrays of up to
+// LSAMD_NOALLOC_MAX_POINTS points (24 bytes per entry, 8 entries per point:
+// 12 MiB) and the radix sort's temporary storage.  One copy per GPU, held
+// until the process ends.  Host code never passes the name to a kernel: it
+// takes the device address from hipGetSymbolAddress, per device.
+constexpr size_t kScratchBytes = (size_t)24 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of the current device's scratch, looked up once per device.
+int scratch_base(unsigned char** out) {
+  int dev = 0;
+  LSAMD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LSAMD_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LSAMD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+    g_scratch_addr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_addr[dev];
+  return LSAMD_OK;
+}
+
 }  // namespace lsamd
 
 extern "C" {
@@ -4104,6 +4136,61 @@ int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
   hipStreamSynchronize(st);
   cleanup();
   if (e != hipSuccess || e2 != hipSuccess) return fail(LSAMD_ERR_HIP, "evaluate_adjoint failed");
+  return LSAMD_OK;
+}
+
+int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, int64_t n,
+                                      const double* gv, double* out, void* stream) {
+  // lsamd_fd_evaluate_adjoint with its per-entry arrays and sort storage
+  // carved from the device's scratch array instead of allocated: no
+  // hipMalloc/hipFree and no stream synchronisation.  Every buffer taken from
+  // the scratch is written in full before it is read (the entries by
+  // k_eval_entries, the sorted copies and the sort's own storage by the
+  // radix sort), so nothing left by an earlier call is seen.
+  if (int rc = injected("evaluate_adjoint_noalloc")) return rc;
+  if (int rc = check_grid(g)) return rc;
+  if (n < 0 || !out || (n > 0 && (!pts || !gv)))
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_noalloc: bad arguments");
+  if (n > LSAMD_NOALLOC_MAX_POINTS)
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_noalloc: " + std::to_string(n) +
+                                   " points exceed the scratch capacity of " +
+                                   std::to_string(LSAMD_NOALLOC_MAX_POINTS));
+  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  if (nn > 2147483647LL) return fail(LSAMD_ERR_ARG, "grid exceeds int32 node ids");
+  unsigned char* base = nullptr;
+  if (int rc = scratch_base(&base)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, out, nn);
+  LSAMD_CHECK_LAUNCH("evaluate_adjoint_noalloc zero");
+  if (n == 0) return LSAMD_OK;
+  const int64_t ne = 8 * n;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base;
+    base += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  unsigned char* const start = base;
+  int32_t* key = (int32_t*)take(ne * 4);
+  int32_t* key_s = (int32_t*)take(ne * 4);
+  int32_t* idx = (int32_t*)take(ne * 4);
+  int32_t* idx_s = (int32_t*)take(ne * 4);
+  double* val = (double*)take(ne * 8);
+  const size_t room = kScratchBytes - (size_t)(base - start);
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
+  size_t tb = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
+                                         st) != hipSuccess)
+    return fail(LSAMD_ERR_HIP, "evaluate_adjoint_noalloc: sort size query failed");
+  if (tb > room)
+    return fail(LSAMD_ERR_ARG, "evaluate_adjoint_noalloc: sort storage exceeds the scratch");
+  launch_eval_entries(to_dev(*g), pts, gv, n, key, idx, val, st);
+  if (hipcub::DeviceRadixSort::SortPairs(base, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
+                                         st) != hipSuccess)
+    return fail(LSAMD_ERR_HIP, "evaluate_adjoint_noalloc: sort failed");
+  hipLaunchKernelGGL(k_eval_segsum, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, key_s,
+                     idx_s, val, ne, out);
+  LSAMD_CHECK_LAUNCH("evaluate_adjoint_noalloc");
   return LSAMD_OK;
 }
 
