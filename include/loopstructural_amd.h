@@ -278,6 +278,18 @@ int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
 int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, int64_t n,
                                       const double* gv, double* out, void* stream);
 
+/* A20 with caller storage: lsamd_fd_evaluate_adjoint (same output, bit for
+ * bit) with its temporaries in the device workspace `ws` (256-byte aligned,
+ * at least lsamd_fd_evaluate_adjoint_ws_bytes(g, n) bytes; LSAMD_ERR_ARG if
+ * smaller).  No allocation and no synchronisation, so a caller that keeps one
+ * workspace per stream (the autograd adjoint does) runs backward passes
+ * without allocating.  Replaces the same reference interface as
+ * lsamd_fd_evaluate_adjoint. */
+int64_t lsamd_fd_evaluate_adjoint_ws_bytes(const lsamd_grid* g, int64_t n);
+int lsamd_fd_evaluate_adjoint_ws(const lsamd_grid* g, const double* pts, int64_t n,
+                                 const double* gv, double* out, void* ws, int64_t ws_bytes,
+                                 void* stream);
+
 /* A20: per data row r of the system (set_rows order), with u = lambda . a_r
  * and the row's normalised coefficients a_r, rhs b_r, weight w_r:
  *   g_b[r] = w_r^2 u            (d loss / d b_hat_r)

@@ -18,8 +18,6 @@ stored CG iterates: only the converged ``c`` is kept.
 """
 from __future__ import annotations
 
-import ctypes as C
-
 import numpy as np
 import torch
 
@@ -126,8 +124,4 @@ class FDEvaluateFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gv):
         (pts,) = ctx.saved_tensors
-        out = torch.empty(ctx.n, dtype=F64, device=pts.device)
-        _lib.call("lsamd_fd_evaluate_adjoint", C.byref(ctx.grid), _lib.ptr(pts.contiguous()),
-                  int(pts.shape[0]), _lib.ptr(gv.detach().to(F64).contiguous()), _lib.ptr(out),
-                  _lib.stream())
-        return out, None, None
+        return fs.evaluate_adjoint(ctx.grid, pts, gv.detach(), ctx.n), None, None

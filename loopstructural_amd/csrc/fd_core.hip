@@ -3532,6 +3532,70 @@ int scratch_base(unsigned char** out) {
   return LSAMD_OK;
 }
 
+// Bytes of caller storage eval_adjoint_into needs for n points on grid g:
+// the per-entry arrays (8 entries per point: two int32 keys, two int32 ids,
+// a double) at 256-byte alignment, and the radix sort's temporary storage.
+size_t eval_adjoint_bytes(const lsamd_grid* g, int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t ne = 8 * n;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
+  size_t tb = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (int32_t*)nullptr, (int32_t*)nullptr,
+                                     (int32_t*)nullptr, (int32_t*)nullptr, (int)ne, 0, bits,
+                                     (hipStream_t)0);
+  return 4 * up(ne * 4) + up(ne * 8) + up(tb);
+}
+
+// out = E^T gv (the adjoint of trilinear evaluation) with every temporary in
+// the caller's storage [base, base + room): no allocation, no synchronisation.
+// Every buffer taken from the storage is written in full before it is read
+// (the entries by k_eval_entries, the sorted copies and the sort's own
+// storage by the radix sort), so earlier contents are never seen.
+int eval_adjoint_into(const lsamd_grid* g, const double* pts, int64_t n, const double* gv,
+                      double* out, unsigned char* base, size_t room, hipStream_t st,
+                      const char* fn) {
+  if (int rc = check_grid(g)) return rc;
+  if (n < 0 || !out || (n > 0 && (!pts || !gv)))
+    return fail(LSAMD_ERR_ARG, std::string(fn) + ": bad arguments");
+  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
+  if (nn > 2147483647LL) return fail(LSAMD_ERR_ARG, "grid exceeds int32 node ids");
+  if (8 * n >= 2147483647LL) return fail(LSAMD_ERR_ARG, std::string(fn) + ": too many points");
+  if (eval_adjoint_bytes(g, n) > room)
+    return fail(LSAMD_ERR_ARG, std::string(fn) + ": workspace smaller than " +
+                                   std::to_string(eval_adjoint_bytes(g, n)) + " bytes");
+  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, out, nn);
+  LSAMD_CHECK_LAUNCH("evaluate_adjoint zero");
+  if (n == 0) return LSAMD_OK;
+  const int64_t ne = 8 * n;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base;
+    base += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  int32_t* key = (int32_t*)take(ne * 4);
+  int32_t* key_s = (int32_t*)take(ne * 4);
+  int32_t* idx = (int32_t*)take(ne * 4);
+  int32_t* idx_s = (int32_t*)take(ne * 4);
+  double* val = (double*)take(ne * 8);
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
+  size_t tb = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
+                                         st) != hipSuccess)
+    return fail(LSAMD_ERR_HIP, std::string(fn) + ": sort size query failed");
+  launch_eval_entries(to_dev(*g), pts, gv, n, key, idx, val, st);
+  if (hipcub::DeviceRadixSort::SortPairs(base, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
+                                         st) != hipSuccess)
+    return fail(LSAMD_ERR_HIP, std::string(fn) + ": sort failed");
+  hipLaunchKernelGGL(k_eval_segsum, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, key_s,
+                     idx_s, val, ne, out);
+  LSAMD_CHECK_LAUNCH(fn);
+  return LSAMD_OK;
+}
+
 }  // namespace lsamd
 
 extern "C" {
@@ -4093,105 +4157,53 @@ int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
                               double* out, void* stream) {
   // out = E^T gv with E the trilinear evaluation matrix of the points (outside
   // points contribute nothing).  Deterministic: entries are sorted by node and
-  // summed in point order.
+  // summed in point order.  One temporary allocation, freed before return.
   if (int rc = injected("evaluate_adjoint")) return rc;
   if (int rc = check_grid(g)) return rc;
-  if (n < 0 || !out || (n > 0 && (!pts || !gv)))
-    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint: bad arguments");
+  if (n < 0) return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint: bad arguments");
+  const size_t bytes = eval_adjoint_bytes(g, n);
+  void* ws = nullptr;
+  if (bytes > 0 && hipMalloc(&ws, bytes) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "evaluate_adjoint: hipMalloc failed");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
-  if (nn > 2147483647LL) return fail(LSAMD_ERR_ARG, "grid exceeds int32 node ids");
-  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, out, nn);
-  if (n == 0) return LSAMD_OK;
-  const int64_t ne = 8 * n;
-  if (ne >= 2147483647LL) return fail(LSAMD_ERR_ARG, "too many points");
-  int32_t *key = nullptr, *key_s = nullptr, *idx = nullptr, *idx_s = nullptr;
-  double* val = nullptr;
-  void* tmp = nullptr;
-  auto cleanup = [&]() {
-    for (void* p : {(void*)key, (void*)key_s, (void*)idx, (void*)idx_s, (void*)val, tmp})
-      if (p) hipFree(p);
-  };
-  if (hipMalloc(&key, ne * 4) != hipSuccess || hipMalloc(&key_s, ne * 4) != hipSuccess ||
-      hipMalloc(&idx, ne * 4) != hipSuccess || hipMalloc(&idx_s, ne * 4) != hipSuccess ||
-      hipMalloc(&val, ne * 8) != hipSuccess) {
-    cleanup();
-    return fail(LSAMD_ERR_ALLOC, "evaluate_adjoint: hipMalloc failed");
+  const int rc = eval_adjoint_into(g, pts, n, gv, out, (unsigned char*)ws, bytes, st,
+                                   "lsamd_fd_evaluate_adjoint");
+  if (ws) {
+    hipStreamSynchronize(st);
+    hipFree(ws);
   }
-  launch_eval_entries(to_dev(*g), pts, gv, n, key, idx, val, st);
-  int bits = 1;
-  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
-  size_t tb = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)ne, 0, bits, st);
-  if (hipMalloc(&tmp, tb + 8) != hipSuccess) {
-    cleanup();
-    return fail(LSAMD_ERR_ALLOC, "evaluate_adjoint: hipMalloc failed");
-  }
-  hipError_t e =
-      hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key_s, idx, idx_s, (int)ne, 0, bits, st);
-  if (e == hipSuccess)
-    hipLaunchKernelGGL(k_eval_segsum, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, key_s,
-                       idx_s, val, ne, out);
-  hipError_t e2 = hipGetLastError();
-  hipStreamSynchronize(st);
-  cleanup();
-  if (e != hipSuccess || e2 != hipSuccess) return fail(LSAMD_ERR_HIP, "evaluate_adjoint failed");
-  return LSAMD_OK;
+  return rc;
 }
 
 int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, int64_t n,
                                       const double* gv, double* out, void* stream) {
   // lsamd_fd_evaluate_adjoint with its per-entry arrays and sort storage
   // carved from the device's scratch array instead of allocated: no
-  // hipMalloc/hipFree and no stream synchronisation.  Every buffer taken from
-  // the scratch is written in full before it is read (the entries by
-  // k_eval_entries, the sorted copies and the sort's own storage by the
-  // radix sort), so nothing left by an earlier call is seen.
+  // hipMalloc/hipFree and no stream synchronisation.
   if (int rc = injected("evaluate_adjoint_noalloc")) return rc;
-  if (int rc = check_grid(g)) return rc;
-  if (n < 0 || !out || (n > 0 && (!pts || !gv)))
-    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_noalloc: bad arguments");
   if (n > LSAMD_NOALLOC_MAX_POINTS)
     return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_noalloc: " + std::to_string(n) +
                                    " points exceed the scratch capacity of " +
                                    std::to_string(LSAMD_NOALLOC_MAX_POINTS));
-  const int64_t nn = g->nnodes[0] * g->nnodes[1] * g->nnodes[2];
-  if (nn > 2147483647LL) return fail(LSAMD_ERR_ARG, "grid exceeds int32 node ids");
   unsigned char* base = nullptr;
   if (int rc = scratch_base(&base)) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_d, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, out, nn);
-  LSAMD_CHECK_LAUNCH("evaluate_adjoint_noalloc zero");
-  if (n == 0) return LSAMD_OK;
-  const int64_t ne = 8 * n;
-  auto take = [&](size_t bytes) {
-    unsigned char* p = base;
-    base += (bytes + 255) & ~(size_t)255;
-    return p;
-  };
-  unsigned char* const start = base;
-  int32_t* key = (int32_t*)take(ne * 4);
-  int32_t* key_s = (int32_t*)take(ne * 4);
-  int32_t* idx = (int32_t*)take(ne * 4);
-  int32_t* idx_s = (int32_t*)take(ne * 4);
-  double* val = (double*)take(ne * 8);
-  const size_t room = kScratchBytes - (size_t)(base - start);
-  int bits = 1;
-  while (bits < 31 && ((int64_t)1 << bits) <= nn) ++bits;
-  size_t tb = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
-                                         st) != hipSuccess)
-    return fail(LSAMD_ERR_HIP, "evaluate_adjoint_noalloc: sort size query failed");
-  if (tb > room)
-    return fail(LSAMD_ERR_ARG, "evaluate_adjoint_noalloc: sort storage exceeds the scratch");
-  launch_eval_entries(to_dev(*g), pts, gv, n, key, idx, val, st);
-  if (hipcub::DeviceRadixSort::SortPairs(base, tb, key, key_s, idx, idx_s, (int)ne, 0, bits,
-                                         st) != hipSuccess)
-    return fail(LSAMD_ERR_HIP, "evaluate_adjoint_noalloc: sort failed");
-  hipLaunchKernelGGL(k_eval_segsum, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, st, key_s,
-                     idx_s, val, ne, out);
-  LSAMD_CHECK_LAUNCH("evaluate_adjoint_noalloc");
-  return LSAMD_OK;
+  return eval_adjoint_into(g, pts, n, gv, out, base, kScratchBytes, (hipStream_t)stream,
+                           "lsamd_fd_evaluate_adjoint_noalloc");
+}
+
+int64_t lsamd_fd_evaluate_adjoint_ws_bytes(const lsamd_grid* g, int64_t n) {
+  if (!g || n < 0) return -1;
+  return (int64_t)eval_adjoint_bytes(g, n);
+}
+
+int lsamd_fd_evaluate_adjoint_ws(const lsamd_grid* g, const double* pts, int64_t n,
+                                 const double* gv, double* out, void* ws, int64_t ws_bytes,
+                                 void* stream) {
+  if (int rc = injected("evaluate_adjoint_ws")) return rc;
+  if (n > 0 && (!ws || ws_bytes < 0 || ((uintptr_t)ws & 255)))
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_ws: workspace NULL or not 256-aligned");
+  return eval_adjoint_into(g, pts, n, gv, out, (unsigned char*)ws, (size_t)ws_bytes,
+                           (hipStream_t)stream, "lsamd_fd_evaluate_adjoint_ws");
 }
 
 int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,

@@ -320,6 +320,24 @@ def evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True, g
     return v, g
 
 
+def evaluate_adjoint(grid: _lib.Grid, pts: torch.Tensor, gv: torch.Tensor, nnodes: int):
+    """A20: E^T gv (E the trilinear evaluation matrix of pts), bit-identical to
+    lsamd_fd_evaluate_adjoint, with its temporaries in a workspace from the
+    caching allocator on the current stream: no device allocation or stream
+    synchronisation once the allocator holds a block of that size."""
+    pts = pts.to(_lib.device(), F64).contiguous()
+    gv = gv.to(pts.device, F64).contiguous()
+    n = int(pts.shape[0])
+    out = torch.empty(int(nnodes), dtype=F64, device=pts.device)
+    nb = int(_lib.load().lsamd_fd_evaluate_adjoint_ws_bytes(C.byref(grid), n))
+    if nb < 0:
+        raise _lib.LsamdError(1, "lsamd_fd_evaluate_adjoint_ws_bytes: bad arguments")
+    ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=pts.device)
+    _lib.call("lsamd_fd_evaluate_adjoint_ws", C.byref(grid), _lib.ptr(pts), n, _lib.ptr(gv),
+              _lib.ptr(out), _lib.ptr(ws), int(ws.numel()), _lib.stream())
+    return out
+
+
 # ------------------------------------------------------------------- P1 ----
 class P1System(FdSystem):
     """``lsamd_fd_system`` handle whose regulariser is the P1 edge-jump
