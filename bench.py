@@ -394,13 +394,29 @@ def _timed(torch, fn, reps=3):
 
 
 def fd_evaluation(torch, _lib, fs, grid, c, n):
+    """Values + gradients at 1e7 uniform random points (each point gathers 8
+    corner coefficients from scattered cache lines), and at the 216^3 points
+    of a linspace grid in the 'ij' order the reference evaluates on."""
     npts = 10_000_000
     g = torch.Generator(device="cuda").manual_seed(7)
     pts = torch.rand((npts, 3), dtype=torch.float64, device="cuda", generator=g)
     dt, _ = _timed(torch, lambda: fs.evaluate(grid, c, pts, True, True))
     bytes_alg = npts * (24 + 8 + 24) + 8 * n
     res = {"points": npts, "ms": round(dt * 1e3, 2), "Mpts_per_s": round(npts / dt / 1e6, 1),
-           "GBs": round(bytes_alg / dt / 1e9, 1), "what": "values + gradients"}
+           "GBs": round(bytes_alg / dt / 1e9, 1),
+           "frac": round(bytes_alg / dt / 1e9 / HBM_PEAK_GBS, 4),
+           "what": "values + gradients, uniform random points"}
+    t = torch.linspace(0.0, 1.0, 216, dtype=torch.float64, device="cuda")
+    xx, yy, zz = torch.meshgrid(t, t, t, indexing="ij")
+    grid_pts = torch.stack([xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)], 1).contiguous()
+    del xx, yy, zz
+    m = grid_pts.shape[0]
+    dt2, _ = _timed(torch, lambda: fs.evaluate(grid, c, grid_pts, True, True))
+    b2 = m * (24 + 8 + 24) + 8 * n
+    res["ordered"] = {"points": m, "ms": round(dt2 * 1e3, 2),
+                      "Mpts_per_s": round(m / dt2 / 1e6, 1), "GBs": round(b2 / dt2 / 1e9, 1),
+                      "frac": round(b2 / dt2 / 1e9 / HBM_PEAK_GBS, 4),
+                      "what": "values + gradients, 216^3 linspace points in 'ij' order"}
     log(f"FD evaluation: {res}")
     return res
 
@@ -444,7 +460,7 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
     log(f"ensemble (batched Jacobi PCG): {dt:.2f} s")
     # the same ensemble by per-system multigrid, shape groups on several streams
     blocks = [fd_rows(fs, _lib, g, v, m, k) for g, (v, m), k in zip(grids, vd, ns)]
-    es = fs.EnsembleSolver(grids, reg, workers=4)
+    es = fs.EnsembleSolver(grids, reg, workers=12)
     t0 = time.perf_counter()
     es.prepare(blocks, args.rtol)
     t_prep = time.perf_counter() - t0
@@ -453,14 +469,32 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
     cm, infm = es.solve(blocks, rtol=args.rtol)
     dt_mg = time.perf_counter() - t0
     log(f"ensemble multigrid solves: {dt_mg:.2f} s")
-    errs_mg = []
+    es_shapes, es_timing = len(es.shapes), dict(es.last_timing)
+    es.close()
+    del es
+    # the same ensemble by batched multigrid: one hierarchy per system, every
+    # system advanced by one launch sequence per iteration
+    bm = fs.BatchedMultigrid(grids, reg, workers=12)
+    t0 = time.perf_counter()
+    bm.prepare(blocks)
+    t_bprep = time.perf_counter() - t0
+    log(f"batched multigrid set-up ({len(ens)} hierarchies): {t_bprep:.2f} s")
+    t0 = time.perf_counter()
+    cb, infb = bm.solve(blocks, rtol=args.rtol)
+    dt_b = time.perf_counter() - t0
+    bm_timing = dict(bm.last_timing)
+    log(f"batched multigrid solve: {dt_b:.2f} s ({bm_timing})")
+    errs_mg, errs_b = [], []
     for s_ in np.linspace(0, len(ens) - 1, 8).astype(int):
         single = fs.FdSystem(grids[s_], fs.reg_struct(W, (ens[s_][0],) * 3, 1e-8))
         single.set_rows(blocks[s_])
         r, _ = single.solve(rtol=1e-13, precision="fp64", check_every=256)
-        errs_mg.append(float(torch.linalg.vector_norm(cm[s_] - r) / torch.linalg.vector_norm(r)))
+        rn = torch.linalg.vector_norm(r)
+        errs_mg.append(float(torch.linalg.vector_norm(cm[s_] - r) / rn))
+        errs_b.append(float(torch.linalg.vector_norm(cb[s_] - r) / rn))
         single.close()
-    es.close()
+    bm.close()
+    del bm
     res = {"systems": len(ens), "total_dof": tot, "seconds": round(dt, 3),
            "solves_per_s": round(len(ens) / dt, 2),
            "converged": sum(i["converged"] for i in infos),
@@ -470,13 +504,25 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
            "stencil_fp32_GBs": round(tot * 8 / (sb_ms * 1e-3) / 1e9, 1),
            "stencil_frac": round(tot * 8 / (sb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "multigrid": {"seconds": round(dt_mg, 3), "solves_per_s": round(len(ens) / dt_mg, 2),
-                         "setup_seconds": round(t_prep, 3), "shapes": len(es.shapes),
+                         "setup_seconds": round(t_prep, 3), "shapes": es_shapes,
                          "converged": sum(i["converged"] for i in infm),
                          "iterations_max": max(i["iterations"] for i in infm),
                          "rel_err_vs_fp64_max_of_8": max(errs_mg),
-                         "thread_seconds": {k: round(v, 3) for k, v in es.last_timing.items()},
+                         "thread_seconds": {k: round(v, 3) for k, v in es_timing.items()},
                          "what": "per-system multigrid, one handle per grid shape (hierarchy "
-                                 "built in setup_seconds), shape groups on 4 streams"}}
+                                 "built in setup_seconds), shape groups on 12 host threads "
+                                 "with a stream each"},
+           "batched_multigrid": {
+               "seconds": round(dt_b, 3), "solves_per_s": round(len(ens) / dt_b, 2),
+               "vs_batched_jacobi": round(dt / dt_b, 2), "setup_seconds": round(t_bprep, 3),
+               "converged": sum(i["converged"] for i in infb),
+               "iterations_max": max(i["iterations"] for i in infb),
+               "rel_err_vs_fp64_max_of_8": max(errs_b),
+               "phase_seconds": {k: round(v, 3) for k, v in bm_timing.items()},
+               "what": "one hierarchy per system (grid part built in setup_seconds); timed: "
+                       "rows of every system, row-dependent hierarchy parts (12 host "
+                       "threads), and the solve of all systems by one launch sequence per "
+                       "iteration"}}
     log(f"ensemble: {res}")
     torch.cuda.empty_cache()
     return res

@@ -228,6 +228,24 @@ int lsamd_fd_system_lsmr(lsamd_fd_system* s, const lsamd_lsmr_opts* opts, double
 int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
                                  double* avg_ms, void* stream);
 
+/* Builds, or brings up to date with the current rows, the multigrid hierarchy
+ * of an FD system (what the first multigrid solve after set_rows does), so
+ * that callers can prepare many systems concurrently, one stream each,
+ * before a batched solve.  No reference counterpart. */
+int lsamd_fd_system_mg_build(lsamd_fd_system* s, void* stream);
+
+/* A12 for an uneven ensemble: the multigrid-preconditioned solve of nsys
+ * independent FD systems (distinct handles, rows set) in one launch sequence
+ * per iteration for all of them; each system keeps its own hierarchy,
+ * stopping test and iteration count (opts as lsamd_fd_system_solve with the
+ * multigrid preconditioner; precision and jacobi are ignored).  c[i]: device
+ * output of system i (its node count); infos[i]: host.  Replaces a loop of
+ * the reference's solve_system('cg') over the ensemble
+ * (LI/_discrete_interpolator.py:1221-1318, LI/_solver_strategy.py:48-71). */
+int lsamd_fd_systems_solve_mg(lsamd_fd_system* const* systems, int32_t nsys,
+                              const lsamd_solve_opts* opts, double* const* c,
+                              lsamd_solve_info* infos, void* stream);
+
 /* Times one kernel of the 32-lane multi-right-hand-side multigrid on the fine
  * level, reusing the work of this handle's last 32-lane
  * lsamd_fd_system_solve_multi group (LSAMD_ERR_ARG without one) and leaving
@@ -241,12 +259,13 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
                                double* avg_ms, double* bytes, void* stream);
 
 /* Autotuner result: form of the plane-streaming stencil chosen for the
- * precision: the z-chunk 8, 16 or 32 with planes staged through registers, or
- * 1008 / 1016 / 1032 for z-chunk 8 / 16 / 32 with planes copied straight into
- * LDS (0 = exact per-node kernel: an axis has < 7 nodes or a per-node weight
- * scale is set; also 0 before the first apply/solve), and the timed forms 8,
- * 16, 32, 1008, 1016, 1032 in microseconds (tune_us[6], nullable, -1 =
- * skipped). */
+ * precision: the z-chunk 8, 16 or 32 with planes staged through registers on
+ * 64 x 8 tiles, 1008 / 1016 / 1032 for those z-chunks with planes copied
+ * straight into LDS, or 2008 / 2016 / 2032 for register-staged planes on
+ * 32 x 16 tiles (0 = exact per-node kernel: an axis has < 7 nodes or a
+ * per-node weight scale is set; also 0 before the first apply/solve), and the
+ * timed forms in that order (8, 16, 32, 1008, 1016, 1032, 2008, 2016, 2032)
+ * in microseconds (tune_us[9], nullable, -1 = skipped). */
 int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,
                                  float* tune_us);
 
@@ -278,17 +297,13 @@ int lsamd_fd_evaluate_adjoint(const lsamd_grid* g, const double* pts, int64_t n,
 int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, int64_t n,
                                       const double* gv, double* out, void* stream);
 
-/* A20 with caller storage: lsamd_fd_evaluate_adjoint (same output, bit for
- * bit) with its temporaries in the device workspace `ws` (256-byte aligned,
- * at least lsamd_fd_evaluate_adjoint_ws_bytes(g, n) bytes; LSAMD_ERR_ARG if
- * smaller).  No allocation and no synchronisation, so a caller that keeps one
- * workspace per stream (the autograd adjoint does) runs backward passes
- * without allocating.  Replaces the same reference interface as
- * lsamd_fd_evaluate_adjoint. */
-int64_t lsamd_fd_evaluate_adjoint_ws_bytes(const lsamd_grid* g, int64_t n);
-int lsamd_fd_evaluate_adjoint_ws(const lsamd_grid* g, const double* pts, int64_t n,
-                                 const double* gv, double* out, void* ws, int64_t ws_bytes,
-                                 void* stream);
+/* A20 on a system handle: lsamd_fd_evaluate_adjoint on the handle's grid (same
+ * output, bit for bit) with its temporaries in a buffer cached on the handle
+ * and grown when a call needs more, so repeated calls (one per autograd
+ * backward pass) neither allocate nor synchronise after the first.  Like
+ * every call on a handle, calls on one handle must not overlap. */
+int lsamd_fd_system_evaluate_adjoint(lsamd_fd_system* s, const double* pts, int64_t n,
+                                     const double* gv, double* out, void* stream);
 
 /* A20: per data row r of the system (set_rows order), with u = lambda . a_r
  * and the row's normalised coefficients a_r, rhs b_r, weight w_r:

@@ -95,10 +95,11 @@ SIGNATURES = {
     "lsamd_fd_nearest_d2": [_P, _P, _I64, _P, _P],
     "lsamd_fd_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_evaluate_adjoint_noalloc": [_P, _P, _I64, _P, _P, _P],
-    "lsamd_fd_evaluate_adjoint_ws_bytes": [_P, _I64],
-    "lsamd_fd_evaluate_adjoint_ws": [_P, _P, _I64, _P, _P, _P, _I64, _P],
+    "lsamd_fd_system_evaluate_adjoint": [_P, _P, _I64, _P, _P, _P],
     "lsamd_fd_system_time_stencil": [_P, _I32, _I32, _P, _P],
     "lsamd_fd_system_time_multi": [_P, _I32, _I32, _P, _P, _P],
+    "lsamd_fd_system_mg_build": [_P, _P],
+    "lsamd_fd_systems_solve_mg": [_P, _I32, _P, _P, _P, _P],
     "lsamd_fd_system_stencil_info": [_P, _I32, _P, _P],
     "lsamd_p1_locate": [_P, _P, _I64, _P, _P, _P, _P, _P, _P],
     "lsamd_p1_elements": [_P, _P, _P, _P],
@@ -114,8 +115,7 @@ SIGNATURES = {
     "lsamd_batch_set_rows": [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P],
     "lsamd_batch_solve": [_P, _P, _P, _P, _P, _P],
 }
-_RESTYPE = {"lsamd_last_error": C.c_char_p, "lsamd_p1_n_pairs": C.c_int64,
-            "lsamd_fd_evaluate_adjoint_ws_bytes": C.c_int64}
+_RESTYPE = {"lsamd_last_error": C.c_char_p, "lsamd_p1_n_pairs": C.c_int64}
 
 _lib = None
 

@@ -70,7 +70,7 @@ class FDProblem:
 
     def evaluate(self, c, points):
         pts = torch.as_tensor(np.asarray(points, dtype=float)).to(self.dev)
-        return FDEvaluateFunction.apply(c, self.grid, pts)
+        return FDEvaluateFunction.apply(c, self.system, pts)
 
 
 class FDSolveFunction(torch.autograd.Function):
@@ -111,17 +111,17 @@ class FDSolveFunction(torch.autograd.Function):
 
 
 class FDEvaluateFunction(torch.autograd.Function):
-    """Trilinear evaluation v = E c (bit-exact kernel) with backward E^T g."""
+    """Trilinear evaluation v = E c (bit-exact kernel) with backward E^T g on
+    the problem's system handle (sort buffers cached there)."""
 
     @staticmethod
-    def forward(ctx, c, grid, pts):
-        v, _ = fs.evaluate(grid, c.detach(), pts, True, False)
-        ctx.grid = grid
+    def forward(ctx, c, system, pts):
+        v, _ = fs.evaluate(system.grid, c.detach(), pts, True, False)
+        ctx.system = system
         ctx.save_for_backward(pts)
-        ctx.n = c.shape[0]
         return v
 
     @staticmethod
     def backward(ctx, gv):
         (pts,) = ctx.saved_tensors
-        return fs.evaluate_adjoint(ctx.grid, pts, gv.detach(), ctx.n), None, None
+        return ctx.system.evaluate_adjoint(pts, gv.detach()), None, None

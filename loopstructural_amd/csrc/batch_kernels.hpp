@@ -33,15 +33,18 @@ constexpr int kUniTile = 1 << 20;
 constexpr int kZbLow = -1, kZbHigh = -2;
 constexpr int kBZc = 16;  // z-chunk of the batched bulk tiles
 
-// Work items of one system (index sys) for z-chunk zc: bulk chunks of every
-// 64x8 xy tile (interior tiles flagged kUniTile), then both z-boundary parts.
-inline void stencil_tiles(int sys, const GridDev& g, int zc, std::vector<int4>& out) {
-  const int nbx = (int)ceil_div(g.nx, TX), nby = (int)ceil_div(g.ny, TY);
+// Work items of one system (index sys) for z-chunk zc and tile width txt
+// (64 x 8 or 32 x 16 tiles): bulk chunks of every xy tile (interior tiles
+// flagged kUniTile), then both z-boundary parts.
+inline void stencil_tiles(int sys, const GridDev& g, int zc, std::vector<int4>& out,
+                          int txt = TX) {
+  const int tw = txt, th = NT / txt;
+  const int nbx = (int)ceil_div(g.nx, tw), nby = (int)ceil_div(g.ny, th);
   const int nbz = (int)ceil_div(g.nz - 6, zc);
   for (int ty = 0; ty < nby; ++ty)
     for (int tx = 0; tx < nbx; ++tx) {
-      const bool inter = tx * TX >= 3 && tx * TX + TX - 1 <= g.nx - 4 && ty * TY >= 3 &&
-                         ty * TY + TY - 1 <= g.ny - 4;
+      const bool inter = tx * tw >= 3 && tx * tw + tw - 1 <= g.nx - 4 && ty * th >= 3 &&
+                         ty * th + th - 1 <= g.ny - 4;
       for (int bz = 0; bz < nbz; ++bz)
         out.push_back(make_int4(sys, tx, ty, inter ? kUniTile + bz : bz));
     }
@@ -65,8 +68,9 @@ __constant__ signed char kBTap[kTaps][3] = {
 // whole system or a whole ensemble); per-block partials of x.y.  The
 // single-system plan is the same table with one system.  DMA: bulk planes
 // staged by direct global -> LDS copies (st_bulk_dma) instead of through
-// registers (st_bulk_body); same outputs bit for bit.
-template <typename T, int ZC, bool DMA = false>
+// registers (st_bulk_body); same outputs bit for bit.  TXT: tile width of
+// the table (64 x 8 or 32 x 16 tiles).
+template <typename T, int ZC, bool DMA = false, int TXT = 64>
 __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __restrict__ y,
                                                 const BSys* __restrict__ sys,
                                                 const int4* __restrict__ tiles,
@@ -78,23 +82,25 @@ __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __re
   if (ctl[t.x].done) return;
   const BSys S = sys[t.x];
   double dot;
+  static_assert(!DMA || TXT == 64, "LDS-DMA forms use 64 x 8 tiles");
   if (t.w >= kUniTile) {
     if constexpr (DMA)
       dot = st_bulk_dma<T, ZC, true>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
                                      t.w - kUniTile);
     else
-      dot = st_bulk_body<T, ZC, 4, true>(x + S.off, y + S.off, S.g, tab, ku,
-                                         make_int2(t.y, t.z), t.w - kUniTile);
+      dot = st_bulk_body<T, ZC, 4, true, TXT>(x + S.off, y + S.off, S.g, tab, ku,
+                                              make_int2(t.y, t.z), t.w - kUniTile);
   } else if (t.w >= 0) {
     if constexpr (DMA)
       dot = st_bulk_dma<T, ZC, false>(x + S.off, y + S.off, S.g, tab, ku, make_int2(t.y, t.z),
                                       t.w);
     else
-      dot = st_bulk_body<T, ZC, 4, false>(x + S.off, y + S.off, S.g, tab, ku,
-                                          make_int2(t.y, t.z), t.w);
+      dot = st_bulk_body<T, ZC, 4, false, TXT>(x + S.off, y + S.off, S.g, tab, ku,
+                                               make_int2(t.y, t.z), t.w);
   }
   else
-    dot = st_zb_body<T, DMA>(x + S.off, y + S.off, S.g, tab, t.y, t.z, t.w == kZbLow ? 0 : 1);
+    dot = st_zb_body<T, DMA, TXT>(x + S.off, y + S.off, S.g, tab, t.y, t.z,
+                                  t.w == kZbLow ? 0 : 1);
   const double bs = block_sum(dot, shm);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }

@@ -91,6 +91,13 @@ struct CgCtl {
 // k_st_general: one node per thread, exact two-pass formula.
 constexpr int TX = 64, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 4, PY = TY + 4, PSZ = PX * PY;
+// Work-item tiles are TXT x (NT / TXT): 64 x 8 (TX, TY above) or 32 x 16.
+// Bodies templated on TXT shadow TX, TY, PX, PY and PSZ with their own.
+#define LSAMD_TILE_CONSTANTS(TXT)                                           \
+  constexpr int TX = (TXT), TY = NT / (TXT), PX = TX + 4, PY = TY + 4;      \
+  constexpr int PSZ = PX * PY;                                              \
+  static_assert(TX == 64 || TX == 32, "64 x 8 or 32 x 16 tiles");           \
+  (void)PY
 
 template <typename T>
 struct Coef25 {
@@ -99,7 +106,7 @@ struct Coef25 {
 
 // 25-tap update of one output node from its plane sums; see the tap order
 // in fd_operator.hpp.  b points at the node in the current LDS plane.
-template <typename T>
+template <typename T, int PX = ::lsamd::PX>
 __device__ __forceinline__ T inplane(const T* k, const T* b) {
   return k[0] * b[0] + k[1] * b[1] + k[2] * b[-1] + k[3] * b[PX] + k[4] * b[-PX] + k[5] * b[2] +
          k[6] * b[-2] + k[7] * b[2 * PX] + k[8] * b[-2 * PX] + k[9] * b[2 * PX + 2] +
@@ -108,10 +115,11 @@ __device__ __forceinline__ T inplane(const T* k, const T* b) {
 
 // Output planes [3 + bz*ZC, min(3 + (bz+1)*ZC, nz-3)) of the 64x8 xy tile tb;
 // returns this thread's share of x.y.
-template <typename T, int ZC, int D, bool UNI>
+template <typename T, int ZC, int D, bool UNI, int TXT = 64>
 __device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __restrict__ y,
                                                const GridDev& g, const T* __restrict__ tab,
                                                const Coef25<T>& ku, int2 tb, int bz) {
+  LSAMD_TILE_CONSTANTS(TXT);
   constexpr int NS = ZC + 4;
   __shared__ T pl[2][PSZ];
   const int x0 = tb.x * TX, y0 = tb.y * TY, z0 = 3 + bz * ZC;
@@ -172,7 +180,7 @@ __device__ __forceinline__ double st_bulk_body(const T* __restrict__ x, T* __res
     if (s + D < NS) ld(pbeg + s + D, qa[s % D], qb[s % D]);
     const T* b = pl[s & 1] + (ly + 2) * PX + lx + 2;
     const T c = b[0], xp2 = b[2], xm2 = b[-2], yp2 = b[2 * PX], ym2 = b[-2 * PX];
-    const T v0 = inplane(k, b);
+    const T v0 = inplane<T, PX>(k, b);
     if (s >= 4) {
       const int z = pbeg + s - 2;
       const T yv = v2 + k[13] * r3[0] + k[14] * r1[0] + k[15] * r4[0] + k[16] * r4[1] +
@@ -389,10 +397,11 @@ __device__ __forceinline__ double st_bulk_dma(const T* __restrict__ x, T* __rest
 
 // Output planes [0,3) (bz = 0) or [nz-3, nz) (bz = 1) of xy tile (tx, ty);
 // coefficients re-read per plane.  Returns this thread's share of x.y.
-template <typename T, bool RING = false>
+template <typename T, bool RING = false, int TXT = 64>
 __device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restrict__ y,
                                              const GridDev& g, const T* __restrict__ tab, int tx,
                                              int ty, int bz) {
+  LSAMD_TILE_CONSTANTS(TXT);
   // RING: the two plane buffers are the first two LDS-DMA ring buffers of
   // the kernel (a block runs one body, so they are free here)
   __shared__ T own[RING ? 1 : 2][RING ? 1 : PSZ];
@@ -433,7 +442,7 @@ __device__ __forceinline__ double st_zb_body(const T* __restrict__ x, T* __restr
     const T* b = pl[s & 1] + (ly + 2) * PX + lx + 2;
     const T* t = tc + (int64_t)(pin ? cls7(p, g.nz) : 3) * 49 * kTaps;
     const T c = b[0], xp2 = b[2], xm2 = b[-2], yp2 = b[2 * PX], ym2 = b[-2 * PX];
-    const T v0 = inplane(t, b);
+    const T v0 = inplane<T, PX>(t, b);
     const int z = p - 2;
     if (s >= 4 && z >= 0 && z < g.nz) {
       const T* o = tc + (int64_t)cls7(z, g.nz) * 49 * kTaps;
@@ -1127,6 +1136,7 @@ __global__ void k_check_cols(const int32_t* __restrict__ cols, int64_t nnz, int6
 #include "lsmr.hpp"
 #include "mg.hpp"
 #include "mrhs.hpp"
+#include "bmg.hpp"
 
 // rows of system s may only address its own nodes; one workgroup per system
 namespace lsamd {
@@ -1169,8 +1179,9 @@ struct lsamd_fd_system {
   bool regular = false;   // class-table kernels usable
   int nbx = 0, nby = 0;   // 64x8 xy tiles
   lsamd::BSys* sys1 = nullptr;           // this system as a one-entry table
-  int4* stiles[3] = {nullptr, nullptr, nullptr};  // work items per z-chunk 8/16/32
-  int nstile[3] = {0, 0, 0};
+  // work items per z-chunk 8/16/32 on 64 x 8 tiles, then on 32 x 16 tiles
+  int4* stiles[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nstile[6] = {0, 0, 0, 0, 0, 0};
   double* tab64 = nullptr;
   float* tab32 = nullptr;
   Coef25<double> ku64;
@@ -1186,8 +1197,12 @@ struct lsamd_fd_system {
   BSys* bsys = nullptr;
   int* blk_sys = nullptr;
   int nblk = 0;
-  int4* btiles = nullptr;       // stencil tiles of all systems
+  int4* btiles = nullptr;       // stencil tiles of all systems (64 x 8 tiles)
   int nbtile = 0;
+  int4* btiles32 = nullptr;     // the same on 32 x 16 tiles
+  int nbtile32 = 0;
+  std::vector<int2> trange64, trange32;  // per system: its range in each table
+  int btx = 0;                  // tile width in use (0: not yet tuned, 64 until then)
   Coef25<double> bku64;         // bulk coefficients (shared by all systems)
   Coef25<float> bku32;
   CgCtl* bctl = nullptr;        // per-system PCG control
@@ -1200,7 +1215,7 @@ struct lsamd_fd_system {
   hipGraphExec_t bgraph[2] = {nullptr, nullptr};
   int bgraph_chunk[2] = {0, 0};
   int zc[2] = {0, 0};          // tuned z-chunk per precision (0: not tuned yet)
-  float tune_us[2][6] = {{0}};  // timings of the stencil forms kStCand
+  float tune_us[2][9] = {{0}};  // timings of the stencil forms kStCand
   // data rows
   int64_t nrows = 0, nnz = 0, ndn = 0;
   int64_t* row_ptr = nullptr;
@@ -1290,7 +1305,7 @@ void dfree(lsamd_fd_system* s, void* p) {
 enum RowSlot {
   kRsRowPtr, kRsCols, kRsCoef64, kRsCoef32, kRsB, kRsW, kRsW2_64, kRsW2_32, kRsT64, kRsT32,
   kRsPartRows, kRsURows, kRsBad, kRsErow, kRsKeys, kRsVals, kRsSegEntry, kRsSegRow, kRsUniq,
-  kRsCounts, kRsNrun, kRsTmp, kRsOff32, kRsSegPtr, kRsDnode, kRsSlots
+  kRsCounts, kRsNrun, kRsTmp, kRsOff32, kRsSegPtr, kRsDnode, kRsAdjoint, kRsSlots
 };
 
 template <typename P>
@@ -1351,16 +1366,19 @@ constexpr int kZcCand[3] = {8, 16, 32};
 // Stencil forms of a regular system: the z-chunk, plus kDmaZc when the bulk
 // planes are staged by direct global -> LDS copies (planes below 2 GiB).
 constexpr int kDmaZc = 1000;
-constexpr int kNStCand = 6;
-constexpr int kStCand[kNStCand] = {8, 16, 32, kDmaZc + 8, kDmaZc + 16, kDmaZc + 32};
+constexpr int kNStCand = 9;
+constexpr int kTile32 = 2000;  // + z-chunk: register-staged planes on 32 x 16 tiles
+constexpr int kStCand[kNStCand] = {8,          16,          32,          kDmaZc + 8,    kDmaZc + 16,
+                                   kDmaZc + 32, kTile32 + 8, kTile32 + 16, kTile32 + 32};
 
 bool dma_fits(const GridDev& g, size_t elem) {
   return (int64_t)g.nx * g.ny * (int64_t)elem < ((int64_t)1 << 31);
 }
 
 int zc_index(int zc) {
+  const int base = zc >= kTile32 ? 3 : 0;
   zc %= kDmaZc;
-  return zc == 8 ? 0 : (zc == 16 ? 1 : 2);
+  return base + (zc == 8 ? 0 : (zc == 16 ? 1 : 2));
 }
 
 // P1 plane-streaming stencil configurations (tile width, z-chunk); the
@@ -1392,7 +1410,7 @@ int64_t p1_max_partials(const lsamd_fd_system* s) {
 }
 
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
-  if (s->op == 2) return std::max(s->nblk, s->nbtile);
+  if (s->op == 2) return std::max(s->nblk, std::max(s->nbtile, s->nbtile32));
   if (s->op == 1) {
     if (zc == kP1Node) return ceil_div(s->n, 256);
     return p1_blocks(s, (zc >= 1 && zc <= kP1NCfg) ? zc - 1 : kP1Default - 1);
@@ -1402,15 +1420,16 @@ int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
 }
 
 // the regular single-system stencil: one launch over the system's work items
-template <typename T, int ZC, bool DMA = false>
+template <typename T, int ZC, bool DMA = false, int TXT = 64>
 void launch_bulk(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipStream_t st) {
   const bool f32 = sizeof(T) == 4;
   const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
-  const int zi = zc_index(ZC);
-  hipLaunchKernelGGL((k_b_tiles<T, ZC, DMA>), dim3((unsigned)s->nstile[zi]), dim3(NT), 0, st, x,
-                     y, (const BSys*)s->sys1, (const int4*)s->stiles[zi], tab, ku, s->part, ctl);
+  const int zi = zc_index(ZC) + (TXT == 32 ? 3 : 0);
+  hipLaunchKernelGGL((k_b_tiles<T, ZC, DMA, TXT>), dim3((unsigned)s->nstile[zi]), dim3(NT), 0, st,
+                     x, y, (const BSys*)s->sys1, (const int4*)s->stiles[zi], tab, ku, s->part,
+                     ctl);
 }
 
 template <typename T>
@@ -1418,9 +1437,57 @@ void launch_b_tiles(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipS
   const bool f32 = sizeof(T) == 4;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->bku32.k[q] : (T)s->bku64.k[q];
-  hipLaunchKernelGGL((k_b_tiles<T, kBZc>), dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
-                     (const BSys*)s->bsys, (const int4*)s->btiles,
-                     f32 ? (const T*)s->tab32 : (const T*)s->tab64, ku, s->part, ctl);
+  const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
+  if (s->btx == 32)
+    hipLaunchKernelGGL((k_b_tiles<T, kBZc, false, 32>), dim3((unsigned)s->nbtile32), dim3(NT), 0,
+                       st, x, y, (const BSys*)s->bsys, (const int4*)s->btiles32, tab, ku, s->part,
+                       ctl);
+  else
+    hipLaunchKernelGGL((k_b_tiles<T, kBZc>), dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
+                       (const BSys*)s->bsys, (const int4*)s->btiles, tab, ku, s->part, ctl);
+}
+
+// Tile width of a batch: the 64 x 8 and 32 x 16 tables timed on the FP32
+// stencil (HIP events, best of three rounds of five launches), once; the
+// systems' partial ranges follow the chosen table.  LSAMD_BATCH_TILE=64|32
+// pins it.
+int ensure_batch_tuned(lsamd_fd_system* s, hipStream_t st) {
+  if (s->btx) return LSAMD_OK;
+  int pick = 64;
+  const char* e = getenv("LSAMD_BATCH_TILE");
+  if (e && (atoi(e) == 64 || atoi(e) == 32)) {
+    pick = atoi(e);
+  } else {
+    hipEvent_t e0, e1;
+    LSAMD_HIP(hipEventCreate(&e0));
+    LSAMD_HIP(hipEventCreate(&e1));
+    float best[2] = {1e30f, 1e30f};
+    for (int round = 0; round < 3; ++round)
+      for (int c = 0; c < 2; ++c) {
+        s->btx = c == 0 ? 64 : 32;
+        LSAMD_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 5; ++r) launch_b_tiles<float>(s, s->p32, s->q32, s->bctl_apply, st);
+        LSAMD_HIP(hipEventRecord(e1, st));
+        LSAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best[c] = std::min(best[c], ms);
+      }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    LSAMD_CHECK_LAUNCH("batch tile tuning");
+    pick = best[1] < best[0] ? 32 : 64;
+  }
+  s->btx = pick;
+  const std::vector<int2>& r = pick == 32 ? s->trange32 : s->trange64;
+  for (int i = 0; i < s->nsys; ++i) {
+    s->bsys_h[i].t0 = r[i].x;
+    s->bsys_h[i].t1 = r[i].y;
+  }
+  LSAMD_HIP(hipMemcpyAsync(s->bsys, s->bsys_h.data(), s->nsys * sizeof(BSys),
+                           hipMemcpyHostToDevice, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  return LSAMD_OK;
 }
 
 // Launch the regulariser stencil; returns the number of block partials.
@@ -1484,6 +1551,9 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
     case kDmaZc + 8: launch_bulk<T, 8, true>(s, x, y, ctl, st); break;
     case kDmaZc + 16: launch_bulk<T, 16, true>(s, x, y, ctl, st); break;
     case kDmaZc + 32: launch_bulk<T, 32, true>(s, x, y, ctl, st); break;
+    case kTile32 + 8: launch_bulk<T, 8, false, 32>(s, x, y, ctl, st); break;
+    case kTile32 + 16: launch_bulk<T, 16, false, 32>(s, x, y, ctl, st); break;
+    case kTile32 + 32: launch_bulk<T, 32, false, 32>(s, x, y, ctl, st); break;
     default: launch_bulk<T, 32>(s, x, y, ctl, st); break;
   }
   return stencil_partials(s, zc);
@@ -1541,7 +1611,7 @@ int ensure_tuned(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
   if (const char* e = getenv("LSAMD_STENCIL_ZC")) {
     const int v = atoi(e);
     for (int c = 0; c < kNStCand; ++c)
-      if (v == kStCand[c] && (v < kDmaZc || dma_fits(s->gd, sizeof(T)))) {
+      if (v == kStCand[c] && (v < kDmaZc || v > kTile32 || dma_fits(s->gd, sizeof(T)))) {
         s->zc[pi] = v;
         return LSAMD_OK;
       }
@@ -1561,7 +1631,7 @@ int ensure_tuned(lsamd_fd_system* s, T* x, T* y, hipStream_t st) {
       // a chunk longer than the bulk gains nothing; LDS-DMA forms address a
       // plane with 32-bit byte offsets
       if (zc % kDmaZc > 8 && s->gd.nz - 6 < zc % kDmaZc) continue;
-      if (zc > kDmaZc && !dma_fits(s->gd, sizeof(T))) continue;
+      if (zc > kDmaZc && zc < kTile32 && !dma_fits(s->gd, sizeof(T))) continue;
       launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
       LSAMD_HIP(hipEventRecord(e0, st));
       for (int r = 0; r < 5; ++r) launch_stencil<T>(s, x, y, s->ctl_apply, st, zc);
@@ -1685,7 +1755,12 @@ int run_pcg(lsamd_fd_system* s, int chunk, T* x, T* r, T* p, T* q, T* t, const T
 // Work buffers, control blocks and the regulariser-only diagonal shared by
 // every system kind.
 int init_common(lsamd_fd_system* s) {
-  s->part_n = std::max<int64_t>(s->op == 1 ? p1_max_partials(s) : stencil_partials(s, 8), 8192);
+  int64_t pm = 8192;
+  if (s->op == 1)
+    pm = std::max(pm, p1_max_partials(s));
+  else
+    for (int c = 0; c < kNStCand; ++c) pm = std::max(pm, stencil_partials(s, kStCand[c]));
+  s->part_n = pm;
   if (int rc = dalloc(s, &s->part, s->part_n)) return rc;
   if (int rc = dalloc(s, &s->ctl, 1)) return rc;
   if (int rc = dalloc(s, &s->ctl_apply, 1)) return rc;
@@ -1853,6 +1928,13 @@ struct Mg {
   std::vector<std::pair<void*, size_t>> dmem, pool;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
+  bool graph_stale = false;  // captured with addresses of an older row set
+  // level-0 data term as a Gram matrix (CSR, FP32 and FP64 values), kept
+  // for the batched solve's node-wise operator (row-dependent, pooled)
+  int64_t* g0ptr = nullptr;
+  int32_t* g0col = nullptr;
+  float* g0v32 = nullptr;
+  double* g0v64 = nullptr;
 };
 
 void mg_destroy(Mg* m) {
@@ -1866,13 +1948,16 @@ void mg_destroy(Mg* m) {
   delete m;
 }
 
-// The data rows changed: return their memory to the pool and drop the
-// captured iteration (it holds their addresses).
+// The data rows changed: return their memory to the pool and mark the
+// captured iteration stale (it holds their addresses); the next solve
+// re-captures it and updates the executable graph in place when the
+// topology is unchanged, which costs far less than instantiating anew.
 void mg_invalidate(Mg* m) {
-  if (m->graph) {
-    hipGraphExecDestroy(m->graph);
-    m->graph = nullptr;
-  }
+  m->graph_stale = m->graph != nullptr;
+  m->g0ptr = nullptr;
+  m->g0col = nullptr;
+  m->g0v32 = nullptr;
+  m->g0v64 = nullptr;
   for (auto& b : m->dmem) m->pool.push_back(b);
   m->dmem.clear();
   for (MgLevel& L : m->lv) {
@@ -2505,10 +2590,11 @@ int mg_build_rows(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   const bool schwarz = m->schwarz;
   const bool single = m->lv.size() == 1;  // level 0 is the coarsest
   // level 0 keeps its data term as rows; a Gram matrix of them is built here
-  // for the entries of its Schwarz blocks (or of its dense inverse)
+  // for the entries of its Schwarz blocks (or of its dense inverse) and kept
+  // for the batched solve
   std::vector<std::vector<int64_t>> cells(1);
   CsrOut G0;
-  if (schwarz || single) {
+  {
     int wmax = 0;
     if (int rc = mg_data_cells(s, cells[0], &wmax, st)) return rc;
     if (s->nrows > 0 && wmax > 0) {
@@ -2534,9 +2620,10 @@ int mg_build_rows(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   if (single) {
     if (int rc = mg_coarsest(m, G0, st)) return rc;
   }
-  for (const void* p : {(const void*)G0.ptr, (const void*)G0.col, (const void*)G0.v32,
-                        (const void*)G0.v64})
-    mg_give(m, p);
+  m->g0ptr = G0.ptr;
+  m->g0col = G0.col;
+  m->g0v32 = G0.v32;
+  m->g0v64 = G0.v64;
   // data rows of the previous level: CSR (level 0: the system's rows)
   const int64_t nr = s->nrows;
   const int64_t* prow = s->row_ptr;
@@ -2715,7 +2802,7 @@ int run_fcg(lsamd_fd_system* s, int chunk, hipStream_t st) {
     hipGraphExecDestroy(m->graph);
     m->graph = nullptr;
   }
-  if (!m->graph) {
+  if (!m->graph || m->graph_stale) {
     std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
     hipStream_t cs;
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -2727,10 +2814,23 @@ int run_fcg(lsamd_fd_system* s, int chunk, hipStream_t st) {
     hipStreamDestroy(cs);
     if (rc) return rc;
     if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid graph capture failed");
-    e = hipGraphInstantiate(&m->graph, graph, nullptr, nullptr, 0);
+    bool updated = false;
+    if (m->graph) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      updated = hipGraphExecUpdate(m->graph, graph, &err_node, &res) == hipSuccess &&
+                res == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        hipGetLastError();  // a refused update is not an error: instantiate instead
+        hipGraphExecDestroy(m->graph);
+        m->graph = nullptr;
+      }
+    }
+    if (!updated) e = hipGraphInstantiate(&m->graph, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(LSAMD_ERR_HIP, "multigrid graph instantiate failed");
     m->graph_chunk = chunk;
+    m->graph_stale = false;
   }
   for (;;) {
     LSAMD_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(CgCtl), hipMemcpyDeviceToHost, st));
@@ -2784,6 +2884,257 @@ int mg_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const double* b, dou
   inf->residual = s->ctl_host->iter > 0 ? sqrt(s->ctl_host->res2) : sqrt(s->ctl_host->rho);
   inf->threshold = s->ctl_host->thr;
   return LSAMD_OK;
+}
+
+// ------------------------------------------- batched multigrid solve ----
+// Multigrid-preconditioned flexible CG of many FD systems in one launch
+// sequence per iteration (kernels in bmg.hpp).  Each system's hierarchy is
+// built or brought up to date by the single-system code; the descriptors,
+// block tables and per-system control live in one temporary allocation.
+int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
+              double* const* c, lsamd_solve_info* infos, hipStream_t st) {
+  for (int i = 0; i < nsys; ++i) {
+    lsamd_fd_system* s = ss[i];
+    if (!s || s->op != 0 || !c[i])
+      return fail(LSAMD_ERR_ARG, "batched multigrid: every system must be an FD system with an "
+                                 "output");
+    if (int rc = ensure_work(s, false)) return rc;
+    if (int rc = mg_ensure(s, st)) return rc;
+  }
+  const int deg = ss[0]->mg->deg, deg0 = ss[0]->mg->deg0;
+  int nl = 0;
+  int64_t nmax = 0;
+  for (int i = 0; i < nsys; ++i) {
+    const Mg* m = ss[i]->mg;
+    if (m->deg != deg || m->deg0 != deg0)
+      return fail(LSAMD_ERR_ARG, "batched multigrid: hierarchies with different smoothers");
+    if (m->nc > kMgCoarse * kMgCoarse * kMgCoarse)
+      return fail(LSAMD_ERR_ARG, "batched multigrid: coarsest level too large");
+    nl = std::max(nl, (int)m->lv.size());
+    nmax = std::max(nmax, ss[i]->n);
+  }
+  // host descriptors and tables; a system's blocks are contiguous
+  std::vector<std::vector<BmgLevel>> hl(nl, std::vector<BmgLevel>(nsys));
+  std::vector<std::vector<int2>> all(nl), sm(nl);
+  std::vector<std::vector<int>> dense(nl);
+  std::vector<BmgSys> hs(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    lsamd_fd_system* s = ss[i];
+    const Mg* m = s->mg;
+    const int last = (int)m->lv.size() - 1;
+    for (int l = 0; l <= last; ++l) {
+      const MgLevel& V = m->lv[l];
+      BmgLevel& d = hl[l][i];
+      memset(&d, 0, sizeof(d));
+      d.A = mg_dev(V);
+      if (l == 0) {
+        d.A.gptr = m->g0ptr;
+        d.A.gcol = m->g0col;
+        d.A.gval = m->g0v32;
+      }
+      d.S = mg_sm(V);
+      d.b = V.b;
+      d.x = V.x;
+      d.q = V.q;
+      d.d = V.d;
+      d.c0 = V.c0;
+      for (int k = 0; k < kMgDegMax; ++k) {
+        d.c1[k] = V.c1[k];
+        d.c2[k] = V.c2[k];
+      }
+      if (l == last) {
+        d.cinv = m->cinv;
+        d.nc = m->nc;
+        dense[l].push_back(i);
+      }
+      const int nb = (int)ceil_div(V.n, 256);
+      if (l == 0) hs[i].b0 = (int)all[0].size();
+      for (int b = 0; b < nb; ++b) {
+        all[l].push_back(make_int2(i, b));
+        if (l < last) sm[l].push_back(make_int2(i, b));
+      }
+      if (l == 0) hs[i].b1 = (int)all[0].size();
+    }
+    BmgSys& y = hs[i];
+    y.g = s->gd;
+    y.n = s->n;
+    y.tab = s->regular ? s->tab64 : nullptr;
+    y.w = make_regw<double>(s->reg);
+    y.gptr = m->g0ptr;
+    y.gcol = m->g0col;
+    y.gval = m->g0v64;
+    y.x = s->x64;
+    y.r = s->r64;
+    y.rp = m->rp64;
+    y.p = s->p64;
+    y.q = s->q64;
+    y.minv = s->minv64;
+    y.rhs = s->rhs;
+    y.out = c[i];
+  }
+  // one device allocation: descriptors, tables, partials, control, flag
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t bytes = up(nsys * sizeof(BmgSys)) + up(nsys * sizeof(CgCtl)) + up(sizeof(int)) +
+                 up(2 * all[0].size() * sizeof(double));
+  for (int l = 0; l < nl; ++l)
+    bytes += up(nsys * sizeof(BmgLevel)) + up(all[l].size() * sizeof(int2)) +
+             up(sm[l].size() * sizeof(int2)) + up(dense[l].size() * sizeof(int));
+  unsigned char* base = nullptr;
+  if (hipMalloc((void**)&base, bytes) != hipSuccess)
+    return fail(LSAMD_ERR_ALLOC, "batched multigrid: hipMalloc failed");
+  int* hflag = nullptr;
+  if (hipHostMalloc((void**)&hflag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    hipFree(base);
+    return fail(LSAMD_ERR_ALLOC, "batched multigrid: hipHostMalloc failed");
+  }
+  unsigned char* cur = base;
+  auto take = [&](size_t b) {
+    unsigned char* p = cur;
+    cur += up(b);
+    return p;
+  };
+  hipGraphExec_t exec = nullptr;
+  auto finish = [&](int rc) {
+    if (exec) hipGraphExecDestroy(exec);
+    hipStreamSynchronize(st);
+    hipFree(base);
+    hipHostFree(hflag);
+    return rc;
+  };
+  BmgSys* dsys = (BmgSys*)take(nsys * sizeof(BmgSys));
+  CgCtl* ctl = (CgCtl*)take(nsys * sizeof(CgCtl));
+  int* flag = (int*)take(sizeof(int));
+  double* part = (double*)take(2 * all[0].size() * sizeof(double));
+  std::vector<BmgLevel*> dlv(nl);
+  std::vector<int2*> dall(nl), dsm(nl);
+  std::vector<int*> ddense(nl);
+  for (int l = 0; l < nl; ++l) {
+    dlv[l] = (BmgLevel*)take(nsys * sizeof(BmgLevel));
+    dall[l] = (int2*)take(all[l].size() * sizeof(int2));
+    dsm[l] = (int2*)take(sm[l].size() * sizeof(int2));
+    ddense[l] = (int*)take(dense[l].size() * sizeof(int));
+  }
+  auto h2d = [&](void* d, const void* h, size_t b) {
+    return b ? hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, st) : hipSuccess;
+  };
+  hipError_t e = h2d(dsys, hs.data(), nsys * sizeof(BmgSys));
+  for (int l = 0; l < nl && e == hipSuccess; ++l) {
+    e = h2d(dlv[l], hl[l].data(), nsys * sizeof(BmgLevel));
+    if (e == hipSuccess) e = h2d(dall[l], all[l].data(), all[l].size() * sizeof(int2));
+    if (e == hipSuccess) e = h2d(dsm[l], sm[l].data(), sm[l].size() * sizeof(int2));
+    if (e == hipSuccess) e = h2d(ddense[l], dense[l].data(), dense[l].size() * sizeof(int));
+  }
+  if (e != hipSuccess) return finish(fail(LSAMD_ERR_HIP, "batched multigrid: upload failed"));
+  const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * nmax;
+  const int maxiter = (int)std::min<int64_t>(maxiter64, 2000000000LL);
+  const int chunk = std::max(1, std::min(o->check_every > 0 ? o->check_every : 8, 8));
+  const dim3 b256(256);
+  auto g = [](size_t nb) { return dim3((unsigned)nb); };
+  const unsigned n0 = (unsigned)all[0].size();
+  hipLaunchKernelGGL(k_bmg_init, g(n0), b256, 0, st, (const BmgSys*)dsys, dlv[0],
+                     (const int2*)dall[0], part);
+  hipLaunchKernelGGL(k_bmg_init_fin, g(nsys), b256, 0, st, (const BmgSys*)dsys,
+                     (const double*)part, o->rtol, o->atol, maxiter, ctl);
+  hipLaunchKernelGGL(k_bmg_all_done, dim3(1), b256, 0, st, (const CgCtl*)ctl, nsys, flag);
+  LSAMD_CHECK_LAUNCH("batched multigrid init");
+  // one iteration for every system
+  auto cheb = [&](int l, int mode, int k, int last, hipStream_t cs) {
+    hipLaunchKernelGGL(k_bmg_cheb, g(sm[l].size()), b256, 0, cs, (const BmgLevel*)dlv[l],
+                       (const int2*)dsm[l], mode, k, last, (const CgCtl*)ctl);
+  };
+  auto apply = [&](int l, hipStream_t cs) {
+    hipLaunchKernelGGL(k_bmg_apply, g(sm[l].size()), b256, 0, cs, (const BmgLevel*)dlv[l],
+                       (const int2*)dsm[l], (const CgCtl*)ctl);
+  };
+  auto smooth = [&](int l, int from_zero, hipStream_t cs) {
+    const int dg = l == 0 ? deg0 : deg;
+    cheb(l, from_zero ? 0 : 1, 0, 0, cs);
+    for (int k = 1; k < dg; ++k) {
+      apply(l, cs);
+      cheb(l, 2, k, k == dg - 1, cs);
+    }
+  };
+  auto iteration = [&](hipStream_t cs) {
+    for (int l = 0; l < nl; ++l) {
+      if (!sm[l].empty()) {
+        smooth(l, 1, cs);
+        apply(l, cs);
+        hipLaunchKernelGGL(k_bmg_restrict, g(all[l + 1].size()), b256, 0, cs,
+                           (const BmgLevel*)dlv[l], (const BmgLevel*)dlv[l + 1],
+                           (const int2*)dall[l + 1], (const CgCtl*)ctl);
+      }
+      if (!dense[l].empty())
+        hipLaunchKernelGGL(k_bmg_dense, dim3(kMgCoarse * kMgCoarse * kMgCoarse,
+                                             (unsigned)dense[l].size()),
+                           dim3(64), 0, cs, (const BmgLevel*)dlv[l], (const int*)ddense[l],
+                           (const CgCtl*)ctl);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+      if (sm[l].empty()) continue;
+      hipLaunchKernelGGL(k_bmg_prolong, g(sm[l].size()), b256, 0, cs, (const BmgLevel*)dlv[l],
+                         (const BmgLevel*)dlv[l + 1], (const int2*)dsm[l], (const CgCtl*)ctl);
+      apply(l, cs);
+      smooth(l, 0, cs);
+    }
+    hipLaunchKernelGGL(k_bmg_dots, g(n0), b256, 0, cs, (const BmgSys*)dsys,
+                       (const BmgLevel*)dlv[0], (const int2*)dall[0], part, (const CgCtl*)ctl);
+    hipLaunchKernelGGL(k_bmg_dots_fin, g(nsys), b256, 0, cs, (const BmgSys*)dsys,
+                       (const double*)part, ctl);
+    hipLaunchKernelGGL(k_bmg_dir, g(n0), b256, 0, cs, (const BmgSys*)dsys,
+                       (const BmgLevel*)dlv[0], (const int2*)dall[0], (const CgCtl*)ctl);
+    hipLaunchKernelGGL(k_bmg_apply64, g(n0), b256, 0, cs, (const BmgSys*)dsys,
+                       (const int2*)dall[0], part, (const CgCtl*)ctl);
+    hipLaunchKernelGGL(k_bmg_alpha, g(nsys), b256, 0, cs, (const BmgSys*)dsys,
+                       (const double*)part, ctl);
+    hipLaunchKernelGGL(k_bmg_upd, g(n0), b256, 0, cs, (const BmgSys*)dsys,
+                       (const BmgLevel*)dlv[0], (const int2*)dall[0], part, (const CgCtl*)ctl);
+    hipLaunchKernelGGL(k_bmg_upd_fin, g(nsys), b256, 0, cs, (const BmgSys*)dsys,
+                       (const double*)part, ctl);
+    hipLaunchKernelGGL(k_bmg_all_done, dim3(1), b256, 0, cs, (const CgCtl*)ctl, nsys, flag);
+  };
+  {
+    std::lock_guard<std::recursive_mutex> capture_lock(g_legacy_mu);
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+      return finish(fail(LSAMD_ERR_HIP, "batched multigrid: stream creation failed"));
+    hipGraph_t graph;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int it = 0; it < chunk; ++it) iteration(cs);
+      e = hipStreamEndCapture(cs, &graph);
+    }
+    hipStreamDestroy(cs);
+    if (e != hipSuccess)
+      return finish(fail(LSAMD_ERR_HIP, "batched multigrid: graph capture failed"));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      return finish(fail(LSAMD_ERR_HIP, "batched multigrid: graph instantiate failed"));
+    }
+  }
+  for (;;) {
+    e = hipMemcpyAsync(hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return finish(fail(LSAMD_ERR_HIP, "batched multigrid: poll failed"));
+    if (*hflag) break;
+    if (hipGraphLaunch(exec, st) != hipSuccess)
+      return finish(fail(LSAMD_ERR_HIP, "batched multigrid: graph launch failed"));
+  }
+  hipLaunchKernelGGL(k_bmg_out, g(n0), b256, 0, st, (const BmgSys*)dsys, (const int2*)dall[0]);
+  std::vector<CgCtl> hc(nsys);
+  e = hipMemcpyAsync(hc.data(), ctl, nsys * sizeof(CgCtl), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess || hipGetLastError() != hipSuccess)
+    return finish(fail(LSAMD_ERR_HIP, "batched multigrid: solve failed"));
+  for (int i = 0; i < nsys; ++i) {
+    infos[i].iterations = hc[i].iter;
+    infos[i].outer = 0;
+    infos[i].converged = hc[i].converged;
+    infos[i].residual = hc[i].iter > 0 ? sqrt(hc[i].res2) : sqrt(hc[i].rho);
+    infos[i].threshold = hc[i].thr;
+  }
+  return finish(LSAMD_OK);
 }
 
 // ------------------------------------------------- multi-rhs multigrid ----
@@ -3637,9 +3988,9 @@ int lsamd_fd_system_create(const lsamd_grid* g, const lsamd_fd_reg* reg, const d
     one.n = s->n;
     if (int rc = dalloc(s, &s->sys1, 1)) return bail(rc);
     LSAMD_HIP(legacy_memcpy(s->sys1, &one, sizeof(BSys), hipMemcpyHostToDevice));
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 6; ++c) {
       std::vector<int4> t;
-      stencil_tiles(0, s->gd, kZcCand[c], t);
+      stencil_tiles(0, s->gd, kZcCand[c % 3], t, c < 3 ? 64 : 32);
       s->nstile[c] = (int)t.size();
       if (int rc = dalloc(s, &s->stiles[c], t.size())) return bail(rc);
       LSAMD_HIP(legacy_memcpy(s->stiles[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -4191,19 +4542,44 @@ int lsamd_fd_evaluate_adjoint_noalloc(const lsamd_grid* g, const double* pts, in
                            "lsamd_fd_evaluate_adjoint_noalloc");
 }
 
-int64_t lsamd_fd_evaluate_adjoint_ws_bytes(const lsamd_grid* g, int64_t n) {
-  if (!g || n < 0) return -1;
-  return (int64_t)eval_adjoint_bytes(g, n);
+int lsamd_fd_system_evaluate_adjoint(lsamd_fd_system* s, const double* pts, int64_t n,
+                                     const double* gv, double* out, void* stream) {
+  // lsamd_fd_evaluate_adjoint on the handle's grid with its temporaries in a
+  // buffer cached on the handle (grown when a call needs more): repeated
+  // calls (an autograd backward pass each) stop allocating and never
+  // synchronise.
+  if (int rc = injected("system_evaluate_adjoint")) return rc;
+  if (!s) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_evaluate_adjoint: NULL system");
+  if (n < 0) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_evaluate_adjoint: n < 0");
+  unsigned char* ws = nullptr;
+  const size_t bytes = eval_adjoint_bytes(&s->grid, n);
+  if (bytes > 0)
+    if (int rc = rbuf(s, kRsAdjoint, &ws, (int64_t)bytes)) return rc;
+  return eval_adjoint_into(&s->grid, pts, n, gv, out, ws,
+                           bytes > 0 ? s->rpool[kRsAdjoint].second : 0, (hipStream_t)stream,
+                           "lsamd_fd_system_evaluate_adjoint");
 }
 
-int lsamd_fd_evaluate_adjoint_ws(const lsamd_grid* g, const double* pts, int64_t n,
-                                 const double* gv, double* out, void* ws, int64_t ws_bytes,
-                                 void* stream) {
-  if (int rc = injected("evaluate_adjoint_ws")) return rc;
-  if (n > 0 && (!ws || ws_bytes < 0 || ((uintptr_t)ws & 255)))
-    return fail(LSAMD_ERR_ARG, "lsamd_fd_evaluate_adjoint_ws: workspace NULL or not 256-aligned");
-  return eval_adjoint_into(g, pts, n, gv, out, (unsigned char*)ws, (size_t)ws_bytes,
-                           (hipStream_t)stream, "lsamd_fd_evaluate_adjoint_ws");
+int lsamd_fd_system_mg_build(lsamd_fd_system* s, void* stream) {
+  if (int rc = injected("mg_build")) return rc;
+  if (!s || s->op != 0) return fail(LSAMD_ERR_ARG, "lsamd_fd_system_mg_build: not an FD system");
+  if (int rc = ensure_work(s, false)) return rc;
+  return mg_ensure(s, (hipStream_t)stream);
+}
+
+int lsamd_fd_systems_solve_mg(lsamd_fd_system* const* systems, int32_t nsys,
+                              const lsamd_solve_opts* o, double* const* c,
+                              lsamd_solve_info* infos, void* stream) {
+  if (int rc = injected("systems_solve_mg")) return rc;
+  if (!systems || nsys < 1 || !o || !c || !infos)
+    return fail(LSAMD_ERR_ARG, "lsamd_fd_systems_solve_mg: bad arguments");
+  if (!(o->rtol >= 0.0) || !(o->atol >= 0.0))
+    return fail(LSAMD_ERR_ARG, "rtol/atol must be non-negative");
+  for (int i = 0; i < nsys; ++i)
+    for (int j = 0; j < i; ++j)
+      if (systems[i] == systems[j])
+        return fail(LSAMD_ERR_ARG, "lsamd_fd_systems_solve_mg: a system appears twice");
+  return bmg_solve(systems, nsys, o, c, infos, (hipStream_t)stream);
 }
 
 int lsamd_fd_system_stencil_info(lsamd_fd_system* s, int32_t precision, int32_t* zc,
@@ -4287,12 +4663,18 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
     for (int b = hs[i].b0; b < hs[i].b1; ++b) bsys_tab[b] = i;
   // stencil tiles: per system, bulk z-chunks of every xy tile (uniform-
   // coefficient tiles flagged) then the two z-boundary parts
-  std::vector<int4> tiles;
+  std::vector<int4> tiles, tiles32;
+  s->trange64.resize(nsys);
+  s->trange32.resize(nsys);
   for (int i = 0; i < nsys; ++i) {
     hs[i].t0 = (int)tiles.size();
     stencil_tiles(i, hs[i].g, kBZc, tiles);
     hs[i].t1 = (int)tiles.size();
-    if (tiles.size() > 2000000000ULL)
+    s->trange64[i] = make_int2(hs[i].t0, hs[i].t1);
+    const int a = (int)tiles32.size();
+    stencil_tiles(i, hs[i].g, kBZc, tiles32, 32);
+    s->trange32[i] = make_int2(a, (int)tiles32.size());
+    if (tiles.size() > 2000000000ULL || tiles32.size() > 2000000000ULL)
       return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
   }
   if (int rc = dalloc(s, &s->bsys, nsys)) return bail(rc);
@@ -4306,6 +4688,10 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   s->nbtile = (int)tiles.size();
   if (int rc = dalloc(s, &s->btiles, tiles.size())) return bail(rc);
   LSAMD_HIP(legacy_memcpy(s->btiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  s->nbtile32 = (int)tiles32.size();
+  if (int rc = dalloc(s, &s->btiles32, tiles32.size())) return bail(rc);
+  LSAMD_HIP(legacy_memcpy(s->btiles32, tiles32.data(), tiles32.size() * sizeof(int4),
+                          hipMemcpyHostToDevice));
   LSAMD_HIP(legacy_memcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
   LSAMD_HIP(legacy_memcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   LSAMD_HIP(legacy_memset(s->bctl, 0, nsys * sizeof(CgCtl)));
@@ -4389,6 +4775,7 @@ int lsamd_batch_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, const doubl
   const int ns = s->nsys;
   const double* b = rhs ? rhs : s->rhs;
   if (int rc = ensure_batch_work(s)) return rc;
+  if (int rc = ensure_batch_tuned(s, st)) return rc;
   int64_t nmax = 0;
   for (const BSys& q : s->bsys_h) nmax = std::max(nmax, q.n);
   const int64_t maxiter64 = o->maxiter > 0 ? o->maxiter : 10 * nmax;

@@ -18,6 +18,8 @@ import torch
 from . import _lib
 
 F64 = torch.float64
+# stencil forms of the autotuner, in the order of lsamd_fd_system_stencil_info
+STENCIL_FORMS = (8, 16, 32, 1008, 1016, 1032, 2008, 2016, 2032)
 
 
 @dataclass
@@ -242,6 +244,17 @@ class FdSystem:
                   _lib.stream())
         return float(ms.value)
 
+    def evaluate_adjoint(self, pts: torch.Tensor, gv: torch.Tensor) -> torch.Tensor:
+        """A20: E^T gv on this handle's grid (E the trilinear evaluation matrix
+        of pts), bit-identical to lsamd_fd_evaluate_adjoint; the sort buffers
+        stay cached on the handle, so repeated calls do not allocate."""
+        pts = pts.to(self.dev, F64).contiguous()
+        gv = gv.to(self.dev, F64).contiguous()
+        out = torch.empty(self.n, dtype=F64, device=self.dev)
+        _lib.call("lsamd_fd_system_evaluate_adjoint", self._h, _lib.ptr(pts), int(pts.shape[0]),
+                  _lib.ptr(gv), _lib.ptr(out), _lib.stream())
+        return out
+
     def time_multi(self, kernel: str, iters=20):
         """Average launch time (ms) and algorithmic bytes of one fine-level
         kernel of the 32-lane multi-rhs multigrid ('cheb', 'update' or
@@ -254,15 +267,16 @@ class FdSystem:
         return float(ms.value), float(nb.value)
 
     def stencil_info(self, precision="fp64"):
-        """Autotuned stencil form (z-chunk 8/16/32, or 1008/1016/1032 for
-        z-chunk 8/16/32 with LDS-DMA staged planes; 0: per-node kernel) and
-        the timings of every form (us, -1 = skipped)."""
+        """Autotuned stencil form (z-chunk 8/16/32 on 64 x 8 tiles, 1008/1016/
+        1032 for those with LDS-DMA staged planes, 2008/2016/2032 on 32 x 16
+        tiles; 0: per-node kernel) and the timings of every form (us, -1 =
+        skipped)."""
         zc = C.c_int32(0)
-        us = (C.c_float * 6)()
+        us = (C.c_float * 9)()
         _lib.call("lsamd_fd_system_stencil_info", self._h,
                   _lib.FP32 if precision == "fp32" else _lib.FP64, C.byref(zc), us)
         return {"zc": int(zc.value),
-                "candidates_us": dict(zip((8, 16, 32, 1008, 1016, 1032), list(us)))}
+                "candidates_us": dict(zip(STENCIL_FORMS, list(us)))}
 
     def adjoint_rows(self, c: torch.Tensor, lam: torch.Tensor):
         R = self.rows.nrows if self.rows is not None else 0
@@ -318,24 +332,6 @@ def evaluate(grid: _lib.Grid, c: torch.Tensor, pts: torch.Tensor, values=True, g
     _lib.call("lsamd_fd_evaluate", C.byref(grid), _lib.ptr(c), _lib.ptr(pts), n, _lib.ptr(v),
               _lib.ptr(g), _lib.stream())
     return v, g
-
-
-def evaluate_adjoint(grid: _lib.Grid, pts: torch.Tensor, gv: torch.Tensor, nnodes: int):
-    """A20: E^T gv (E the trilinear evaluation matrix of pts), bit-identical to
-    lsamd_fd_evaluate_adjoint, with its temporaries in a workspace from the
-    caching allocator on the current stream: no device allocation or stream
-    synchronisation once the allocator holds a block of that size."""
-    pts = pts.to(_lib.device(), F64).contiguous()
-    gv = gv.to(pts.device, F64).contiguous()
-    n = int(pts.shape[0])
-    out = torch.empty(int(nnodes), dtype=F64, device=pts.device)
-    nb = int(_lib.load().lsamd_fd_evaluate_adjoint_ws_bytes(C.byref(grid), n))
-    if nb < 0:
-        raise _lib.LsamdError(1, "lsamd_fd_evaluate_adjoint_ws_bytes: bad arguments")
-    ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=pts.device)
-    _lib.call("lsamd_fd_evaluate_adjoint_ws", C.byref(grid), _lib.ptr(pts), n, _lib.ptr(gv),
-              _lib.ptr(out), _lib.ptr(ws), int(ws.numel()), _lib.stream())
-    return out
 
 
 # ------------------------------------------------------------------- P1 ----
@@ -557,6 +553,105 @@ class BatchSystem:
         return float(ms.value)
 
 
+def solve_mg_batch(systems, rtol=1e-5, atol=0.0, maxiter=None, check_every=8):
+    """Multigrid-preconditioned solves of independent FdSystems (distinct
+    handles, rows set) in one launch sequence per iteration for all of them
+    (``lsamd_fd_systems_solve_mg``); each keeps its own hierarchy, stopping
+    test and iteration count.  Returns (list of c, list of infos)."""
+    n = len(systems)
+    o = _lib.SolveOpts()
+    o.rtol, o.atol = float(rtol), float(atol)
+    o.maxiter = int(maxiter) if maxiter else 0
+    o.precision = _lib.FP64
+    o.check_every = int(check_every)
+    o.jacobi = 2
+    dev = _lib.device()
+    outs = [torch.empty(sy.n, dtype=F64, device=dev) for sy in systems]
+    handles = (C.c_void_p * n)(*[sy._h.value for sy in systems])
+    ptrs = (C.c_void_p * n)(*[t.data_ptr() for t in outs])
+    infos = (_lib.SolveInfo * n)()
+    _lib.call("lsamd_fd_systems_solve_mg", handles, n, C.byref(o), ptrs, infos, _lib.stream())
+    return outs, [{"iterations": int(i.iterations), "outer": int(i.outer),
+                   "converged": bool(i.converged), "residual": float(i.residual),
+                   "threshold": float(i.threshold)} for i in infos]
+
+
+class BatchedMultigrid:
+    """Uneven ensemble solved by batched multigrid: one handle per system
+    (its own hierarchy), all systems advanced together by one launch sequence
+    per iteration.  ``prepare`` creates the handles and builds the grid-only
+    part of every hierarchy; ``solve`` sets each system's rows, brings its
+    hierarchy up to date (host threads, one stream each) and runs the
+    batched solve."""
+
+    def __init__(self, grids, reg: _lib.FdReg, workers: int = 12):
+        self.dev = _lib.device()
+        self.grids = list(grids)
+        self.reg = reg
+        self.workers = max(1, int(workers))
+        self.systems = []
+        self.last_timing = {}
+
+    def close(self):
+        for h in self.systems:
+            h.close()
+        self.systems = []
+
+    def _threaded(self, fn):
+        import threading
+
+        idx = list(range(len(self.systems)))
+        lock = threading.Lock()
+        errors = []
+
+        def work():
+            stream = torch.cuda.Stream(device=self.dev)
+            with torch.cuda.stream(stream):
+                while True:
+                    with lock:
+                        if not idx or errors:
+                            break
+                        s = idx.pop(0)
+                    try:
+                        fn(s)
+                    except Exception as e:  # surfaced on the calling thread
+                        with lock:
+                            errors.append(e)
+                        break
+            stream.synchronize()
+
+        threads = [threading.Thread(target=work) for _ in range(min(self.workers, len(idx)))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+
+    def prepare(self, blocks):
+        """One handle per system, its rows and its full hierarchy."""
+        if not self.systems:
+            self.systems = [FdSystem(g, self.reg) for g in self.grids]
+
+        def build(s):
+            self.systems[s].set_rows(blocks[s])
+            _lib.call("lsamd_fd_system_mg_build", self.systems[s]._h, _lib.stream())
+
+        self._threaded(build)
+        torch.cuda.synchronize()
+
+    def solve(self, blocks, rtol=1e-5, atol=0.0, maxiter=None):
+        """``blocks[s]``: FdRows of system s.  Returns (list of c, list of infos)."""
+        t0 = time.perf_counter()
+        self.prepare(blocks)
+        t1 = time.perf_counter()
+        out = solve_mg_batch(self.systems, rtol=rtol, atol=atol, maxiter=maxiter)
+        torch.cuda.synchronize()
+        self.last_timing = {"rows_and_hierarchies_s": t1 - t0,
+                            "batched_solve_s": time.perf_counter() - t1}
+        return out
+
+
 class EnsembleSolver:
     """Multigrid solves of the independent FD systems of an uneven ensemble,
     several at a time.
@@ -570,7 +665,7 @@ class EnsembleSolver:
     are the single-system multigrid solves: same stopping test and minimiser
     as ``FdSystem.solve(multigrid=True)``."""
 
-    def __init__(self, grids, reg: _lib.FdReg, workers: int = 4):
+    def __init__(self, grids, reg: _lib.FdReg, workers: int = 12):
         self.dev = _lib.device()
         self.grids = list(grids)
         self.reg = reg

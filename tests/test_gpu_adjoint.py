@@ -112,12 +112,15 @@ def test_evaluate_adjoint_is_transpose():
     """<E c, g> == <c, E^T g> for random c, g (E^T by the sort-based kernel)."""
     from loopstructural_amd.adjoint import FDEvaluateFunction
     from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
 
     g = _lib.make_grid((0, 0, 0), (0.1, 0.1, 0.1), (11, 11, 11))
+    w = {k: 0.1 for k in ("dxx", "dyy", "dzz", "dxy", "dxz", "dyz")}
+    system = fs.FdSystem(g, fs.reg_struct(w, (11, 11, 11), 1e-8))
     rng = np.random.default_rng(4)
     pts = torch.tensor(rng.uniform(-0.05, 1.05, (3000, 3)), device="cuda")
     c = torch.tensor(rng.standard_normal(1331), device="cuda", requires_grad=True)
-    v = FDEvaluateFunction.apply(c, g, pts)
+    v = FDEvaluateFunction.apply(c, system, pts)
     gv = torch.tensor(rng.standard_normal(3000), device="cuda")
     gv[torch.isnan(v)] = 0.0
     v = torch.nan_to_num(v)
@@ -125,3 +128,4 @@ def test_evaluate_adjoint_is_transpose():
     lhs = float((v * gv).sum())
     rhs = float((c.detach() * c.grad).sum())
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    system.close()

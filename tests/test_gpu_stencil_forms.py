@@ -1,6 +1,7 @@
 """Every form of the regular FD stencil gives the same bits: z-chunks 8, 16
-and 32 with register-staged planes and with planes copied straight into LDS
-(LSAMD_STENCIL_ZC pins the form per handle), on grids with partial xy tiles,
+and 32 with register-staged planes on 64 x 8 and 32 x 16 tiles and with
+planes copied straight into LDS (LSAMD_STENCIL_ZC pins the form per handle),
+on grids with partial xy tiles,
 short and long z extents, in FP32 and FP64; and the FP64 result equals the
 oracle's explicit normal operator to 1e-13."""
 import os
@@ -15,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 W = {"dxx": 0.1, "dyy": 0.1, "dzz": 0.1, "dxy": 0.1, "dxz": 0.1, "dyz": 0.1,
      "dx": 1.0, "dy": 1.0, "dz": 1.0}
-FORMS = (8, 16, 32, 1008, 1016, 1032)
+FORMS = (8, 16, 32, 1008, 1016, 1032, 2008, 2016, 2032)
 
 
 def _apply(nn, form, x):
