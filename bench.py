@@ -63,8 +63,12 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip ensemble, P1, evaluation and adjoint measurements")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run a 512 MiB copy and the stencil at 128^3 and 512^3 (the "
-                         "workload of the FETCH_SIZE / WRITE_SIZE counter runs)")
+                    help="only run a 512 MiB copy, the stencil at 128^3 and 512^3 and the "
+                         "256-system ensemble stencil on both tile widths (the workload of the "
+                         "FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--p1-probe", action="store_true",
+                    help="only time the P1 stencil at 129^3 and 257^3 nodes (every tuned form, "
+                         "the workload of the P1 kernel traces)")
     return ap.parse_args()
 
 
@@ -89,7 +93,45 @@ def traffic_probe():
             ms = s.time_stencil(prec, iters=3)
             out[f"{n1}:{prec}"] = {"avg_us": round(ms * 1e3, 2), **s.stencil_info(prec)}
         s.close()
+    # the 256-system ensemble (32^3..96^3 nodes), FP32 stencil on each tile width
+    from loopstructural_amd import synthetic
+
+    ens = synthetic.ensemble(256, 1004)
+    grids = [_lib.make_grid((0, 0, 0), (1.0 / (k - 1),) * 3, (k,) * 3) for k, _, _ in ens]
+    tot = sum(k ** 3 for k, _, _ in ens)
+    for tile in (64, 32):
+        os.environ["LSAMD_BATCH_TILE"] = str(tile)
+        bs = fs.BatchSystem(grids, fs.reg_struct(W, (32, 32, 32), 1e-8))
+        bs.solve(rtol=1e-5, maxiter=1, precision="fp32", check_every=1)  # fixes the tile width
+        ms = bs.time_stencil("fp32", iters=3)
+        out[f"ensemble:fp32:{tile}"] = {"avg_us": round(ms * 1e3, 2), "dof": tot,
+                                        "GBs": round(tot * 8 / (ms * 1e-3) / 1e9, 1)}
+        bs.close()
+        del os.environ["LSAMD_BATCH_TILE"]
     print(json.dumps({"traffic_probe": out}), flush=True)
+
+
+def p1_probe():
+    """The P1 edge-jump stencil at 129^3 and 257^3 nodes in FP64 and FP32:
+    the autotuner times every tile form, then 20 launches of the chosen one."""
+    import torch
+
+    from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
+
+    out = {}
+    for cells in (128, 256):
+        g = _lib.make_grid((0, 0, 0), (1 / cells,) * 3, (cells + 1,) * 3)
+        s = fs.P1System(g, 0.1, 1e-8)
+        n = (cells + 1) ** 3
+        for prec, sb in (("fp64", 8), ("fp32", 4)):
+            ms = s.time_stencil(prec, iters=20)
+            gbs = 2 * sb * n / (ms * 1e-3) / 1e9
+            out[f"{cells + 1}:{prec}"] = {"avg_us": round(ms * 1e3, 2), "GBs": round(gbs, 1),
+                                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        s.close()
+    torch.cuda.synchronize()
+    print(json.dumps({"p1_probe": out}), flush=True)
 
 
 def stencil_traffic(nn, prec, zc):
@@ -121,6 +163,9 @@ def main():
     args = parse()
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.p1_probe:
+        p1_probe()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched and "
