@@ -1886,6 +1886,18 @@ int ensure_batch_work(lsamd_fd_system* s) {
 // ------------------------------------------------------------ multigrid ----
 namespace lsamd {
 
+// device arrays of a sliced row set (Sell), row-dependent (pooled)
+struct SellBuf {
+  int64_t* off = nullptr;
+  int32_t* width = nullptr;
+  int32_t* col = nullptr;
+  float* val = nullptr;
+  const int32_t* node = nullptr;
+  int64_t nrows = 0;
+  int64_t nent = 0;  // entries with padding
+};
+inline Sell sell_dev(const SellBuf& b) { return Sell{b.off, b.width, b.col, b.val, b.node, b.nrows}; }
+
 struct MgLevel {
   lsamd_fd_system* sys = nullptr;  // level 0: the system itself (not owned)
   bool owned = false;
@@ -1909,6 +1921,11 @@ struct MgLevel {
   int64_t nblocks = 0;
   int32_t* cov = nullptr;  // nodes with CSR rows, listed on large levels (split smoother)
   int64_t ncov = 0;
+  // the multi-rhs kernels' copies of the smoother rows (of the covered list
+  // on split levels, else of every node, Jacobi nodes as one-entry rows) and
+  // of the Gram rows, in 64-row slices; built on the first multi-rhs solve
+  // after the rows change
+  SellBuf ssl, gsl;
 };
 
 struct Mg {
@@ -1929,6 +1946,7 @@ struct Mg {
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
   bool graph_stale = false;  // captured with addresses of an older row set
+  bool sell_ready = false;   // the levels' sliced rows match the row set
   // level-0 data term as a Gram matrix (CSR, FP32 and FP64 values), kept
   // for the batched solve's node-wise operator (row-dependent, pooled)
   int64_t* g0ptr = nullptr;
@@ -1973,7 +1991,10 @@ void mg_invalidate(Mg* m) {
     L.nblocks = 0;
     L.cov = nullptr;
     L.ncov = 0;
+    L.ssl = SellBuf();
+    L.gsl = SellBuf();
   }
+  m->sell_ready = false;
   m->stale = true;
 }
 
@@ -2416,6 +2437,43 @@ int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOu
     L.cov = cov;
     L.ncov = h;
   }
+  return LSAMD_OK;
+}
+
+// Sliced copy (Sell) of the CSR rows (ptr, col, val) of the nodes `node`
+// (nullptr: nodes 0 .. nrows-1); with tags sm, a node whose tag is >= 0 gets
+// the one-entry row (node, tag).
+int sell_build(Mg* m, const int64_t* ptr, const int32_t* col, const float* val, const float* sm,
+               const int32_t* node, int64_t nrows, SellBuf* out, hipStream_t st) {
+  *out = SellBuf();
+  if (nrows <= 0) return LSAMD_OK;
+  const int64_t ns = ceil_div(nrows, 64);
+  if (int rc = mg_dalloc(m, &out->width, ns)) return rc;
+  if (int rc = mg_dalloc(m, &out->off, ns)) return rc;
+  hipLaunchKernelGGL(k_sell_width, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, st, ptr, sm,
+                     node, nrows, out->width);
+  LSAMD_CHECK_LAUNCH("sliced rows: widths");
+  std::vector<int32_t> hw(ns);
+  std::vector<int64_t> ho(ns);
+  LSAMD_HIP(hipMemcpyAsync(hw.data(), out->width, ns * 4, hipMemcpyDeviceToHost, st));
+  LSAMD_HIP(hipStreamSynchronize(st));
+  int64_t tot = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    ho[i] = tot;
+    tot += 64 * (int64_t)hw[i];
+  }
+  if (tot >= ((int64_t)1 << 40)) return fail(LSAMD_ERR_ARG, "sliced rows: too many entries");
+  if (int rc = mg_dalloc(m, &out->col, tot)) return rc;
+  if (int rc = mg_dalloc(m, &out->val, tot)) return rc;
+  LSAMD_HIP(hipMemcpyAsync(out->off, ho.data(), ns * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)ceil_div(ns * 64, 256)), dim3(256), 0, st, ptr,
+                     col, val, sm, node, nrows, (const int64_t*)out->off,
+                     (const int32_t*)out->width, out->col, out->val);
+  LSAMD_CHECK_LAUNCH("sliced rows: fill");
+  LSAMD_HIP(hipStreamSynchronize(st));  // ho is read by the copy
+  out->node = node;
+  out->nrows = nrows;
+  out->nent = tot;
   return LSAMD_OK;
 }
 
@@ -3157,7 +3215,7 @@ struct MrWork {
   BSys* sysR = nullptr;
   int4* tilesR[2] = {nullptr, nullptr};
   int ntR[2] = {0, 0};
-  int zcR[2] = {0, 0};
+  int zcR[2] = {0, 0};  // stencil form per precision (kMrForm code)
   CgCtl* ctlR = nullptr;
   double* partR = nullptr;
   // coarse levels on regular grids (index = level; entries of level 0 and of
@@ -3165,7 +3223,7 @@ struct MrWork {
   // level's class table, the Gram rows added for all lanes after it
   std::vector<BSys*> lsys;
   std::vector<int4*> ltiles;
-  std::vector<int> lnt, lzc;
+  std::vector<int> lnt, lzc;  // work items and stencil form (kMrForm code) per level
   std::vector<void*> mem;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
@@ -3198,11 +3256,50 @@ int mr_alloc(MrWork* w, P** p, int64_t count) {
   return LSAMD_OK;
 }
 
+// stencil forms of the multi-rhs levels: z-chunk 8/16/32 on 64 x 8 tiles,
+// or the same chunks on 32 x 16 tiles (code 2000 + chunk)
+constexpr int kMrNForm = 6;
+constexpr int kMrForm[kMrNForm] = {8, 16, 32, 2008, 2016, 2032};
+inline int mr_form_zc(int form) { return form % 1000; }
+inline int mr_form_txt(int form) { return form >= 2000 ? 32 : 64; }
+
+template <typename T>
+void mr_form_launch(int form, unsigned nt, hipStream_t st, const T* x, T* y, const BSys* sys,
+                    const int4* tiles, const T* tab, const Coef25<T>& ku, double* part,
+                    const CgCtl* ctl) {
+  const dim3 g(nt), b(NT);
+  switch (form) {
+    case 8:
+      hipLaunchKernelGGL((k_b_tiles<T, 8>), g, b, 0, st, x, y, sys, tiles, tab, ku, part, ctl);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_b_tiles<T, 16>), g, b, 0, st, x, y, sys, tiles, tab, ku, part, ctl);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_b_tiles<T, 32>), g, b, 0, st, x, y, sys, tiles, tab, ku, part, ctl);
+      break;
+    case 2008:
+      hipLaunchKernelGGL((k_b_tiles<T, 8, false, 32>), g, b, 0, st, x, y, sys, tiles, tab, ku,
+                         part, ctl);
+      break;
+    case 2016:
+      hipLaunchKernelGGL((k_b_tiles<T, 16, false, 32>), g, b, 0, st, x, y, sys, tiles, tab, ku,
+                         part, ctl);
+      break;
+    default:
+      hipLaunchKernelGGL((k_b_tiles<T, 32, false, 32>), g, b, 0, st, x, y, sys, tiles, tab, ku,
+                         part, ctl);
+      break;
+  }
+}
+
 // lane vectors, controls and the level-0 stencil plan of R lanes for the
 // system's hierarchy
 template <typename T>
 void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st,
               const CgCtl* ctl = nullptr);
+void mr_level_tiles(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y,
+                    hipStream_t st);
 
 int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
   const Mg* m = s->mg;
@@ -3242,29 +3339,39 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
   }
   if (int rc = mr_alloc(w, &w->sysR, R)) return rc;
   LSAMD_HIP(legacy_memcpy(w->sysR, hs.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
-  // work-item tables of every z-chunk candidate; the R lanes make this level
-  // HBM-bound (unlike one cache-resident system), so the chunk is timed here
-  // per precision rather than taken from the single-system tuning
+  // work-item tables of every stencil form; the R lanes make level 0
+  // HBM-bound (unlike one cache-resident system), so the form is timed here
+  // per precision rather than taken from the single-system tuning, and each
+  // coarse level times its own (their axis lengths 2^k + 1 fit the two tile
+  // widths differently)
+  const char* pin = getenv("LSAMD_STENCIL_ZC");
+  const int pinned = pin ? atoi(pin) : 0;
+  auto eligible = [&](int form, const GridDev& g) {
+    if (pinned == 8 || pinned == 16 || pinned == 32) return mr_form_zc(form) == pinned;
+    return mr_form_zc(form) == 8 || g.nz - 6 >= mr_form_zc(form);
+  };
+  int4* tc[kMrNForm] = {};
+  int ntc[kMrNForm] = {};
   int64_t maxt = 1;
-  int4* tc[3] = {nullptr, nullptr, nullptr};
-  int ntc[3] = {0, 0, 0};
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kMrNForm; ++c) {
+    if (!eligible(kMrForm[c], s->gd)) continue;
     std::vector<int4> t;
-    for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, kZcCand[c], t);
+    for (int r = 0; r < R; ++r) stencil_tiles(r, s->gd, mr_form_zc(kMrForm[c]), t, mr_form_txt(kMrForm[c]));
     if (int rc = mr_alloc(w, &tc[c], (int64_t)t.size())) return rc;
     LSAMD_HIP(legacy_memcpy(tc[c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
     ntc[c] = (int)t.size();
     maxt = std::max<int64_t>(maxt, (int64_t)t.size());
   }
-  if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
-  LSAMD_HIP(legacy_memset(w->ctlR, 0, R * sizeof(CgCtl)));
-  if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
-  // coarse-level plans (the coarsest level is solved densely)
-  w->lsys.assign(m->lv.size(), nullptr);
-  w->ltiles.assign(m->lv.size(), nullptr);
-  w->lnt.assign(m->lv.size(), 0);
-  w->lzc.assign(m->lv.size(), 0);
-  for (size_t l = 1; l + 1 < m->lv.size(); ++l) {
+  // coarse-level tables of every eligible form (the coarsest level is solved
+  // densely); the best is kept after timing
+  const size_t nl = m->lv.size();
+  w->lsys.assign(nl, nullptr);
+  w->ltiles.assign(nl, nullptr);
+  w->lnt.assign(nl, 0);
+  w->lzc.assign(nl, 0);
+  std::vector<std::vector<int4*>> lt(nl, std::vector<int4*>(kMrNForm, nullptr));
+  std::vector<std::vector<int>> lnt(nl, std::vector<int>(kMrNForm, 0));
+  for (size_t l = 1; l + 1 < nl; ++l) {
     const MgLevel& L = m->lv[l];
     if (!L.sys->regular) continue;
     std::vector<BSys> ls(R);
@@ -3276,46 +3383,31 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
     }
     if (int rc = mr_alloc(w, &w->lsys[l], R)) return rc;
     LSAMD_HIP(legacy_memcpy(w->lsys[l], ls.data(), R * sizeof(BSys), hipMemcpyHostToDevice));
-    const int zc = L.g.nz - 6 >= 16 ? 16 : 8;
-    std::vector<int4> t;
-    for (int r = 0; r < R; ++r) stencil_tiles(r, L.g, zc, t);
-    if (int rc = mr_alloc(w, &w->ltiles[l], (int64_t)t.size())) return rc;
-    LSAMD_HIP(legacy_memcpy(w->ltiles[l], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
-    w->lnt[l] = (int)t.size();
-    w->lzc[l] = zc;
-    if ((int64_t)t.size() > maxt) {
-      LSAMD_HIP(hipFree(w->partR));
-      for (size_t i = 0; i < w->mem.size(); ++i)
-        if (w->mem[i] == (void*)w->partR) w->mem.erase(w->mem.begin() + i);
-      w->partR = nullptr;
-      maxt = (int64_t)t.size();
-      if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
+    for (int c = 0; c < kMrNForm; ++c) {
+      if (!eligible(kMrForm[c], L.g)) continue;
+      std::vector<int4> t;
+      for (int r = 0; r < R; ++r) stencil_tiles(r, L.g, mr_form_zc(kMrForm[c]), t, mr_form_txt(kMrForm[c]));
+      if (int rc = mr_alloc(w, &lt[l][c], (int64_t)t.size())) return rc;
+      LSAMD_HIP(legacy_memcpy(lt[l][c], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+      lnt[l][c] = (int)t.size();
+      maxt = std::max<int64_t>(maxt, (int64_t)t.size());
     }
   }
-  const char* pin = getenv("LSAMD_STENCIL_ZC");
-  const int pinned = pin ? atoi(pin) : 0;
+  if (int rc = mr_alloc(w, &w->ctlR, R)) return rc;
+  LSAMD_HIP(legacy_memset(w->ctlR, 0, R * sizeof(CgCtl)));
+  if (int rc = mr_alloc(w, &w->partR, maxt)) return rc;
   hipEvent_t e0, e1;
   LSAMD_HIP(hipEventCreate(&e0));
   LSAMD_HIP(hipEventCreate(&e1));
-  for (int p = 0; p < 2; ++p) {
+  // fastest of the forms with tables: run() launches the candidate set up
+  // by use(c)
+  auto tune = [&](int4* const* tabs, auto use, auto run) -> int {
     float best = 1e30f;
-    int bc = 1;
+    int bc = -1;
     for (int round = 0; round < 2; ++round)
-      for (int c = 0; c < 3; ++c) {
-        if (pinned == 8 || pinned == 16 || pinned == 32) {
-          if (kZcCand[c] != pinned) continue;
-        } else if (kZcCand[c] > 8 && s->gd.nz - 6 < kZcCand[c]) {
-          continue;
-        }
-        w->tilesR[p] = tc[c];
-        w->ntR[p] = ntc[c];
-        w->zcR[p] = kZcCand[c];
-        auto run = [&]() {
-          if (p)
-            mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], 0);
-          else
-            mr_tiles<double>(s, w, (const double*)w->p64, w->q64, 0);
-        };
+      for (int c = 0; c < kMrNForm; ++c) {
+        if (!tabs[c]) continue;
+        use(c);
         run();
         LSAMD_HIP(hipEventRecord(e0, 0));
         for (int k = 0; k < 3; ++k) run();
@@ -3328,9 +3420,40 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
           bc = c;
         }
       }
-    w->tilesR[p] = tc[bc];
-    w->ntR[p] = ntc[bc];
-    w->zcR[p] = kZcCand[bc];
+    return bc;
+  };
+  for (int p = 0; p < 2; ++p) {
+    auto use = [&](int c) {
+      w->tilesR[p] = tc[c];
+      w->ntR[p] = ntc[c];
+      w->zcR[p] = kMrForm[c];
+    };
+    const int bc = tune(tc, use, [&]() {
+      if (p)
+        mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], 0);
+      else
+        mr_tiles<double>(s, w, (const double*)w->p64, w->q64, 0);
+    });
+    if (bc < 0) return fail(LSAMD_ERR_ARG, "multi-rhs: no stencil form fits the grid");
+    use(bc);
+  }
+  for (size_t l = 1; l + 1 < nl; ++l) {
+    if (!w->lsys[l]) continue;
+    auto use = [&](int c) {
+      w->ltiles[l] = lt[l][c];
+      w->lnt[l] = lnt[l][c];
+      w->lzc[l] = kMrForm[c];
+    };
+    const int bc = tune(lt[l].data(), use,
+                        [&]() { mr_level_tiles(s, w, (int)l, w->x[l], w->q[l], 0); });
+    if (bc < 0) return fail(LSAMD_ERR_ARG, "multi-rhs: no stencil form fits a coarse grid");
+    use(bc);
+  }
+  if (getenv("LSAMD_MG_TIMING")) {
+    fprintf(stderr, "[multi-rhs stencil] R %d level 0 forms fp64 %d fp32 %d;", R, w->zcR[0],
+            w->zcR[1]);
+    for (size_t l = 1; l + 1 < nl; ++l) fprintf(stderr, " level %zu %d", l, w->lzc[l]);
+    fprintf(stderr, "\n");
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -3396,26 +3519,54 @@ void mr_tiles(lsamd_fd_system* s, MrWork* w, const T* x, T* y, hipStream_t st,
   const T* tab = p ? (const T*)s->tab32 : (const T*)s->tab64;
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = p ? (T)s->ku32.k[q] : (T)s->ku64.k[q];
-  const dim3 g((unsigned)w->ntR[p]), b(NT);
-  switch (w->zcR[p]) {
-    case 8:
-      hipLaunchKernelGGL((k_b_tiles<T, 8>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
-      break;
-    case 32:
-      hipLaunchKernelGGL((k_b_tiles<T, 32>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
-      break;
-    default:
-      hipLaunchKernelGGL((k_b_tiles<T, 16>), g, b, 0, st, x, y, (const BSys*)w->sysR,
-                         (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
-      break;
-  }
+  mr_form_launch<T>(w->zcR[p], (unsigned)w->ntR[p], st, x, y, (const BSys*)w->sysR,
+                    (const int4*)w->tilesR[p], tab, ku, w->partR, ctl);
+}
+
+// regulariser of coarse level l over the R lanes (its tuned form)
+void mr_level_tiles(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y,
+                    hipStream_t st) {
+  const lsamd_fd_system* ls = s->mg->lv[l].sys;
+  mr_form_launch<float>(w->lzc[l], (unsigned)w->lnt[l], st, x, y, (const BSys*)w->lsys[l],
+                        (const int4*)w->ltiles[l], (const float*)ls->tab32, ls->ku32, w->partR,
+                        (const CgCtl*)w->ctlR);
 }
 
 // coarse levels of at least this many nodes add their Gram term four lanes
 // per thread, smaller ones one lane per thread
 constexpr int64_t kMrGramGroup = 65536;
+
+// sliced smoother and Gram rows of every level (see MgLevel::ssl, gsl)
+int mr_sell_ensure(lsamd_fd_system* s, hipStream_t st) {
+  Mg* m = s->mg;
+  if (m->sell_ready) return LSAMD_OK;
+  const bool verbose = getenv("LSAMD_MG_TIMING") != nullptr;
+  for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
+    MgLevel& L = m->lv[l];
+    // levels with a covered list smooth those rows from CSR (k_mr_cheb_cov_lg
+    // measured faster there); the others get every node's row sliced
+    if (L.sptr && !L.cov)
+      if (int rc = sell_build(m, L.sptr, L.scol, L.sval, L.sm, nullptr, L.n, &L.ssl, st)) return rc;
+    if (l > 0 && L.gptr)
+      if (int rc = sell_build(m, L.gptr, L.gcol, L.gval, nullptr, nullptr, L.n, &L.gsl, st))
+        return rc;
+    if (verbose)
+      fprintf(stderr, "[multigrid sliced rows] level %zu: n %lld, smoother rows %lld (%lld entries), "
+              "gram rows %lld (%lld entries)\n", l, (long long)L.n, (long long)L.ssl.nrows,
+              (long long)L.ssl.nent, (long long)L.gsl.nrows, (long long)L.gsl.nent);
+  }
+  m->sell_ready = true;
+  return LSAMD_OK;
+}
+
+// lanes per thread of a sliced-row kernel over `rows` rows (environment
+// override: 1, 2, 4 or 8); at most R
+int mr_sell_lanes(const char* env, int64_t rows, int R, int64_t big) {
+  const char* e = getenv(env);
+  int k = e ? atoi(e) : (rows >= big ? 4 : 1);
+  if (k != 1 && k != 2 && k != 4 && k != 8) k = 1;
+  return std::min(k, R);
+}
 
 template <int R>
 void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hipStream_t st) {
@@ -3436,17 +3587,27 @@ void mr_apply(lsamd_fd_system* s, MrWork* w, int l, const float* x, float* y, hi
     return;
   }
   if (l < (int)w->ltiles.size() && w->ltiles[l]) {
-    const lsamd_fd_system* ls = L.sys;
-    const dim3 g((unsigned)w->lnt[l]), b(NT);
-    if (w->lzc[l] == 16)
-      hipLaunchKernelGGL((k_b_tiles<float, 16>), g, b, 0, st, x, y, (const BSys*)w->lsys[l],
-                         (const int4*)w->ltiles[l], (const float*)ls->tab32, ls->ku32, w->partR,
-                         (const CgCtl*)w->ctlR);
-    else
-      hipLaunchKernelGGL((k_b_tiles<float, 8>), g, b, 0, st, x, y, (const BSys*)w->lsys[l],
-                         (const int4*)w->ltiles[l], (const float*)ls->tab32, ls->ku32, w->partR,
-                         (const CgCtl*)w->ctlR);
-    if (L.gptr) {
+    mr_level_tiles(s, w, l, x, y, st);
+    if (L.gsl.nrows > 0) {
+      const Sell G = sell_dev(L.gsl);
+      const int lg = mr_sell_lanes("LSAMD_MR_GRAM_LG", L.gsl.nrows, R, 32768);
+      const int xcd = 1;  // XCD-contiguous workgroups (measured faster)
+      const dim3 gg((unsigned)ceil_div(L.gsl.nrows, 256), R / lg);
+      switch (lg) {
+        case 1:
+          hipLaunchKernelGGL(k_mr_gram_sell<1>, gg, b256, 0, st, G, L.n, x, y, ad, xcd);
+          break;
+        case 2:
+          hipLaunchKernelGGL(k_mr_gram_sell<2>, gg, b256, 0, st, G, L.n, x, y, ad, xcd);
+          break;
+        case 4:
+          hipLaunchKernelGGL(k_mr_gram_sell<4>, gg, b256, 0, st, G, L.n, x, y, ad, xcd);
+          break;
+        default:
+          hipLaunchKernelGGL(k_mr_gram_sell<8>, gg, b256, 0, st, G, L.n, x, y, ad, xcd);
+          break;
+      }
+    } else if (L.gptr) {
       if (L.n >= kMrGramGroup)
         hipLaunchKernelGGL(k_mr_gram_add<4>, dim3((unsigned)ceil_div(L.n, 256), R / 4), b256, 0,
                            st, (const int64_t*)L.gptr, (const int32_t*)L.gcol,
@@ -3490,8 +3651,42 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   const dim3 nb((unsigned)ceil_div(L.n, 256), R);
   const MgSm S = mg_sm(L);
   const int split = L.cov ? 1 : 0;
+  // XCD-contiguous workgroups where measured faster: the sliced smoother
+  // and the restriction; the covered-node step and the prolongation keep the
+  // hardware order
+  const int xcd = 1;
   const dim3 ncb((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)), R);
+  // the smoother rows of every node in slices (levels without a covered list)
+  auto step_sell = [&](int mode, float c1, float c2) {
+    const Sell P = sell_dev(L.ssl);
+    const int lg = mr_sell_lanes("LSAMD_MR_SM_LG", L.ssl.nrows, R, ((int64_t)1) << 40);
+    const dim3 gg((unsigned)ceil_div(L.ssl.nrows, 256), R / lg);
+    const float *bl = w->b[l], *ql = w->q[l];
+    float *xl = w->x[l], *dl = w->d[l];
+    switch (lg) {
+      case 1:
+        hipLaunchKernelGGL(k_mr_cheb_sell<1>, gg, b256, 0, st, P, bl, ql, xl, dl, L.n, mode, L.c0,
+                           c1, c2, ad, xcd);
+        break;
+      case 2:
+        hipLaunchKernelGGL(k_mr_cheb_sell<2>, gg, b256, 0, st, P, bl, ql, xl, dl, L.n, mode, L.c0,
+                           c1, c2, ad, xcd);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_mr_cheb_sell<4>, gg, b256, 0, st, P, bl, ql, xl, dl, L.n, mode, L.c0,
+                           c1, c2, ad, xcd);
+        break;
+      default:
+        hipLaunchKernelGGL(k_mr_cheb_sell<8>, gg, b256, 0, st, P, bl, ql, xl, dl, L.n, mode, L.c0,
+                           c1, c2, ad, xcd);
+        break;
+    }
+  };
   auto step = [&](int mode, float c1, float c2) {
+    if (!split && L.ssl.nrows > 0) {
+      step_sell(mode, c1, c2);
+      return;
+    }
     hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
                        w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
     if (split && L.ncov > 0) {
@@ -3506,15 +3701,15 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
           break;
         case 2:
           hipLaunchKernelGGL(k_mr_cheb_cov_lg<2>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
-                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad, 0);
           break;
         case 8:
           hipLaunchKernelGGL(k_mr_cheb_cov_lg<8>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
-                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad, 0);
           break;
         default:
           hipLaunchKernelGGL(k_mr_cheb_cov_lg<4>, ncg, b256, 0, st, cv, L.ncov, bl, ql, S,
-                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad);
+                             w->x[l], w->d[l], L.n, mode, L.c0, c1, c2, ad, 0);
           break;
       }
     }
@@ -3533,10 +3728,10 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   constexpr int TG = 4;  // lanes per thread of the transfers
   static_assert(R % TG == 0, "lane groups");
   hipLaunchKernelGGL(k_mr_restrict_lg<TG>, dim3((unsigned)ceil_div(C.n, 256), R / TG), b256, 0, st,
-                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, ad);
+                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, ad, xcd);
   mr_vcycle<R>(s, w, l + 1, st);
   hipLaunchKernelGGL(k_mr_prolong_lg<TG>, dim3((unsigned)ceil_div(L.n, 256), R / TG), b256, 0, st,
-                     (const float*)w->x[l + 1], C.g, w->x[l], L.g, ad);
+                     (const float*)w->x[l + 1], C.g, w->x[l], L.g, ad, 0);
   mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
   smooth(0);
 }
@@ -3636,6 +3831,7 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
              double* c, lsamd_solve_info* infos, hipStream_t st) {
   if (s->op != 0) return fail(LSAMD_ERR_ARG, "multi-rhs multigrid: FD systems only");
   if (int rc = mg_ensure(s, st)) return rc;
+  if (int rc = mr_sell_ensure(s, st)) return rc;
   const Mg* m = s->mg;
   bool lanes = s->regular && s->tab32 && s->tab64;
   for (size_t l = 1; l + 1 < m->lv.size(); ++l) lanes = lanes && m->lv[l].sys->regular;

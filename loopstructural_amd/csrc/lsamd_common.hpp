@@ -95,6 +95,23 @@ __device__ inline double block_sum(double v, double* sh) {
   return s;
 }
 
+// Workgroup index renumbered so that each XCD takes one contiguous run of
+// the grid: gfx950 hands workgroups to its 8 XCDs round-robin by linear id,
+// so neighbouring workgroups (which gather from overlapping neighbourhoods)
+// otherwise sit on different XCDs and each XCD's L2 fetches the shared lines
+// again.  A bijection of [0, gridDim.x * gridDim.y); on = 0 keeps blockIdx.
+__device__ inline void xcd_block(int on, unsigned& bx, unsigned& by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (!on) return;
+  const unsigned gx = gridDim.x, t = gx * gridDim.y;
+  const unsigned l = blockIdx.y * gx + blockIdx.x;
+  const unsigned q = t / 8, rem = t % 8, k = l % 8;
+  const unsigned lp = k * q + (k < rem ? k : rem) + l / 8;
+  bx = lp % gx;
+  by = lp / gx;
+}
+
 // Device-coherent load of a value written by another workgroup.
 __device__ inline double coherent_load(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

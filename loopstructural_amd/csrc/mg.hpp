@@ -723,6 +723,84 @@ __global__ void k_blk_tags(const int64_t* __restrict__ sptr, const float* __rest
   sm[i] = sptr[i + 1] > sptr[i] ? -1.f : minv[i];
 }
 
+// ------------------------------------------------- sliced rows (SELL-64) ----
+// A sparse row set in slices of 64 rows, one wavefront each: entry k of the
+// slice's row t at off[s] + 64 k + t (columns and values), slice width
+// width[s] (the longest row of the slice, rounded up to a multiple of 4).
+// The 64 threads of a wavefront read one entry index of their 64 rows as one
+// contiguous 256-byte line, where CSR with a thread per row touches 64
+// lines.  Padding entries point at the row's own node with value 0.  Row r
+// is node node[r] (node == nullptr: node r).
+struct Sell {
+  const int64_t* off;
+  const int32_t* width;
+  const int32_t* col;
+  const float* val;
+  const int32_t* node;
+  int64_t nrows;
+};
+
+// row r of a CSR (ptr, with tags sm when given: a row whose tag is >= 0 is
+// the single entry (node, tag), the smoother's point-Jacobi nodes)
+__device__ inline int64_t sell_src_len(const int64_t* __restrict__ ptr,
+                                       const float* __restrict__ sm, int64_t i) {
+  if (sm && sm[i] >= 0.f) return 1;
+  return ptr[i + 1] - ptr[i];
+}
+
+__global__ void k_sell_width(const int64_t* __restrict__ ptr, const float* __restrict__ sm,
+                             const int32_t* __restrict__ node, int64_t nrows,
+                             int32_t* __restrict__ width) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s * 64 >= nrows) return;
+  int64_t w = 0;
+  const int64_t r1 = s * 64 + 64 < nrows ? s * 64 + 64 : nrows;
+  for (int64_t r = s * 64; r < r1; ++r) {
+    const int64_t l = sell_src_len(ptr, sm, node ? node[r] : r);
+    w = l > w ? l : w;
+  }
+  width[s] = (int32_t)((w + 3) & ~(int64_t)3);
+}
+
+__global__ void k_sell_fill(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const float* __restrict__ val, const float* __restrict__ sm,
+                            const int32_t* __restrict__ node, int64_t nrows,
+                            const int64_t* __restrict__ off, const int32_t* __restrict__ width,
+                            int32_t* __restrict__ ocol, float* __restrict__ oval) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = r >> 6;
+  if (s * 64 >= nrows) return;
+  const int t = (int)(r & 63);
+  const int w = width[s];
+  const int64_t o = off[s] + t;
+  if (r >= nrows) {
+    for (int k = 0; k < w; ++k) {
+      ocol[o + 64 * (int64_t)k] = 0;
+      oval[o + 64 * (int64_t)k] = 0.f;
+    }
+    return;
+  }
+  const int64_t i = node ? node[r] : r;
+  int k = 0;
+  if (sm && sm[i] >= 0.f) {
+    if (w > 0) {
+      ocol[o] = (int32_t)i;
+      oval[o] = sm[i];
+      k = 1;
+    }
+  } else {
+    const int64_t e0 = ptr[i], len = ptr[i + 1] - e0;
+    for (; k < len; ++k) {
+      ocol[o + 64 * (int64_t)k] = col[e0 + k];
+      oval[o + 64 * (int64_t)k] = val[e0 + k];
+    }
+  }
+  for (; k < w; ++k) {
+    ocol[o + 64 * (int64_t)k] = (int32_t)i;
+    oval[o + 64 * (int64_t)k] = 0.f;
+  }
+}
+
 // data rows of level 0 in fixed width wc (repeated columns merged, padded
 // with zero coefficients on the first column), for the level's Gram matrix
 __global__ void k_rows_fixed(const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,

@@ -211,6 +211,44 @@ __global__ __launch_bounds__(256) void k_mr_gram_add(const int64_t* __restrict__
   for (int l = 0; l < LG; ++l) y[(r0 + l) * n + id] += acc[l];
 }
 
+// y += G x with G's rows sliced (SELL-64: the column and value loads of a
+// wavefront are contiguous), LG lanes per thread (lane group in blockIdx.y).
+// A slice without entries leaves its nodes' y untouched.
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_gram_sell(Sell G, int64_t n,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ y,
+                                                      const int* __restrict__ all_done,
+                                                      int xcd) {
+  if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(xcd, bx, by);
+  const int64_t r = (int64_t)bx * blockDim.x + threadIdx.x;
+  if (r >= G.nrows) return;
+  const int64_t s = r >> 6;
+  const int w = G.width[s];
+  if (w == 0) return;
+  const int64_t o = G.off[s] + (r & 63);
+  const int64_t r0 = (int64_t)by * LG;
+  const float* xl = x + r0 * n;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int k = 0; k < w; k += 4) {
+    const int64_t e = o + 64 * (int64_t)k;
+    const int32_t j0 = G.col[e], j1 = G.col[e + 64], j2 = G.col[e + 128], j3 = G.col[e + 192];
+    const float a0 = G.val[e], a1 = G.val[e + 64], a2 = G.val[e + 128], a3 = G.val[e + 192];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const float* xx = xl + l * n;
+      acc[l] += a0 * xx[j0] + a1 * xx[j1] + a2 * xx[j2] + a3 * xx[j3];
+    }
+  }
+  const int64_t i = G.node ? G.node[r] : r;
+#pragma unroll
+  for (int l = 0; l < LG; ++l) y[(r0 + l) * n + i] += acc[l];
+}
+
 // ----------------------------------------------------------- smoother ----
 // Chebyshev step with the Schwarz preconditioner M: mode 0 first step from
 // zero (d = c0 M b, x = d), 1 first step (d = c0 M (b - q), x += d), 2 later
@@ -267,12 +305,14 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restric
                                                         float* __restrict__ x,
                                                         float* __restrict__ d, int64_t n, int mode,
                                                         float c0, float c1, float c2,
-                                                        const int* __restrict__ all_done) {
+                                                        const int* __restrict__ all_done, int xcd) {
   if (*all_done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned bx, by;
+  xcd_block(xcd, bx, by);
+  const int64_t t = (int64_t)bx * blockDim.x + threadIdx.x;
   if (t >= ncov) return;
   const int64_t i = cov[t];
-  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  const int64_t r0 = (int64_t)by * LG;
   const float* bl = b + r0 * n;
   const float* ql = q + r0 * n;
   const int m = mode & 3;
@@ -308,6 +348,54 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_lg(const int32_t* __restric
   }
 }
 
+// the same step over the rows of the sliced smoother matrix P (rows of the
+// covered nodes, or of every node with the Jacobi scale as a one-entry row),
+// LG lanes per thread (lane group in blockIdx.y)
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_cheb_sell(Sell P, const float* __restrict__ b,
+                                                      const float* __restrict__ q,
+                                                      float* __restrict__ x,
+                                                      float* __restrict__ d, int64_t n, int mode,
+                                                      float c0, float c1, float c2,
+                                                      const int* __restrict__ all_done, int xcd) {
+  if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(xcd, bx, by);
+  const int64_t r = (int64_t)bx * blockDim.x + threadIdx.x;
+  if (r >= P.nrows) return;
+  const int64_t s = r >> 6;
+  const int w = P.width[s];
+  const int64_t o = P.off[s] + (r & 63);
+  const int64_t r0 = (int64_t)by * LG;
+  const float* bl = b + r0 * n;
+  const float* ql = q + r0 * n;
+  const int m = mode & 3;
+  const int fz = m == 0;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int k = 0; k < w; k += 4) {
+    const int64_t e = o + 64 * (int64_t)k;
+    const int32_t j0 = P.col[e], j1 = P.col[e + 64], j2 = P.col[e + 128], j3 = P.col[e + 192];
+    const float a0 = P.val[e], a1 = P.val[e + 64], a2 = P.val[e + 128], a3 = P.val[e + 192];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const float* bb = bl + l * n;
+      const float* qq = ql + l * n;
+      acc[l] += a0 * mg_r(bb, qq, j0, fz) + a1 * mg_r(bb, qq, j1, fz) +
+                a2 * mg_r(bb, qq, j2, fz) + a3 * mg_r(bb, qq, j3, fz);
+    }
+  }
+  const int64_t i = P.node ? P.node[r] : r;
+#pragma unroll
+  for (int l = 0; l < LG; ++l) {
+    const int64_t el = (r0 + l) * n + i;
+    const float dv = m == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
+    if (!(mode & kMrLast)) d[el] = dv;
+    x[el] = m == 0 ? dv : x[el] + dv;
+  }
+}
+
 // ------------------------------------------------- transfer, coarsest ----
 // Restriction and prolongation with LG lanes per thread (lane group in
 // blockIdx.y): the node's index arithmetic is done once for the LG lanes;
@@ -316,13 +404,16 @@ template <int LG>
 __global__ __launch_bounds__(256) void k_mr_restrict_lg(const float* __restrict__ bf,
                                                         const float* __restrict__ qf, GridDev gf,
                                                         float* __restrict__ bc, GridDev gc,
-                                                        const int* __restrict__ all_done) {
+                                                        const int* __restrict__ all_done,
+                                                        int xcd) {
   if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(xcd, bx, by);
   const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
   const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t id = (int64_t)bx * blockDim.x + threadIdx.x;
   if (id >= nc) return;
-  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  const int64_t r0 = (int64_t)by * LG;
   int I, J, K;
   mg_ijk(id, gc.nx, gc.ny, I, J, K);
   const int64_t fnxy = (int64_t)gf.nx * gf.ny;
@@ -357,13 +448,16 @@ __global__ __launch_bounds__(256) void k_mr_restrict_lg(const float* __restrict_
 template <int LG>
 __global__ __launch_bounds__(256) void k_mr_prolong_lg(const float* __restrict__ xc, GridDev gc,
                                                        float* __restrict__ xf, GridDev gf,
-                                                       const int* __restrict__ all_done) {
+                                                       const int* __restrict__ all_done,
+                                                       int xcd) {
   if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(xcd, bx, by);
   const int64_t nc = (int64_t)gc.nx * gc.ny * gc.nz;
   const int64_t nf = (int64_t)gf.nx * gf.ny * gf.nz;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t id = (int64_t)bx * blockDim.x + threadIdx.x;
   if (id >= nf) return;
-  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  const int64_t r0 = (int64_t)by * LG;
   int i, j, k;
   mg_ijk(id, gf.nx, gf.ny, i, j, k);
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
