@@ -1926,6 +1926,10 @@ struct MgLevel {
   // of the Gram rows, in 64-row slices; built on the first multi-rhs solve
   // after the rows change
   SellBuf ssl, gsl;
+  // levels with a covered list: the covered rows sliced, their columns as
+  // covered-list positions (the multi-rhs step gathers an interleaved copy
+  // of r over the list)
+  SellBuf csl;
 };
 
 struct Mg {
@@ -1993,6 +1997,7 @@ void mg_invalidate(Mg* m) {
     L.ncov = 0;
     L.ssl = SellBuf();
     L.gsl = SellBuf();
+    L.csl = SellBuf();
   }
   m->sell_ready = false;
   m->stale = true;
@@ -3224,6 +3229,10 @@ struct MrWork {
   std::vector<BSys*> lsys;
   std::vector<int4*> ltiles;
   std::vector<int> lnt, lzc;  // work items and stencil form (kMrForm code) per level
+  // per level with a covered list: interleaved r over the list (R floats per
+  // covered node), capacity in floats
+  std::vector<float*> rint;
+  std::vector<int64_t> rint_cap;
   std::vector<void*> mem;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
@@ -3543,19 +3552,69 @@ int mr_sell_ensure(lsamd_fd_system* s, hipStream_t st) {
   const bool verbose = getenv("LSAMD_MG_TIMING") != nullptr;
   for (size_t l = 0; l + 1 < m->lv.size(); ++l) {
     MgLevel& L = m->lv[l];
-    // levels with a covered list smooth those rows from CSR (k_mr_cheb_cov_lg
-    // measured faster there); the others get every node's row sliced
+    // levels with a covered list: the covered rows over list positions;
+    // the others: every node's row
     if (L.sptr && !L.cov)
       if (int rc = sell_build(m, L.sptr, L.scol, L.sval, L.sm, nullptr, L.n, &L.ssl, st)) return rc;
+    if (L.sptr && L.cov && L.ncov > 0) {
+      if (int rc = sell_build(m, L.sptr, L.scol, L.sval, nullptr, L.cov, L.ncov, &L.csl, st))
+        return rc;
+      int32_t* pos = nullptr;
+      int* bad = nullptr;
+      if (int rc = mg_dalloc(m, &pos, L.n)) return rc;
+      if (int rc = mg_dalloc(m, &bad, 1)) return rc;
+      LSAMD_HIP(hipMemsetAsync(pos, 0xff, L.n * 4, st));
+      LSAMD_HIP(hipMemsetAsync(bad, 0, 4, st));
+      hipLaunchKernelGGL(k_cov_pos, dim3((unsigned)ceil_div(L.ncov, 256)), dim3(256), 0, st,
+                         (const int32_t*)L.cov, L.ncov, pos);
+      hipLaunchKernelGGL(k_sell_to_pos, dim3((unsigned)ceil_div(L.csl.nent, 256)), dim3(256), 0,
+                         st, L.csl.col, L.csl.nent, (const int32_t*)pos, bad);
+      LSAMD_CHECK_LAUNCH("covered rows over list positions");
+      int hbad = 0;
+      LSAMD_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+      LSAMD_HIP(hipStreamSynchronize(st));
+      mg_give(m, pos);
+      mg_give(m, bad);
+      if (hbad) return fail(LSAMD_ERR_ARG, "multigrid: a covered row reaches an uncovered node");
+    }
     if (l > 0 && L.gptr)
       if (int rc = sell_build(m, L.gptr, L.gcol, L.gval, nullptr, nullptr, L.n, &L.gsl, st))
         return rc;
     if (verbose)
       fprintf(stderr, "[multigrid sliced rows] level %zu: n %lld, smoother rows %lld (%lld entries), "
-              "gram rows %lld (%lld entries)\n", l, (long long)L.n, (long long)L.ssl.nrows,
-              (long long)L.ssl.nent, (long long)L.gsl.nrows, (long long)L.gsl.nent);
+              "covered rows %lld (%lld entries), gram rows %lld (%lld entries)\n", l,
+              (long long)L.n, (long long)L.ssl.nrows, (long long)L.ssl.nent,
+              (long long)L.csl.nrows, (long long)L.csl.nent, (long long)L.gsl.nrows,
+              (long long)L.gsl.nent);
   }
   m->sell_ready = true;
+  return LSAMD_OK;
+}
+
+// interleaved r buffers of the levels with covered rows, sized for the
+// current covered lists (a larger list reallocates and drops the captured
+// iteration, which holds the old addresses)
+int mr_rint_ensure(lsamd_fd_system* s, MrWork* w) {
+  const Mg* m = s->mg;
+  w->rint.resize(m->lv.size(), nullptr);
+  w->rint_cap.resize(m->lv.size(), 0);
+  for (size_t l = 0; l < m->lv.size(); ++l) {
+    const int64_t need = m->lv[l].csl.nrows * w->R;
+    if (need == 0 || w->rint_cap[l] >= need) continue;
+    mr_invalidate(w);
+    if (float* old = w->rint[l]) {
+      for (size_t i = 0; i < w->mem.size(); ++i)
+        if (w->mem[i] == (void*)old) {
+          hipFree(old);
+          w->mem.erase(w->mem.begin() + i);
+          break;
+        }
+    }
+    w->rint[l] = nullptr;
+    w->rint_cap[l] = 0;
+    if (int rc = mr_alloc(w, &w->rint[l], need)) return rc;
+    w->rint_cap[l] = need;
+  }
   return LSAMD_OK;
 }
 
@@ -3689,7 +3748,24 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
     }
     hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
                        w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
-    if (split && L.ncov > 0) {
+    if (split && L.csl.nrows > 0 && l < (int)w->rint.size() && w->rint[l]) {
+      hipLaunchKernelGGL(k_mr_cov_pack<R>, dim3((unsigned)ceil_div(L.ncov, 64)), b256, 0, st,
+                         (const int32_t*)L.cov, L.ncov, (const float*)w->b[l],
+                         (const float*)w->q[l], L.n, (mode & 3) == 0 ? 1 : 0, w->rint[l], ad);
+      const Sell P = sell_dev(L.csl);
+      const unsigned nbc = (unsigned)ceil_div(L.csl.nrows, 256);
+      const char* cl = getenv("LSAMD_MR_COVINT_LG");
+      if (R == 32 && cl && atoi(cl) == 32) {
+        hipLaunchKernelGGL((k_mr_cheb_cov_int<R, R>), dim3(nbc, 1), b256, 0, st, P,
+                           (const float*)w->rint[l], w->x[l], w->d[l], L.n, mode, L.c0, c1, c2,
+                           ad);
+      } else {
+        constexpr int LG = R < 16 ? R : 16;
+        hipLaunchKernelGGL((k_mr_cheb_cov_int<R, LG>), dim3(nbc, R / LG), b256, 0, st, P,
+                           (const float*)w->rint[l], w->x[l], w->d[l], L.n, mode, L.c0, c1, c2,
+                           ad);
+      }
+    } else if (split && L.ncov > 0) {
       const int lg = mr_cov_lanes(R);
       const dim3 ncg((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)), R / lg);
       const int32_t* cv = (const int32_t*)L.cov;
@@ -3743,11 +3819,12 @@ void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
   const int* ad = w->all_done;
   const dim3 b256(256), fin(mr_fin_threads(R));
   mr_vcycle<R>(s, w, 0, st);  // from w->b[0] = (float) r
-  hipLaunchKernelGGL(k_mr_dots<R>, dim3(vb), b256, 0, st, (const float*)w->x[0],
+  const dim3 vbg(vb, R / mr_outer_lanes(R));  // lane groups of the vector kernels
+  hipLaunchKernelGGL(k_mr_dots<R>, vbg, b256, 0, st, (const float*)w->x[0],
                      (const double*)w->r64, (const double*)w->q64, n, w->part, ad);
   hipLaunchKernelGGL(k_mr_dots_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
                      ad);
-  hipLaunchKernelGGL(k_mr_dir<R>, dim3(vb), b256, 0, st, (const float*)w->x[0], w->p64, n,
+  hipLaunchKernelGGL(k_mr_dir<R>, vbg, b256, 0, st, (const float*)w->x[0], w->p64, n,
                      (const MrCtl*)w->ctl, ad);
   // q = N p; p.q from the stencil's per-work-item partials and the rows'
   // per-block partials
@@ -3763,8 +3840,9 @@ void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
                        Rw, (const double*)w->t64, w->q64, n, R, ad);
   hipLaunchKernelGGL(k_mr_alpha_st<R>, dim3(1), fin, 0, st, (const double*)w->partR,
                      w->ntR[0] / R, (const double*)w->upart, nbu, w->ctl, ad);
-  hipLaunchKernelGGL(k_mr_upd<R>, dim3(vb), b256, 0, st, w->x64, w->r64, (const double*)w->p64, (const double*)w->q64, (const double*)s->minv64,
-                     w->b[0], n, w->part, (const MrCtl*)w->ctl, ad);
+  hipLaunchKernelGGL(k_mr_upd<R>, vbg, b256, 0, st, w->x64, w->r64, (const double*)w->p64,
+                     (const double*)w->q64, (const double*)s->minv64, w->b[0], n, w->part,
+                     (const MrCtl*)w->ctl, ad);
   hipLaunchKernelGGL(k_mr_upd_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
                      w->all_done);
 }
@@ -3850,6 +3928,7 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
     const int nr = std::min(left, R);
     MrWork* w = nullptr;
     if (int rc = mr_ensure(s, R, &w)) return rc;
+    if (int rc = mr_rint_ensure(s, w)) return rc;
     const double* rr = rhs + (int64_t)r0 * n;
     double* cc = c + (int64_t)r0 * n;
     const int rc = R == 4    ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
@@ -4635,7 +4714,8 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
                          (const float*)w->b[0], (const float*)w->q[0], mg_sm(L), w->x[0],
                          w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
     else if (kernel == 1)
-      hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n)), dim3(256), 0, st, w->x64, w->r64,
+      hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n), R / mr_outer_lanes(R)), dim3(256), 0,
+                         st, w->x64, w->r64,
                          (const double*)w->p64, (const double*)w->q64,
                          (const double*)s->minv64, w->b[0], n, w->part, (const MrCtl*)w->ctl,
                          zflag);

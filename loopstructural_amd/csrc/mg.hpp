@@ -801,6 +801,26 @@ __global__ void k_sell_fill(const int64_t* __restrict__ ptr, const int32_t* __re
   }
 }
 
+// pos[cov[t]] = t (pos initialised to -1)
+__global__ void k_cov_pos(const int32_t* __restrict__ cov, int64_t ncov, int32_t* __restrict__ pos) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ncov) pos[cov[t]] = (int32_t)t;
+}
+// sliced columns (node ids) to covered-list positions; *bad = 1 for a node
+// outside the list
+__global__ void k_sell_to_pos(int32_t* __restrict__ col, int64_t nent,
+                              const int32_t* __restrict__ pos, int* __restrict__ bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int32_t p = pos[col[e]];
+  if (p < 0) {
+    *bad = 1;
+    col[e] = 0;
+  } else {
+    col[e] = p;
+  }
+}
+
 // data rows of level 0 in fixed width wc (repeated columns merged, padded
 // with zero coefficients on the first column), for the level's Gram matrix
 __global__ void k_rows_fixed(const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,

@@ -396,6 +396,81 @@ __global__ __launch_bounds__(256) void k_mr_cheb_sell(Sell P, const float* __res
   }
 }
 
+// Covered nodes of a large level, interleaved: ri[t R + l] = r of lane l at
+// the t-th covered node (r = b, or b - q), so that a covered row's gather
+// reads one contiguous run of R values per column instead of R lines
+// n apart (lane-major gathers over R lanes overflow the L2 and re-fetch the
+// same lines many times).  64 covered nodes per block, transposed in LDS so
+// that both the lane-major reads and the interleaved writes are coalesced.
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_cov_pack(const int32_t* __restrict__ cov,
+                                                     int64_t ncov, const float* __restrict__ b,
+                                                     const float* __restrict__ q, int64_t n,
+                                                     int from_zero, float* __restrict__ ri,
+                                                     const int* __restrict__ all_done) {
+  __shared__ float tile[64][R + 1];
+  if (*all_done) return;
+  const int64_t t0 = (int64_t)blockIdx.x * 64;
+  const int tl = threadIdx.x & 63, lg = threadIdx.x >> 6;
+  const int64_t t = t0 + tl;
+  if (t < ncov) {
+    const int64_t i = cov[t];
+    for (int l = lg; l < R; l += 4) {
+      const int64_t e = (int64_t)l * n + i;
+      tile[tl][l] = from_zero ? b[e] : b[e] - q[e];
+    }
+  }
+  __syncthreads();
+  const int64_t cnt = ncov - t0 < 64 ? ncov - t0 : 64;
+  for (int k = threadIdx.x; k < cnt * R; k += 256) ri[t0 * R + k] = tile[k / R][k % R];
+}
+
+// Chebyshev step of the covered nodes from the interleaved r: sliced rows
+// over the covered list whose columns are covered-list positions; LG lanes
+// per thread (lane group in blockIdx.y), read as LG / 4 float4 per column.
+template <int R, int LG>
+__global__ __launch_bounds__(256) void k_mr_cheb_cov_int(Sell P, const float* __restrict__ ri,
+                                                         float* __restrict__ x,
+                                                         float* __restrict__ d, int64_t n,
+                                                         int mode, float c0, float c1, float c2,
+                                                         const int* __restrict__ all_done) {
+  static_assert(LG % 4 == 0 && R % LG == 0, "lane groups of float4");
+  if (*all_done) return;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.nrows) return;
+  const int64_t s = r >> 6;
+  const int w = P.width[s];
+  const int64_t o = P.off[s] + (r & 63);
+  const int l0 = blockIdx.y * LG;
+  const int m = mode & 3;
+  float acc[LG];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) acc[l] = 0.f;
+  for (int k = 0; k < w; k += 2) {
+    const int64_t e = o + 64 * (int64_t)k;
+    const int32_t h0 = P.col[e], h1 = P.col[e + 64];
+    const float a0 = P.val[e], a1 = P.val[e + 64];
+    const float4* v0 = (const float4*)(ri + (int64_t)h0 * R + l0);
+    const float4* v1 = (const float4*)(ri + (int64_t)h1 * R + l0);
+#pragma unroll
+    for (int g = 0; g < LG / 4; ++g) {
+      const float4 p = v0[g], u = v1[g];
+      acc[4 * g] += a0 * p.x + a1 * u.x;
+      acc[4 * g + 1] += a0 * p.y + a1 * u.y;
+      acc[4 * g + 2] += a0 * p.z + a1 * u.z;
+      acc[4 * g + 3] += a0 * p.w + a1 * u.w;
+    }
+  }
+  const int64_t i = P.node[r];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) {
+    const int64_t el = (int64_t)(l0 + l) * n + i;
+    const float dv = m == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
+    if (!(mode & kMrLast)) d[el] = dv;
+    x[el] = m == 0 ? dv : x[el] + dv;
+  }
+}
+
 // ------------------------------------------------- transfer, coarsest ----
 // Restriction and prolongation with LG lanes per thread (lane group in
 // blockIdx.y): the node's index arithmetic is done once for the LG lanes;
@@ -511,6 +586,31 @@ __device__ inline void mr_block_partials(const double* v, double* part) {
     part[(int64_t)k * gridDim.x + blockIdx.x] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
   }
 }
+// the same for the G lanes lane0 .. lane0 + G - 1 of an R-lane group:
+// v[k G + j] is product k of lane lane0 + j, stored where mr_block_partials
+// would store it
+template <int G, int NP>
+__device__ inline void mr_block_partials_g(const double* v, double* part, int R, int lane0) {
+  __shared__ double sh[4][NP * G];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NP * G; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) sh[wid][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP * G) {
+    const int k = threadIdx.x, slot = k / G, j = k - slot * G;
+    part[(int64_t)(slot * R + lane0 + j) * gridDim.x + blockIdx.x] =
+        ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+  }
+}
+
+// lanes per thread of the outer-iteration vector kernels (lane group in
+// blockIdx.y): few enough that the per-lane scalars and sums leave room for
+// eight wavefronts per SIMD
+constexpr int mr_outer_lanes(int R) { return R < 8 ? R : 8; }
+
 // Sum of the nblk partials of product k of lane r, by one wavefront (fixed
 // order: four strided lane sums, then the butterfly); result in every lane.
 template <int R>
@@ -594,17 +694,20 @@ __global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
                                                  double* __restrict__ part,
                                                  const int* __restrict__ all_done) {
   if (*all_done) return;
-  double acc[2 * R];
+  constexpr int G = mr_outer_lanes(R);
+  const int r0 = blockIdx.y * G;
+  double acc[2 * G];
 #pragma unroll
-  for (int r = 0; r < 2 * R; ++r) acc[r] = 0.0;
+  for (int r = 0; r < 2 * G; ++r) acc[r] = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double zv = (double)z[r * n + i];
-      acc[r] += zv * rr[r * n + i];
-      acc[R + r] += zv * q[r * n + i];
+    for (int j = 0; j < G; ++j) {
+      const int64_t e = (r0 + j) * n + i;
+      const double zv = (double)z[e];
+      acc[j] += zv * rr[e];
+      acc[G + j] += zv * q[e];
     }
-  mr_block_partials<R, 2>(acc, part);
+  mr_block_partials_g<G, 2>(acc, part, R, r0);
 }
 
 // rho = z.r ; beta = (z.r - z.r_prev) / rho_prev = -alpha_prev z.q / rho_prev
@@ -630,18 +733,20 @@ template <int R>
 __global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, int64_t n,
                          const MrCtl* __restrict__ ctl, const int* __restrict__ all_done) {
   if (*all_done) return;
-  double beta[R];
-  bool first[R];
+  constexpr int G = mr_outer_lanes(R);
+  const int r0 = blockIdx.y * G;
+  double beta[G];
+  bool first[G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    beta[r] = ctl[r].beta;
-    first[r] = ctl[r].iter == 0;
+  for (int j = 0; j < G; ++j) {
+    beta[j] = ctl[r0 + j].beta;
+    first[j] = ctl[r0 + j].iter == 0;
   }
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t e = r * n + i;
-      p[e] = first[r] ? (double)z[e] : (double)z[e] + beta[r] * p[e];
+    for (int j = 0; j < G; ++j) {
+      const int64_t e = (r0 + j) * n + i;
+      p[e] = first[j] ? (double)z[e] : (double)z[e] + beta[j] * p[e];
     }
 }
 
@@ -673,25 +778,27 @@ __global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* 
                                                 const MrCtl* __restrict__ ctl,
                                                 const int* __restrict__ all_done) {
   if (*all_done) return;
-  double alpha[R], acc[R];
+  constexpr int G = mr_outer_lanes(R);
+  const int r0 = blockIdx.y * G;
+  double alpha[G], acc[G];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    alpha[r] = ctl[r].alpha;
-    acc[r] = 0.0;
+  for (int j = 0; j < G; ++j) {
+    alpha[j] = ctl[r0 + j].alpha;
+    acc[j] = 0.0;
   }
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const double m = minv[i];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t e = r * n + i;
-      x[e] += alpha[r] * p[e];
-      const double rn = rr[e] - alpha[r] * q[e];
+    for (int j = 0; j < G; ++j) {
+      const int64_t e = (r0 + j) * n + i;
+      x[e] += alpha[j] * p[e];
+      const double rn = rr[e] - alpha[j] * q[e];
       rr[e] = rn;
       rf[e] = (float)rn;
-      acc[r] += rn * (m * rn);
+      acc[j] += rn * (m * rn);
     }
   }
-  mr_block_partials<R, 1>(acc, part);
+  mr_block_partials_g<G, 1>(acc, part, R, r0);
 }
 
 // rho_prev = rho ; iter++ ; stopping test on |S r| per lane
