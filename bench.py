@@ -370,8 +370,8 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
         # fine-level kernels of the 32-lane group, HIP-event timed on the
         # launch stream (after the timed region: the group's work is reused)
         res["roofline_batched"] = {}
-        for kname, kfull in (("cheb", "k_mr_cheb (Chebyshev step, nodes without Schwarz "
-                                      "blocks, 32 lanes)"),
+        for kname, kfull in (("cheb", "k_mr_cheb_all<4> (Chebyshev step over every fine "
+                                      "node, 32 lanes)"),
                              ("update", "k_mr_upd<32> (FP64 x, r update, 32 lanes)"),
                              ("stencil", "k_b_tiles<float, zc> over 32 lanes (regulariser)")):
             ms, nbytes = sysb.time_multi(kname, iters=20)

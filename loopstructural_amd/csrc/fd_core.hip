@@ -1928,8 +1928,9 @@ struct MgLevel {
   SellBuf ssl, gsl;
   // levels with a covered list: the covered rows sliced, their columns as
   // covered-list positions (the multi-rhs step gathers an interleaved copy
-  // of r over the list)
+  // of r over the list), and each node's list position (-1: uncovered)
   SellBuf csl;
+  int32_t* cpos = nullptr;
 };
 
 struct Mg {
@@ -1998,6 +1999,7 @@ void mg_invalidate(Mg* m) {
     L.ssl = SellBuf();
     L.gsl = SellBuf();
     L.csl = SellBuf();
+    L.cpos = nullptr;
   }
   m->sell_ready = false;
   m->stale = true;
@@ -3229,9 +3231,10 @@ struct MrWork {
   std::vector<BSys*> lsys;
   std::vector<int4*> ltiles;
   std::vector<int> lnt, lzc;  // work items and stencil form (kMrForm code) per level
-  // per level with a covered list: interleaved r over the list (R floats per
-  // covered node), capacity in floats
-  std::vector<float*> rint;
+  // per level with a covered list: interleaved r over the list and the
+  // lane-major M r of the listed nodes (R floats per covered node each),
+  // capacity in floats
+  std::vector<float*> rint, zcov;
   std::vector<int64_t> rint_cap;
   std::vector<void*> mem;
   hipGraphExec_t graph = nullptr;
@@ -3573,9 +3576,9 @@ int mr_sell_ensure(lsamd_fd_system* s, hipStream_t st) {
       int hbad = 0;
       LSAMD_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
       LSAMD_HIP(hipStreamSynchronize(st));
-      mg_give(m, pos);
       mg_give(m, bad);
       if (hbad) return fail(LSAMD_ERR_ARG, "multigrid: a covered row reaches an uncovered node");
+      L.cpos = pos;
     }
     if (l > 0 && L.gptr)
       if (int rc = sell_build(m, L.gptr, L.gcol, L.gval, nullptr, nullptr, L.n, &L.gsl, st))
@@ -3597,22 +3600,26 @@ int mr_sell_ensure(lsamd_fd_system* s, hipStream_t st) {
 int mr_rint_ensure(lsamd_fd_system* s, MrWork* w) {
   const Mg* m = s->mg;
   w->rint.resize(m->lv.size(), nullptr);
+  w->zcov.resize(m->lv.size(), nullptr);
   w->rint_cap.resize(m->lv.size(), 0);
   for (size_t l = 0; l < m->lv.size(); ++l) {
     const int64_t need = m->lv[l].csl.nrows * w->R;
     if (need == 0 || w->rint_cap[l] >= need) continue;
     mr_invalidate(w);
-    if (float* old = w->rint[l]) {
-      for (size_t i = 0; i < w->mem.size(); ++i)
-        if (w->mem[i] == (void*)old) {
-          hipFree(old);
-          w->mem.erase(w->mem.begin() + i);
-          break;
-        }
+    for (float** v : {&w->rint[l], &w->zcov[l]}) {
+      if (*v) {
+        for (size_t i = 0; i < w->mem.size(); ++i)
+          if (w->mem[i] == (void*)*v) {
+            hipFree(*v);
+            w->mem.erase(w->mem.begin() + i);
+            break;
+          }
+      }
+      *v = nullptr;
     }
-    w->rint[l] = nullptr;
     w->rint_cap[l] = 0;
     if (int rc = mr_alloc(w, &w->rint[l], need)) return rc;
+    if (int rc = mr_alloc(w, &w->zcov[l], need)) return rc;
     w->rint_cap[l] = need;
   }
   return LSAMD_OK;
@@ -3746,26 +3753,24 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
       step_sell(mode, c1, c2);
       return;
     }
-    hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
-                       w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
     if (split && L.csl.nrows > 0 && l < (int)w->rint.size() && w->rint[l]) {
+      // covered rows from the interleaved r, then one pass over every node
       hipLaunchKernelGGL(k_mr_cov_pack<R>, dim3((unsigned)ceil_div(L.ncov, 64)), b256, 0, st,
                          (const int32_t*)L.cov, L.ncov, (const float*)w->b[l],
                          (const float*)w->q[l], L.n, (mode & 3) == 0 ? 1 : 0, w->rint[l], ad);
-      const Sell P = sell_dev(L.csl);
-      const unsigned nbc = (unsigned)ceil_div(L.csl.nrows, 256);
-      const char* cl = getenv("LSAMD_MR_COVINT_LG");
-      if (R == 32 && cl && atoi(cl) == 32) {
-        hipLaunchKernelGGL((k_mr_cheb_cov_int<R, R>), dim3(nbc, 1), b256, 0, st, P,
-                           (const float*)w->rint[l], w->x[l], w->d[l], L.n, mode, L.c0, c1, c2,
-                           ad);
-      } else {
-        constexpr int LG = R < 16 ? R : 16;
-        hipLaunchKernelGGL((k_mr_cheb_cov_int<R, LG>), dim3(nbc, R / LG), b256, 0, st, P,
-                           (const float*)w->rint[l], w->x[l], w->d[l], L.n, mode, L.c0, c1, c2,
-                           ad);
-      }
-    } else if (split && L.ncov > 0) {
+      constexpr int LG = R < 16 ? R : 16;
+      hipLaunchKernelGGL((k_mr_cov_z<R, LG>), dim3((unsigned)ceil_div(L.csl.nrows, 256), R / LG),
+                         b256, 0, st, sell_dev(L.csl), (const float*)w->rint[l], w->zcov[l], ad);
+      constexpr int LA = 4;  // lanes per thread of the all-node pass
+      hipLaunchKernelGGL(k_mr_cheb_all<LA>, dim3((unsigned)ceil_div(L.n, 256), R / LA), b256, 0, st,
+                         (const float*)w->b[l], (const float*)w->q[l], (const float*)L.sm,
+                         (const int32_t*)L.cpos, (const float*)w->zcov[l], L.ncov, w->x[l],
+                         w->d[l], L.n, mode, L.c0, c1, c2, ad);
+      return;
+    }
+    hipLaunchKernelGGL(k_mr_cheb, nb, b256, 0, st, (const float*)w->b[l], (const float*)w->q[l], S,
+                       w->x[l], w->d[l], L.n, split, mode, L.c0, c1, c2, ad);
+    if (split && L.ncov > 0) {
       const int lg = mr_cov_lanes(R);
       const dim3 ncg((unsigned)std::max<int64_t>(1, ceil_div(L.ncov, 256)), R / lg);
       const int32_t* cv = (const int32_t*)L.cov;
@@ -3803,11 +3808,36 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   const MgLevel& C = m->lv[l + 1];
   constexpr int TG = 4;  // lanes per thread of the transfers
   static_assert(R % TG == 0, "lane groups");
-  hipLaunchKernelGGL(k_mr_restrict_lg<TG>, dim3((unsigned)ceil_div(C.n, 256), R / TG), b256, 0, st,
-                     (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g, ad, xcd);
+  if (C.n < kMrNodeThreads) {
+    hipLaunchKernelGGL(k_mr_restrict_lg<TG>, dim3((unsigned)ceil_div(C.n, 256), R / TG), b256, 0,
+                       st, (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g,
+                       ad, xcd);
+  } else {
+    // z-streamed restriction on large levels (small ones lack the work
+    // items): 256-position bands of the coarse plane, kRz coarse planes per
+    // work item
+    constexpr int kRz = 4;
+    const int64_t ncp = (int64_t)C.g.nx * C.g.ny;
+    const int nband = (int)ceil_div(ncp, 256);
+    const int nkc = (int)ceil_div(C.g.nz, kRz);
+    hipLaunchKernelGGL((k_mr_restrict_z<TG, kRz>), dim3((unsigned)(nband * nkc), R / TG), b256, 0,
+                       st, (const float*)w->b[l], (const float*)w->q[l], L.g, w->b[l + 1], C.g,
+                       nband, ad);
+  }
   mr_vcycle<R>(s, w, l + 1, st);
-  hipLaunchKernelGGL(k_mr_prolong_lg<TG>, dim3((unsigned)ceil_div(L.n, 256), R / TG), b256, 0, st,
-                     (const float*)w->x[l + 1], C.g, w->x[l], L.g, ad, 0);
+  if (L.n < kMrNodeThreads) {
+    hipLaunchKernelGGL(k_mr_prolong_lg<TG>, dim3((unsigned)ceil_div(L.n, 256), R / TG), b256, 0,
+                       st, (const float*)w->x[l + 1], C.g, w->x[l], L.g, ad, 0);
+  } else {
+    // z-streamed prolongation on large levels: 256-position bands of the
+    // fine plane, kPz fine planes per work item
+    constexpr int kPz = 8;
+    const int64_t fnxy = (int64_t)L.g.nx * L.g.ny;
+    const int nband = (int)ceil_div(fnxy, 256);
+    const int nkc = (int)ceil_div(L.g.nz, kPz);
+    hipLaunchKernelGGL((k_mr_prolong_z<TG, kPz>), dim3((unsigned)(nband * nkc), R / TG), b256, 0,
+                       st, (const float*)w->x[l + 1], C.g, w->x[l], L.g, nband, ad);
+  }
   mr_apply<R>(s, w, l, w->x[l], w->q[l], st);
   smooth(0);
 }
@@ -4707,9 +4737,17 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
   const MgLevel& L = s->mg->lv[0];
   const int64_t n = s->n;
   const int split = L.cov ? 1 : 0;
-  const int64_t nu = split ? n - L.ncov : n;  // nodes of the Chebyshev step
+  // the all-node step (covered nodes read M r from the list) where the
+  // level has one, else the split step over the uncovered nodes
+  const bool all = split && L.cpos && w->zcov.size() > 0 && w->zcov[0];
+  const int64_t nu = split ? n - L.ncov : n;  // uncovered nodes
   auto launch = [&]() {
-    if (kernel == 0)
+    if (kernel == 0 && all)
+      hipLaunchKernelGGL(k_mr_cheb_all<4>, dim3((unsigned)ceil_div(n, 256), R / 4), dim3(256), 0,
+                         st, (const float*)w->b[0], (const float*)w->q[0], (const float*)L.sm,
+                         (const int32_t*)L.cpos, (const float*)w->zcov[0], L.ncov, w->x[0],
+                         w->d[0], n, 2, L.c0, L.c1[1], L.c2[1], zflag);
+    else if (kernel == 0)
       hipLaunchKernelGGL(k_mr_cheb, dim3((unsigned)ceil_div(n, 256), R), dim3(256), 0, st,
                          (const float*)w->b[0], (const float*)w->q[0], mg_sm(L), w->x[0],
                          w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
@@ -4723,7 +4761,11 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
       mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], st, (const CgCtl*)s->tzero);
   };
   // algorithmic bytes: every input and output element once
-  *bytes = kernel == 0   ? 24.0 * R * (double)nu + 4.0 * (double)n
+  // (all-node step: 24 B per uncovered node-lane, 20 B per covered one (M r,
+  // d, x in; d, x out), the tags and the covered nodes' list positions)
+  *bytes = kernel == 0 && all ? 24.0 * R * (double)nu + 20.0 * R * (double)L.ncov +
+                                    4.0 * (double)n + 4.0 * (double)L.ncov
+           : kernel == 0      ? 24.0 * R * (double)nu + 4.0 * (double)n
            : kernel == 1 ? 52.0 * R * (double)n + 8.0 * (double)n
                          : 8.0 * R * (double)n;
   hipEvent_t e0, e1;

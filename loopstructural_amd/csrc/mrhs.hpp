@@ -425,15 +425,14 @@ __global__ __launch_bounds__(256) void k_mr_cov_pack(const int32_t* __restrict__
   for (int k = threadIdx.x; k < cnt * R; k += 256) ri[t0 * R + k] = tile[k / R][k % R];
 }
 
-// Chebyshev step of the covered nodes from the interleaved r: sliced rows
-// over the covered list whose columns are covered-list positions; LG lanes
-// per thread (lane group in blockIdx.y), read as LG / 4 float4 per column.
+// M r at the covered nodes from the interleaved r: sliced rows over the
+// covered list whose columns are covered-list positions; LG lanes per
+// thread (lane group in blockIdx.y), read as LG / 4 float4 per column.
+// Output lane-major over the list, zc[l ncov + t].
 template <int R, int LG>
-__global__ __launch_bounds__(256) void k_mr_cheb_cov_int(Sell P, const float* __restrict__ ri,
-                                                         float* __restrict__ x,
-                                                         float* __restrict__ d, int64_t n,
-                                                         int mode, float c0, float c1, float c2,
-                                                         const int* __restrict__ all_done) {
+__global__ __launch_bounds__(256) void k_mr_cov_z(Sell P, const float* __restrict__ ri,
+                                                  float* __restrict__ zc,
+                                                  const int* __restrict__ all_done) {
   static_assert(LG % 4 == 0 && R % LG == 0, "lane groups of float4");
   if (*all_done) return;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -442,7 +441,6 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_int(Sell P, const float* __
   const int w = P.width[s];
   const int64_t o = P.off[s] + (r & 63);
   const int l0 = blockIdx.y * LG;
-  const int m = mode & 3;
   float acc[LG];
 #pragma unroll
   for (int l = 0; l < LG; ++l) acc[l] = 0.f;
@@ -461,13 +459,48 @@ __global__ __launch_bounds__(256) void k_mr_cheb_cov_int(Sell P, const float* __
       acc[4 * g + 3] += a0 * p.w + a1 * u.w;
     }
   }
-  const int64_t i = P.node[r];
+#pragma unroll
+  for (int l = 0; l < LG; ++l) zc[(int64_t)(l0 + l) * P.nrows + r] = acc[l];
+}
+
+// Chebyshev step over every node of a level with a covered list, LG lanes
+// per thread (lane group in blockIdx.y): M r is the Jacobi scale times r on
+// uncovered nodes and zc (k_mr_cov_z) on covered ones, pos[i] the node's
+// list position.  Writing whole lines (no skipped nodes) avoids the partial
+// line writes that the covered runs left in a split update.
+template <int LG>
+__global__ __launch_bounds__(256) void k_mr_cheb_all(const float* __restrict__ b,
+                                                     const float* __restrict__ q,
+                                                     const float* __restrict__ sm,
+                                                     const int32_t* __restrict__ pos,
+                                                     const float* __restrict__ zc, int64_t ncov,
+                                                     float* __restrict__ x, float* __restrict__ d,
+                                                     int64_t n, int mode, float c0, float c1,
+                                                     float c2, const int* __restrict__ all_done) {
+  if (*all_done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t r0 = (int64_t)blockIdx.y * LG;
+  const int m = mode & 3;
+  const float t = sm[i];
+  float z[LG];
+  if (t >= 0.f) {
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const int64_t e = (r0 + l) * n + i;
+      z[l] = t * (m == 0 ? b[e] : b[e] - q[e]);
+    }
+  } else {
+    const int64_t p = pos[i];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) z[l] = zc[(r0 + l) * ncov + p];
+  }
 #pragma unroll
   for (int l = 0; l < LG; ++l) {
-    const int64_t el = (int64_t)(l0 + l) * n + i;
-    const float dv = m == 2 ? c1 * d[el] + c2 * acc[l] : c0 * acc[l];
-    if (!(mode & kMrLast)) d[el] = dv;
-    x[el] = m == 0 ? dv : x[el] + dv;
+    const int64_t e = (r0 + l) * n + i;
+    const float dv = m == 2 ? c1 * d[e] + c2 * z[l] : c0 * z[l];
+    if (!(mode & kMrLast)) d[e] = dv;
+    x[e] = m == 0 ? dv : x[e] + dv;
   }
 }
 
@@ -518,6 +551,124 @@ __global__ __launch_bounds__(256) void k_mr_restrict_lg(const float* __restrict_
   }
 #pragma unroll
   for (int l = 0; l < LG; ++l) bc[(r0 + l) * nc + id] = acc[l];
+}
+
+// The same restriction streamed along z: a thread per coarse column (I, J)
+// of a band of 256 consecutive positions of the coarse plane walks KC coarse
+// planes, summing each fine plane's 3 x 3 (I, J) neighbourhood once (the
+// plane 2K + 1 serves K and K + 1).  Neighbouring columns share their fine
+// rows inside the band, so the fine level is read about once, where the
+// node-per-thread form re-reads the y and z neighbours from beyond the L2.
+// blockIdx.x = band + nband * (z chunk); lane group in blockIdx.y.
+template <int LG, int KC>
+__global__ __launch_bounds__(256) void k_mr_restrict_z(const float* __restrict__ bf,
+                                                       const float* __restrict__ qf, GridDev gf,
+                                                       float* __restrict__ bc, GridDev gc,
+                                                       int nband,
+                                                       const int* __restrict__ all_done) {
+  if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(1, bx, by);
+  const int band = (int)(bx % (unsigned)nband), kch = (int)(bx / (unsigned)nband);
+  const int64_t pos = (int64_t)band * 256 + threadIdx.x;
+  const int64_t ncp = (int64_t)gc.nx * gc.ny;
+  if (pos >= ncp) return;
+  const int I = (int)(pos % gc.nx), J = (int)(pos / gc.nx);
+  const int K0 = kch * KC, K1 = K0 + KC < gc.nz ? K0 + KC : gc.nz;
+  const int64_t nc = ncp * gc.nz;
+  const int64_t fnxy = (int64_t)gf.nx * gf.ny;
+  const int64_t nf = fnxy * gf.nz;
+  const int64_t r0 = (int64_t)by * LG;
+  // S(k) = sum over the in-grid (i, j) of {2I-1, 2I, 2I+1} x {2J-1, 2J, 2J+1}
+  // of w_i w_j (bf - qf)(i, j, k); 0 outside the fine grid in z
+  auto plane = [&](int k, float* s) {
+#pragma unroll
+    for (int l = 0; l < LG; ++l) s[l] = 0.f;
+    if (k < 0 || k >= gf.nz) return;
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j >= gf.ny) continue;
+      const float wj = dj == 0 ? 1.f : 0.5f;
+      for (int di = -1; di <= 1; ++di) {
+        const int i = 2 * I + di;
+        if (i < 0 || i >= gf.nx) continue;
+        const float w = wj * (di == 0 ? 1.f : 0.5f);
+        const int64_t f = (int64_t)i + (int64_t)j * gf.nx + (int64_t)k * fnxy;
+#pragma unroll
+        for (int l = 0; l < LG; ++l) {
+          const int64_t e = (r0 + l) * nf + f;
+          s[l] += w * (bf[e] - qf[e]);
+        }
+      }
+    }
+  };
+  float sp[LG], s0[LG], s1[LG];
+  plane(2 * K0 - 1, sp);
+  for (int K = K0; K < K1; ++K) {
+    plane(2 * K, s0);
+    plane(2 * K + 1, s1);
+    const int64_t c = pos + (int64_t)K * ncp;
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      bc[(r0 + l) * nc + c] = 0.5f * sp[l] + s0[l] + 0.5f * s1[l];
+      sp[l] = s1[l];
+    }
+  }
+}
+
+// xf += P xc streamed along z: a thread per fine column (i, j) of a band of
+// 256 consecutive positions of the fine plane walks KC fine planes; the
+// (i, j) interpolation of a coarse plane is formed once and serves the fine
+// planes 2K - 1, 2K and 2K + 1.  blockIdx.x = band + nband * (z chunk); lane
+// group in blockIdx.y.
+template <int LG, int KC>
+__global__ __launch_bounds__(256) void k_mr_prolong_z(const float* __restrict__ xc, GridDev gc,
+                                                      float* __restrict__ xf, GridDev gf,
+                                                      int nband,
+                                                      const int* __restrict__ all_done) {
+  static_assert(KC % 2 == 0, "chunks start on even fine planes");
+  if (*all_done) return;
+  unsigned bx, by;
+  xcd_block(1, bx, by);
+  const int band = (int)(bx % (unsigned)nband), kch = (int)(bx / (unsigned)nband);
+  const int64_t pos = (int64_t)band * 256 + threadIdx.x;
+  const int64_t fnxy = (int64_t)gf.nx * gf.ny;
+  if (pos >= fnxy) return;
+  const int i = (int)(pos % gf.nx), j = (int)(pos / gf.nx);
+  const int k0 = kch * KC, k1 = k0 + KC < gf.nz ? k0 + KC : gf.nz;
+  const int64_t cnxy = (int64_t)gc.nx * gc.ny;
+  const int64_t nc = cnxy * gc.nz, nf = fnxy * gf.nz;
+  const int64_t r0 = (int64_t)by * LG;
+  const int i0 = i >> 1, j0 = j >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1;
+  const float wij = (ni == 2 ? 0.5f : 1.f) * (nj == 2 ? 0.5f : 1.f);
+  // (i, j) interpolation of coarse plane K
+  auto plane = [&](int K, float* s) {
+#pragma unroll
+    for (int l = 0; l < LG; ++l) s[l] = 0.f;
+    for (int b = 0; b < nj; ++b)
+      for (int a = 0; a < ni; ++a) {
+        const int64_t o = (int64_t)(i0 + a) + (int64_t)(j0 + b) * gc.nx + (int64_t)K * cnxy;
+#pragma unroll
+        for (int l = 0; l < LG; ++l) s[l] += xc[(r0 + l) * nc + o];
+      }
+#pragma unroll
+    for (int l = 0; l < LG; ++l) s[l] *= wij;
+  };
+  float pc[LG], pn[LG];
+  plane(k0 >> 1, pc);
+  for (int k = k0; k < k1; k += 2) {
+    const int64_t f = pos + (int64_t)k * fnxy;
+#pragma unroll
+    for (int l = 0; l < LG; ++l) xf[(r0 + l) * nf + f] += pc[l];
+    if (k + 1 >= k1) break;
+    plane((k >> 1) + 1, pn);
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      xf[(r0 + l) * nf + f + fnxy] += 0.5f * (pc[l] + pn[l]);
+      pc[l] = pn[l];
+    }
+  }
 }
 
 template <int LG>
