@@ -372,7 +372,7 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
         res["roofline_batched"] = {}
         for kname, kfull in (("cheb", "k_mr_cheb_all<4> (Chebyshev step over every fine "
                                       "node, 32 lanes)"),
-                             ("update", "k_mr_upd<32> (FP64 x, r update, 32 lanes)"),
+                             ("update", "k_mr_upd<32> (FP64 x, FP32 r update, 32 lanes)"),
                              ("stencil", "k_b_tiles<float, zc> over 32 lanes (regulariser)")):
             ms, nbytes = sysb.time_multi(kname, iters=20)
             ach = nbytes / (ms * 1e-3) / 1e9

@@ -3206,7 +3206,11 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
 struct MrWork {
   int R = 0;
   std::vector<float*> b, x, q, d;  // per level, R lane vectors one after the other
-  double *rhs = nullptr, *x64 = nullptr, *r64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  // outer flexible CG: x in FP64; the search direction p, N p and the
+  // residual (b[0], the V-cycle's input) in FP32, the residual replaced by
+  // the FP64 rhs - N x at the start of every captured chunk (q64 = N x)
+  double *rhs = nullptr, *x64 = nullptr, *q64 = nullptr;
+  float *p32 = nullptr, *q32 = nullptr;
   float* t32 = nullptr;
   double* t64 = nullptr;
   double* upart = nullptr;  // per-block partials of the row part of p.Np, per lane
@@ -3329,9 +3333,13 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
     w->d.push_back(d);
   }
   if (int rc = injected("multi_rhs_work")) return rc;
-  for (double** v : {&w->rhs, &w->x64, &w->r64, &w->p64, &w->q64}) {
+  for (double** v : {&w->rhs, &w->x64, &w->q64}) {
     if (int rc = mr_alloc(w, v, n * R)) return rc;
     LSAMD_HIP(legacy_memset(*v, 0, n * R * 8));
+  }
+  for (float** v : {&w->p32, &w->q32}) {
+    if (int rc = mr_alloc(w, v, n * R)) return rc;
+    LSAMD_HIP(legacy_memset(*v, 0, n * R * 4));
   }
   if (int rc = mr_alloc(w, &w->part, (int64_t)vec_blocks(n) * 2 * R)) return rc;
   if (int rc = mr_alloc(w, &w->ctl, R)) return rc;
@@ -3444,7 +3452,7 @@ int mr_build(lsamd_fd_system* s, MrWork* w, int R) {
       if (p)
         mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], 0);
       else
-        mr_tiles<double>(s, w, (const double*)w->p64, w->q64, 0);
+        mr_tiles<double>(s, w, (const double*)w->x64, w->q64, 0);
     });
     if (bc < 0) return fail(LSAMD_ERR_ARG, "multi-rhs: no stencil form fits the grid");
     use(bc);
@@ -3842,36 +3850,60 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
   smooth(0);
 }
 
+// r = (float) (rhs - N x) with N x applied in FP64, and the stopping test
+// on it (see k_mr_resid)
+template <int R>
+void mr_replace_residual(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
+  const int64_t n = s->n;
+  const int vb = vec_blocks(n);
+  const int* ad = w->all_done;
+  const dim3 b256(256), fin(mr_fin_threads(R));
+  RowsDev Rw = rows_dev(s, false);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL(k_mr_rows_t<double>, dim3((unsigned)ceil_div(s->nrows, 256), R), b256, 0,
+                       st, (const double*)w->x64, n, Rw, w->t64, R, ad);
+  mr_tiles<double>(s, w, (const double*)w->x64, w->q64, st);
+  if (s->nrows > 0)
+    hipLaunchKernelGGL(k_mr_rows_scatter<double>,
+                       dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0, st,
+                       Rw, (const double*)w->t64, w->q64, n, R, ad);
+  const dim3 vbg(vb, R / mr_outer_lanes(R));
+  hipLaunchKernelGGL(k_mr_resid<R>, vbg, b256, 0, st, (const double*)w->rhs,
+                     (const double*)w->q64, (const double*)s->minv64, w->b[0], n, w->part, ad);
+  hipLaunchKernelGGL(k_mr_resid_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
+                     w->all_done);
+}
+
 template <int R>
 void mr_iteration(lsamd_fd_system* s, MrWork* w, hipStream_t st) {
   const int64_t n = s->n;
   const int vb = vec_blocks(n);
   const int* ad = w->all_done;
   const dim3 b256(256), fin(mr_fin_threads(R));
-  mr_vcycle<R>(s, w, 0, st);  // from w->b[0] = (float) r
+  mr_vcycle<R>(s, w, 0, st);  // from w->b[0] = r
   const dim3 vbg(vb, R / mr_outer_lanes(R));  // lane groups of the vector kernels
   hipLaunchKernelGGL(k_mr_dots<R>, vbg, b256, 0, st, (const float*)w->x[0],
-                     (const double*)w->r64, (const double*)w->q64, n, w->part, ad);
+                     (const float*)w->b[0], (const float*)w->q32, n, w->part, ad);
   hipLaunchKernelGGL(k_mr_dots_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
                      ad);
-  hipLaunchKernelGGL(k_mr_dir<R>, vbg, b256, 0, st, (const float*)w->x[0], w->p64, n,
+  hipLaunchKernelGGL(k_mr_dir<R>, vbg, b256, 0, st, (const float*)w->x[0], w->p32, n,
                      (const MrCtl*)w->ctl, ad);
-  // q = N p; p.q from the stencil's per-work-item partials and the rows'
-  // per-block partials
-  RowsDev Rw = rows_dev(s, false);
+  // q = N p in FP32; p.q from the stencil's per-work-item partials and the
+  // rows' per-block partials (both summed in FP64)
+  RowsDev Rw = rows_dev(s, true);
   const int nbu = s->nrows > 0 ? (int)ceil_div(s->nrows, 256) : 0;
   if (s->nrows > 0)
-    hipLaunchKernelGGL(k_mr_rows_tu, dim3((unsigned)nbu, R), b256, 0, st, (const double*)w->p64,
-                       n, Rw, w->t64, w->upart, ad);
-  mr_tiles<double>(s, w, (const double*)w->p64, w->q64, st);
+    hipLaunchKernelGGL(k_mr_rows_tu<float>, dim3((unsigned)nbu, R), b256, 0, st,
+                       (const float*)w->p32, n, Rw, w->t32, w->upart, ad);
+  mr_tiles<float>(s, w, (const float*)w->p32, w->q32, st);
   if (s->nrows > 0)
-    hipLaunchKernelGGL(k_mr_rows_scatter<double>,
+    hipLaunchKernelGGL(k_mr_rows_scatter<float>,
                        dim3((unsigned)std::max<int64_t>(1, ceil_div(s->ndn, 256)), R), b256, 0, st,
-                       Rw, (const double*)w->t64, w->q64, n, R, ad);
+                       Rw, (const float*)w->t32, w->q32, n, R, ad);
   hipLaunchKernelGGL(k_mr_alpha_st<R>, dim3(1), fin, 0, st, (const double*)w->partR,
-                     w->ntR[0] / R, (const double*)w->upart, nbu, w->ctl, ad);
-  hipLaunchKernelGGL(k_mr_upd<R>, vbg, b256, 0, st, w->x64, w->r64, (const double*)w->p64,
-                     (const double*)w->q64, (const double*)s->minv64, w->b[0], n, w->part,
+                     w->ntR[1] / R, (const double*)w->upart, nbu, w->ctl, ad);
+  hipLaunchKernelGGL(k_mr_upd<R>, vbg, b256, 0, st, w->x64, (const float*)w->p32,
+                     (const float*)w->q32, (const double*)s->minv64, w->b[0], n, w->part,
                      (const MrCtl*)w->ctl, ad);
   hipLaunchKernelGGL(k_mr_upd_fin<R>, dim3(1), fin, 0, st, (const double*)w->part, vb, w->ctl,
                      w->all_done);
@@ -3888,7 +3920,7 @@ int mr_solve_group(lsamd_fd_system* s, MrWork* w, const lsamd_solve_opts* o, int
   hipLaunchKernelGGL(k_mr_pack, dim3((unsigned)ceil_div(n * R, 256)), dim3(256), 0, st, rhs, nr,
                      R, n, w->rhs);
   hipLaunchKernelGGL(k_mr_init<R>, dim3(vb), dim3(256), 0, st, (const double*)w->rhs,
-                     (const double*)s->minv64, w->x64, w->r64, w->b[0], n, w->part);
+                     (const double*)s->minv64, w->x64, w->b[0], n, w->part);
   hipLaunchKernelGGL(k_mr_init_fin<R>, dim3(1), dim3(mr_fin_threads(R)), 0, st, (const double*)w->part, vb,
                      nr, o->rtol, o->atol, maxiter, w->ctl, w->all_done);
   LSAMD_CHECK_LAUNCH("multi-rhs init");
@@ -3899,6 +3931,8 @@ int mr_solve_group(lsamd_fd_system* s, MrWork* w, const lsamd_solve_opts* o, int
     LSAMD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph;
     LSAMD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    // each chunk starts from the true residual (at the first chunk: the rhs)
+    mr_replace_residual<R>(s, w, cs);
     for (int i = 0; i < chunk; ++i) mr_iteration<R>(s, w, cs);
     hipError_t e = hipStreamEndCapture(cs, &graph);
     hipStreamDestroy(cs);
@@ -4753,8 +4787,7 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
                          w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
     else if (kernel == 1)
       hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n), R / mr_outer_lanes(R)), dim3(256), 0,
-                         st, w->x64, w->r64,
-                         (const double*)w->p64, (const double*)w->q64,
+                         st, w->x64, (const float*)w->p32, (const float*)w->q32,
                          (const double*)s->minv64, w->b[0], n, w->part, (const MrCtl*)w->ctl,
                          zflag);
     else
@@ -4766,7 +4799,7 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
   *bytes = kernel == 0 && all ? 24.0 * R * (double)nu + 20.0 * R * (double)L.ncov +
                                     4.0 * (double)n + 4.0 * (double)L.ncov
            : kernel == 0      ? 24.0 * R * (double)nu + 4.0 * (double)n
-           : kernel == 1 ? 52.0 * R * (double)n + 8.0 * (double)n
+           : kernel == 1 ? 32.0 * R * (double)n + 8.0 * (double)n
                          : 8.0 * R * (double)n;
   hipEvent_t e0, e1;
   LSAMD_HIP(hipEventCreate(&e0));

@@ -7,9 +7,11 @@
 // indices and Schwarz rows are read once for R lanes and the launch count
 // per solve drops R-fold.  The hierarchy (coarse grids, Gram
 // matrices, Schwarz blocks, coarsest inverse, Chebyshev polynomials) is the
-// single-system one.  Lanes converge independently: a converged lane gets
-// alpha = 0 and keeps its iterate; every kernel returns at once when all
-// lanes are done.
+// single-system one.  The outer flexible CG keeps x in FP64 and p, N p and
+// the residual in FP32 (half the bytes of its vector passes), with the
+// residual replaced by the FP64 rhs - N x every few iterations.  Lanes
+// converge independently: a converged lane gets alpha = 0 and keeps its
+// iterate; every kernel returns at once when all lanes are done.
 #pragma once
 
 namespace lsamd {
@@ -85,27 +87,28 @@ __global__ __launch_bounds__(256) void k_mr_rows_t(const T* __restrict__ x, int6
   t[r * Rw.nrows + row] = w2[row] * acc;
 }
 
-// FP64 form with the row part of x.Nx: t = w^2 (a.x) per lane and per-block
-// partials of sum_rows t (a.x), written upart[lane * gridDim.x + blk]
-__global__ __launch_bounds__(256) void k_mr_rows_tu(const double* __restrict__ x, int64_t n,
-                                                    RowsDev Rw, double* __restrict__ t,
+// the same with the row part of x.Nx: t = w^2 (a.x) per lane and per-block
+// partials (FP64) of sum_rows t (a.x), written upart[lane * gridDim.x + blk]
+template <typename T>
+__global__ __launch_bounds__(256) void k_mr_rows_tu(const T* __restrict__ x, int64_t n,
+                                                    RowsDev Rw, T* __restrict__ t,
                                                     double* __restrict__ upart,
                                                     const int* __restrict__ all_done) {
   __shared__ double sh[4];
   if (*all_done) return;
-  const double* coef = (const double*)Rw.coef;
-  const double* w2 = (const double*)Rw.w2;
+  const T* coef = (const T*)Rw.coef;
+  const T* w2 = (const T*)Rw.w2;
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t r = blockIdx.y;
   double u = 0.0;
   if (row < Rw.nrows) {
-    const double* xs = x + r * n;
-    double acc = 0;
+    const T* xs = x + r * n;
+    T acc = 0;
     for (int64_t q = Rw.row_ptr[row]; q < Rw.row_ptr[row + 1]; ++q)
       acc += coef[q] * xs[Rw.cols[q]];
-    const double tv = w2[row] * acc;
+    const T tv = w2[row] * acc;
     t[r * Rw.nrows + row] = tv;
-    u = tv * acc;
+    u = (double)tv * (double)acc;
   }
   const double bs = block_sum(u, sh);
   if (threadIdx.x == 0) upart[r * gridDim.x + blockIdx.x] = bs;
@@ -780,13 +783,16 @@ __device__ inline double mr_wave_reduce(const double* part, int nblk, int k, int
   return wave_sum((v0 + v1) + (v2 + v3));
 }
 
-// x = 0 ; r = rhs ; rf = (float) r ; partials of r.(minv r)
+// The one-block kernels run min(R, 16) wavefronts; wavefront w
+// serves lanes w, w + nw, ...
+constexpr int mr_fin_threads(int R) { return 64 * (R < 16 ? R : 16); }
+
+// x = 0 ; r = (float) rhs ; partials of r.(minv r)
 template <int R>
 __global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
                                                  const double* __restrict__ minv,
-                                                 double* __restrict__ x, double* __restrict__ rr,
-                                                 float* __restrict__ rf, int64_t n,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ x, float* __restrict__ rf,
+                                                 int64_t n, double* __restrict__ part) {
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -796,7 +802,6 @@ __global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
     for (int r = 0; r < R; ++r) {
       const double v = rhs[r * n + i];
       x[r * n + i] = 0.0;
-      rr[r * n + i] = v;
       rf[r * n + i] = (float)v;
       acc[r] += v * (m * v);
     }
@@ -804,9 +809,65 @@ __global__ __launch_bounds__(256) void k_mr_init(const double* __restrict__ rhs,
   mr_block_partials<R, 1>(acc, part);
 }
 
-// The one-block kernels below run min(R, 16) wavefronts; wavefront w
-// serves lanes w, w + nw, ...
-constexpr int mr_fin_threads(int R) { return 64 * (R < 16 ? R : 16); }
+// Residual replacement: r = (float) (rhs - N x) with N x in FP64 (q, from the
+// FP64 apply of x), partials of r.(minv r); every few iterations, so that
+// the FP32 recurrence r -= alpha q never drifts from the true residual by
+// more than a few FP32 roundings of the residual since the last one
+template <int R>
+__global__ __launch_bounds__(256) void k_mr_resid(const double* __restrict__ rhs,
+                                                  const double* __restrict__ q,
+                                                  const double* __restrict__ minv,
+                                                  float* __restrict__ rf, int64_t n,
+                                                  double* __restrict__ part,
+                                                  const int* __restrict__ all_done) {
+  if (*all_done) return;
+  constexpr int G = mr_outer_lanes(R);
+  const int r0 = blockIdx.y * G;
+  double acc[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) acc[j] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double m = minv[i];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const int64_t e = (r0 + j) * n + i;
+      const double v = rhs[e] - q[e];
+      rf[e] = (float)v;
+      acc[j] += v * (m * v);
+    }
+  }
+  mr_block_partials_g<G, 1>(acc, part, R, r0);
+}
+
+// after a residual replacement: the stopping test on the true residual
+// (no iteration counted)
+template <int R>
+__global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_resid_fin(const double* __restrict__ part,
+                                                                    int nblk,
+                                                                    MrCtl* __restrict__ ctl,
+                                                                    int* __restrict__ all_done) {
+  if (*all_done) return;
+  __shared__ int done[R];
+  for (int r = threadIdx.x >> 6; r < R; r += blockDim.x >> 6) {
+    const double s = mr_wave_reduce<R>(part, nblk, 0, r);
+    if ((threadIdx.x & 63) == 0) {
+      MrCtl& c = ctl[r];
+      if (!c.done) {
+        c.res2 = s;
+        c.converged = sqrt(s) < c.thr;
+        c.done = c.converged;
+      }
+      done[r] = c.done;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int k = 0; k < R; ++k) all = all && done[k];
+    *all_done = all;
+  }
+}
+
 
 template <int R>
 __global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_init_fin(const double* __restrict__ part, int nblk,
@@ -840,8 +901,8 @@ __global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_init_fin(const double*
 // holds the previous N p, so z.r_prev = z.r + alpha_prev z.q)
 template <int R>
 __global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
-                                                 const double* __restrict__ rr,
-                                                 const double* __restrict__ q, int64_t n,
+                                                 const float* __restrict__ rr,
+                                                 const float* __restrict__ q, int64_t n,
                                                  double* __restrict__ part,
                                                  const int* __restrict__ all_done) {
   if (*all_done) return;
@@ -855,8 +916,8 @@ __global__ __launch_bounds__(256) void k_mr_dots(const float* __restrict__ z,
     for (int j = 0; j < G; ++j) {
       const int64_t e = (r0 + j) * n + i;
       const double zv = (double)z[e];
-      acc[j] += zv * rr[e];
-      acc[G + j] += zv * q[e];
+      acc[j] += zv * (double)rr[e];
+      acc[G + j] += zv * (double)q[e];
     }
   mr_block_partials_g<G, 2>(acc, part, R, r0);
 }
@@ -881,7 +942,7 @@ __global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_dots_fin(const double*
 
 // p = z + beta p
 template <int R>
-__global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, int64_t n,
+__global__ void k_mr_dir(const float* __restrict__ z, float* __restrict__ p, int64_t n,
                          const MrCtl* __restrict__ ctl, const int* __restrict__ all_done) {
   if (*all_done) return;
   constexpr int G = mr_outer_lanes(R);
@@ -897,7 +958,7 @@ __global__ void k_mr_dir(const float* __restrict__ z, double* __restrict__ p, in
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const int64_t e = (r0 + j) * n + i;
-      p[e] = first[j] ? (double)z[e] : (double)z[e] + beta[j] * p[e];
+      p[e] = first[j] ? z[e] : (float)((double)z[e] + beta[j] * (double)p[e]);
     }
 }
 
@@ -918,11 +979,12 @@ __global__ __launch_bounds__(mr_fin_threads(R)) void k_mr_alpha_st(
   }
 }
 
-// x += alpha p ; r -= alpha q ; rf = (float) r ; partials of r.(minv r)
+// x += alpha p (FP64 x) ; r -= alpha q (FP32 r, the V-cycle's input) ;
+// partials of r.(minv r)
 template <int R>
-__global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* __restrict__ rr,
-                                                const double* __restrict__ p,
-                                                const double* __restrict__ q,
+__global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x,
+                                                const float* __restrict__ p,
+                                                const float* __restrict__ q,
                                                 const double* __restrict__ minv,
                                                 float* __restrict__ rf, int64_t n,
                                                 double* __restrict__ part,
@@ -942,9 +1004,8 @@ __global__ __launch_bounds__(256) void k_mr_upd(double* __restrict__ x, double* 
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const int64_t e = (r0 + j) * n + i;
-      x[e] += alpha[j] * p[e];
-      const double rn = rr[e] - alpha[j] * q[e];
-      rr[e] = rn;
+      x[e] += alpha[j] * (double)p[e];
+      const double rn = (double)rf[e] - alpha[j] * (double)q[e];
       rf[e] = (float)rn;
       acc[j] += rn * (m * rn);
     }
