@@ -2319,7 +2319,7 @@ int mg_schwarz(Mg* m, MgLevel& L, const std::vector<int64_t>& cells, const CsrOu
   };
   auto groups = [](int n) {
     std::vector<Grp> v;
-    if (n <= 6) {
+    if (n < 6) {  // the first and last three would overlap
       v.push_back({0, n, true});
       return v;
     }

@@ -15,8 +15,10 @@
 // additive Schwarz: the exact inverse of N on each block, summed over
 // blocks, point Jacobi on the nodes no block covers.  Blocks are the 8 nodes
 // of every cell that holds a data row, and the boundary layers: per axis the
-// first three and the last three nodes form one group (a short axis of <= 6
-// nodes is one group), every other index is its own group, and each product
+// first three and the last three nodes form one group (a short axis of < 6
+// nodes is one group: an axis of 6 gives two groups of 3, whose blocks of
+// 27 invert in LDS where one block of 216 took 35 ms in global memory),
+// every other index is its own group, and each product
 // of groups with at least one boundary group is a block.  Point Jacobi alone
 // leaves the modes that those strong local couplings (data rows of weight
 // ~1 against regulariser rows of ~0.1; border rows of weight dx) do not see:
