@@ -22,6 +22,8 @@ struct BSys {
   int64_t r0, r1;  // data rows
   int b0, b1;      // node blocks
   int t0, t1;      // stencil tiles
+  const void* xp;  // pointer form (k_b_tiles<..., true>): the system's own x and y
+  void* yp;
 };
 
 // Stencil work item of the batched plane-streaming kernel: system, xy tile
@@ -69,9 +71,11 @@ __constant__ signed char kBTap[kTaps][3] = {
 // single-system plan is the same table with one system.  DMA: bulk planes
 // staged by direct global -> LDS copies (st_bulk_dma) instead of through
 // registers (st_bulk_body); same outputs bit for bit.  TXT: tile width of
-// the table (64 x 8 or 32 x 16 tiles).
-template <typename T, int ZC, bool DMA = false, int TXT = 64>
-__global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __restrict__ y,
+// the table (64 x 8 or 32 x 16 tiles).  PTR: each system's x and y are the
+// descriptor's own pointers xp, yp (systems in separate allocations; the
+// descriptors' off is 0) instead of offsets into x and y.
+template <typename T, int ZC, bool DMA = false, int TXT = 64, bool PTR = false>
+__global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ xg, T* __restrict__ yg,
                                                 const BSys* __restrict__ sys,
                                                 const int4* __restrict__ tiles,
                                                 const T* __restrict__ tab, Coef25<T> ku,
@@ -81,6 +85,9 @@ __global__ __launch_bounds__(NT) void k_b_tiles(const T* __restrict__ x, T* __re
   const int4 t = tiles[blockIdx.x];
   if (ctl[t.x].done) return;
   const BSys S = sys[t.x];
+  // pointer form: descriptors with off = 0
+  const T* x = PTR ? (const T*)S.xp : xg;
+  T* y = PTR ? (T*)S.yp : yg;
   double dot;
   static_assert(!DMA || TXT == 64, "LDS-DMA forms use 64 x 8 tiles");
   if (t.w >= kUniTile) {

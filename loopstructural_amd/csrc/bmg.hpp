@@ -6,10 +6,13 @@
 // coarsest inverse).  A batched kernel runs over a table of 256-node blocks
 // of every system at one level (the blocks of a system contiguous) and reads
 // the system's descriptor: pointers into its own vectors and operators.
-// Blocks of systems that have converged return at once.  Level operators are
-// applied node by node (class-table stencil, or the exact per-node formula,
-// plus the data term as a Gram row; level 0 uses its data Gram matrix too),
-// so no launch depends on tile tables.  Systems of different depth: at level
+// Blocks of systems that have converged return at once.  Coarse level
+// operators are applied node by node (class-table stencil, or the exact
+// per-node formula, plus the data term as a Gram row).  Level 0 of an
+// ensemble whose systems share one class table takes the plane-streaming
+// stencil over all systems (k_b_tiles in its pointer form, one table of
+// every system's work items) and adds its data Gram rows after it; otherwise
+// it is applied node by node too.  Systems of different depth: at level
 // l the smoothing and transfer kernels cover the systems for which l lies
 // above their coarsest level, the dense kernel those whose coarsest level is
 // l, so each system sees the operations of its own V-cycle in order.
@@ -28,6 +31,8 @@ struct BmgLevel {
   float c0, c1[kMgDegMax], c2[kMgDegMax];
   const float* cinv;
   int nc;
+  const int32_t* g0list;  // level 0: the nodes with a nonempty Gram row
+  int64_t g0n;
 };
 
 // one system's outer iteration (FP64 on level 0)
@@ -44,6 +49,10 @@ struct BmgSys {
   const double* rhs;
   double* out;
   int b0, b1;  // its blocks in the level-0 table
+  int t0, t1;  // its work items in the level-0 stencil table (empty: node by node)
+  const int32_t* g0list;  // the nodes with a nonempty Gram row
+  int64_t g0n;
+  int gb0, gb1;  // its partials of p.q in the node-block or Gram-list table
 };
 
 // y = N x at node id in FP64: the regulariser from the class table (or the
@@ -279,14 +288,62 @@ __global__ __launch_bounds__(256) void k_bmg_apply64(const BmgSys* __restrict__ 
 }
 
 // per system: alpha = rho / p.q
+// level 0 after the plane-streaming stencil: y += G0 x (the data Gram rows,
+// FP32) over the listed nodes of the smoothed systems (blocks of 256 list
+// entries)
+__global__ __launch_bounds__(256) void k_bmg_gram0(const BmgLevel* __restrict__ lv,
+                                                   const int2* __restrict__ blk,
+                                                   const CgCtl* __restrict__ ctl) {
+  const int2 t = blk[blockIdx.x];
+  if (ctl[t.x].done) return;
+  const BmgLevel& L = lv[t.x];
+  const int64_t k = (int64_t)t.y * 256 + threadIdx.x;
+  if (k >= L.g0n) return;
+  const int64_t i = L.g0list[k];
+  const int64_t e0 = L.A.gptr[i], e1 = L.A.gptr[i + 1];
+  float acc = 0.f;
+  for (int64_t e = e0; e < e1; ++e) acc += L.A.gval[e] * L.x[L.A.gcol[e]];
+  L.q[i] += acc;
+}
+
+// the same in FP64 for the outer iteration: q += G0 p, per-block partials of
+// p.(G0 p) (the stencil's own partials hold p.(R p))
+__global__ __launch_bounds__(256) void k_bmg_gram64(const BmgSys* __restrict__ sys,
+                                                    const int2* __restrict__ blk,
+                                                    double* __restrict__ part,
+                                                    const CgCtl* __restrict__ ctl) {
+  __shared__ double sh[4];
+  const int2 t = blk[blockIdx.x];
+  if (ctl[t.x].done) return;
+  const BmgSys& S = sys[t.x];
+  const int64_t k = (int64_t)t.y * 256 + threadIdx.x;
+  double acc = 0.0;
+  if (k < S.g0n) {
+    const int64_t i = S.g0list[k];
+    const int64_t e0 = S.gptr[i], e1 = S.gptr[i + 1];
+    double g = 0.0;
+    for (int64_t e = e0; e < e1; ++e) g += S.gval[e] * S.p[S.gcol[e]];
+    S.q[i] += g;
+    acc = S.p[i] * g;
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// per system: alpha = rho / p.q, p.q summed from its partials [gb0, gb1)
+// (node blocks, or Gram-list blocks in the plane-streaming form) and, in that
+// form, its work items' partials
 __global__ __launch_bounds__(256) void k_bmg_alpha(const BmgSys* __restrict__ sys,
                                                    const double* __restrict__ part,
+                                                   const double* __restrict__ tpart,
                                                    CgCtl* __restrict__ ctl) {
   __shared__ double sh[4];
   CgCtl& c = ctl[blockIdx.x];
   if (c.done) return;
   const BmgSys& S = sys[blockIdx.x];
-  const double pq = reduce_partials(part + S.b0, S.b1 - S.b0, sh);
+  double pq = reduce_partials(part + S.gb0, S.gb1 - S.gb0, sh);
+  __syncthreads();
+  if (S.t1 > S.t0) pq += reduce_partials(tpart + S.t0, S.t1 - S.t0, sh);
   if (threadIdx.x == 0) c.alpha = c.rho / pq;
 }
 

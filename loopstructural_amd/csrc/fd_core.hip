@@ -1958,6 +1958,8 @@ struct Mg {
   int32_t* g0col = nullptr;
   float* g0v32 = nullptr;
   double* g0v64 = nullptr;
+  int32_t* g0list = nullptr;  // the nodes with a nonempty Gram row
+  int64_t g0n = 0;
 };
 
 void mg_destroy(Mg* m) {
@@ -1981,6 +1983,8 @@ void mg_invalidate(Mg* m) {
   m->g0col = nullptr;
   m->g0v32 = nullptr;
   m->g0v64 = nullptr;
+  m->g0list = nullptr;
+  m->g0n = 0;
   for (auto& b : m->dmem) m->pool.push_back(b);
   m->dmem.clear();
   for (MgLevel& L : m->lv) {
@@ -2689,6 +2693,31 @@ int mg_build_rows(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   m->g0col = G0.col;
   m->g0v32 = G0.v32;
   m->g0v64 = G0.v64;
+  if (G0.ptr) {
+    // compact list of the nodes with Gram entries (the batched solve adds
+    // the level-0 data term over these only)
+    uint8_t* flag = nullptr;
+    int32_t *list = nullptr, *nsel = nullptr;
+    if (int rc = mg_dalloc(m, &flag, s->n)) return rc;
+    if (int rc = mg_dalloc(m, &list, s->n)) return rc;
+    if (int rc = mg_dalloc(m, &nsel, 1)) return rc;
+    hipLaunchKernelGGL(k_rows_nonempty, dim3((unsigned)ceil_div(s->n, 256)), dim3(256), 0, st,
+                       (const int64_t*)G0.ptr, flag, s->n);
+    hipcub::CountingInputIterator<int32_t> it(0);
+    size_t tb = 0;
+    LSAMD_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, it, flag, list, nsel, (int)s->n, st));
+    void* work = mg_take(m, tb);
+    if (!work) return fail(LSAMD_ERR_ALLOC, "hipMalloc failed (multigrid Gram node list)");
+    LSAMD_HIP(hipcub::DeviceSelect::Flagged(work, tb, it, flag, list, nsel, (int)s->n, st));
+    int32_t h = 0;
+    LSAMD_HIP(hipMemcpyAsync(&h, nsel, 4, hipMemcpyDeviceToHost, st));
+    LSAMD_HIP(hipStreamSynchronize(st));
+    mg_give(m, work);
+    mg_give(m, flag);
+    mg_give(m, nsel);
+    m->g0list = list;
+    m->g0n = h;
+  }
   // data rows of the previous level: CSR (level 0: the system's rows)
   const int64_t nr = s->nrows;
   const int64_t* prow = s->row_ptr;
@@ -2996,6 +3025,8 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
         d.A.gptr = m->g0ptr;
         d.A.gcol = m->g0col;
         d.A.gval = m->g0v32;
+        d.g0list = m->g0list;
+        d.g0n = m->g0list ? m->g0n : 0;
       }
       d.S = mg_sm(V);
       d.b = V.b;
@@ -3036,11 +3067,55 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
     y.minv = s->minv64;
     y.rhs = s->rhs;
     y.out = c[i];
+    y.g0list = m->g0list;
+    y.g0n = m->g0list ? m->g0n : 0;
+    y.gb0 = y.b0;  // node-block partials (the plane-streaming form resets these)
+    y.gb1 = y.b1;
+  }
+  // level 0 by the plane-streaming stencil when every system shares the
+  // class table (one regulariser) and is large enough for z-chunks of 16 on
+  // 32 x 16 tiles, and level 0 is not its coarsest; the work items of every
+  // system in one table, the descriptors in the pointer form (x, y = the
+  // system's own level-0 x, q in FP32 and p, q in FP64)
+  constexpr int kB0Zc = 16, kB0Tx = 32;
+  bool tiles0 = true;
+  for (int i = 0; i < nsys && tiles0; ++i) {
+    const lsamd_fd_system* s = ss[i];
+    tiles0 = s->regular && s->tab32 && s->tab64 && s->mg->lv.size() > 1 &&
+             memcmp(&s->reg, &ss[0]->reg, sizeof(lsamd_fd_reg)) == 0 && s->gd.nx >= 16 &&
+             s->gd.ny >= 16 && s->gd.nz - 6 >= kB0Zc;
+  }
+  std::vector<int4> ht;
+  std::vector<BSys> hb32, hb64;
+  std::vector<int2> hgl;  // blocks of 256 entries of every system's Gram node list
+  if (tiles0) {
+    hb32.resize(nsys);
+    hb64.resize(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      lsamd_fd_system* s = ss[i];
+      hs[i].gb0 = (int)hgl.size();
+      for (int64_t b = 0; b < ceil_div(hs[i].g0n, 256); ++b) hgl.push_back(make_int2(i, (int)b));
+      hs[i].gb1 = (int)hgl.size();
+      hs[i].t0 = (int)ht.size();
+      stencil_tiles(i, s->gd, kB0Zc, ht, kB0Tx);
+      hs[i].t1 = (int)ht.size();
+      for (BSys* b : {&hb32[i], &hb64[i]}) {
+        memset(b, 0, sizeof(BSys));
+        b->g = s->gd;
+        b->n = s->n;
+      }
+      hb32[i].xp = s->mg->lv[0].x;
+      hb32[i].yp = s->mg->lv[0].q;
+      hb64[i].xp = s->p64;
+      hb64[i].yp = s->q64;
+    }
   }
   // one device allocation: descriptors, tables, partials, control, flag
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = up(nsys * sizeof(BmgSys)) + up(nsys * sizeof(CgCtl)) + up(sizeof(int)) +
-                 up(2 * all[0].size() * sizeof(double));
+                 up(2 * all[0].size() * sizeof(double)) + up(ht.size() * sizeof(int4)) +
+                 up(ht.size() * sizeof(double)) + 2 * up(hb32.size() * sizeof(BSys)) +
+                 up(hgl.size() * sizeof(int2));
   for (int l = 0; l < nl; ++l)
     bytes += up(nsys * sizeof(BmgLevel)) + up(all[l].size() * sizeof(int2)) +
              up(sm[l].size() * sizeof(int2)) + up(dense[l].size() * sizeof(int));
@@ -3070,6 +3145,11 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
   CgCtl* ctl = (CgCtl*)take(nsys * sizeof(CgCtl));
   int* flag = (int*)take(sizeof(int));
   double* part = (double*)take(2 * all[0].size() * sizeof(double));
+  int4* dtiles = (int4*)take(ht.size() * sizeof(int4));
+  double* tpart = (double*)take(ht.size() * sizeof(double));
+  BSys* db32 = (BSys*)take(hb32.size() * sizeof(BSys));
+  BSys* db64 = (BSys*)take(hb64.size() * sizeof(BSys));
+  int2* dgl = (int2*)take(hgl.size() * sizeof(int2));
   std::vector<BmgLevel*> dlv(nl);
   std::vector<int2*> dall(nl), dsm(nl);
   std::vector<int*> ddense(nl);
@@ -3083,6 +3163,10 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
     return b ? hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, st) : hipSuccess;
   };
   hipError_t e = h2d(dsys, hs.data(), nsys * sizeof(BmgSys));
+  if (e == hipSuccess) e = h2d(dtiles, ht.data(), ht.size() * sizeof(int4));
+  if (e == hipSuccess) e = h2d(db32, hb32.data(), hb32.size() * sizeof(BSys));
+  if (e == hipSuccess) e = h2d(db64, hb64.data(), hb64.size() * sizeof(BSys));
+  if (e == hipSuccess) e = h2d(dgl, hgl.data(), hgl.size() * sizeof(int2));
   for (int l = 0; l < nl && e == hipSuccess; ++l) {
     e = h2d(dlv[l], hl[l].data(), nsys * sizeof(BmgLevel));
     if (e == hipSuccess) e = h2d(dall[l], all[l].data(), all[l].size() * sizeof(int2));
@@ -3107,7 +3191,18 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
     hipLaunchKernelGGL(k_bmg_cheb, g(sm[l].size()), b256, 0, cs, (const BmgLevel*)dlv[l],
                        (const int2*)dsm[l], mode, k, last, (const CgCtl*)ctl);
   };
+  const unsigned nt0 = (unsigned)ht.size();
   auto apply = [&](int l, hipStream_t cs) {
+    if (l == 0 && tiles0) {
+      hipLaunchKernelGGL((k_b_tiles<float, kB0Zc, false, kB0Tx, true>), dim3(nt0), dim3(NT), 0,
+                         cs, (const float*)nullptr, (float*)nullptr, (const BSys*)db32,
+                         (const int4*)dtiles, (const float*)ss[0]->tab32, ss[0]->ku32, tpart,
+                         (const CgCtl*)ctl);
+      if (!hgl.empty())
+        hipLaunchKernelGGL(k_bmg_gram0, g(hgl.size()), b256, 0, cs, (const BmgLevel*)dlv[0],
+                           (const int2*)dgl, (const CgCtl*)ctl);
+      return;
+    }
     hipLaunchKernelGGL(k_bmg_apply, g(sm[l].size()), b256, 0, cs, (const BmgLevel*)dlv[l],
                        (const int2*)dsm[l], (const CgCtl*)ctl);
   };
@@ -3147,10 +3242,20 @@ int bmg_solve(lsamd_fd_system* const* ss, int nsys, const lsamd_solve_opts* o,
                        (const double*)part, ctl);
     hipLaunchKernelGGL(k_bmg_dir, g(n0), b256, 0, cs, (const BmgSys*)dsys,
                        (const BmgLevel*)dlv[0], (const int2*)dall[0], (const CgCtl*)ctl);
-    hipLaunchKernelGGL(k_bmg_apply64, g(n0), b256, 0, cs, (const BmgSys*)dsys,
-                       (const int2*)dall[0], part, (const CgCtl*)ctl);
+    if (tiles0) {
+      hipLaunchKernelGGL((k_b_tiles<double, kB0Zc, false, kB0Tx, true>), dim3(nt0), dim3(NT), 0,
+                         cs, (const double*)nullptr, (double*)nullptr, (const BSys*)db64,
+                         (const int4*)dtiles, (const double*)ss[0]->tab64, ss[0]->ku64, tpart,
+                         (const CgCtl*)ctl);
+      if (!hgl.empty())
+        hipLaunchKernelGGL(k_bmg_gram64, g(hgl.size()), b256, 0, cs, (const BmgSys*)dsys,
+                           (const int2*)dgl, part, (const CgCtl*)ctl);
+    } else {
+      hipLaunchKernelGGL(k_bmg_apply64, g(n0), b256, 0, cs, (const BmgSys*)dsys,
+                         (const int2*)dall[0], part, (const CgCtl*)ctl);
+    }
     hipLaunchKernelGGL(k_bmg_alpha, g(nsys), b256, 0, cs, (const BmgSys*)dsys,
-                       (const double*)part, ctl);
+                       (const double*)part, (const double*)tpart, ctl);
     hipLaunchKernelGGL(k_bmg_upd, g(n0), b256, 0, cs, (const BmgSys*)dsys,
                        (const BmgLevel*)dlv[0], (const int2*)dall[0], part, (const CgCtl*)ctl);
     hipLaunchKernelGGL(k_bmg_upd_fin, g(nsys), b256, 0, cs, (const BmgSys*)dsys,

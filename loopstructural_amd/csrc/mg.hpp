@@ -803,6 +803,13 @@ __global__ void k_sell_fill(const int64_t* __restrict__ ptr, const int32_t* __re
   }
 }
 
+// flags of the nodes with a nonempty CSR row
+__global__ void k_rows_nonempty(const int64_t* __restrict__ ptr, uint8_t* __restrict__ flag,
+                                int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = ptr[i + 1] > ptr[i] ? 1 : 0;
+}
+
 // pos[cov[t]] = t (pos initialised to -1)
 __global__ void k_cov_pos(const int32_t* __restrict__ cov, int64_t ncov, int32_t* __restrict__ pos) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

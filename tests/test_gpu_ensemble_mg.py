@@ -57,17 +57,19 @@ def test_ensemble_multigrid_matches_minimiser():
     es.close()
 
 
-@pytest.mark.parametrize("nodes", [(14, 18), (9, 24)])
-def test_batched_multigrid_matches_minimiser(nodes):
+@pytest.mark.parametrize("nodes,count", [((14, 18), 12), ((9, 24), 12), ((22, 24), 6)])
+def test_batched_multigrid_matches_minimiser(nodes, count):
     """Every system of the batched multigrid solve (one launch sequence per
     iteration for all systems; hierarchies of two to four levels) reaches the
     oracle's direct minimiser to 1e-8 (sizes kept to <= 24^3 nodes: the
     direct solve grows fast with the grid); a second solve of the same rows
-    returns the same bits."""
+    returns the same bits.  Ensembles of 22-24 nodes per axis take the
+    plane-streaming fine-level operator (every axis >= 16 nodes, z-chunks of
+    16), the others the node-by-node one."""
     from loopstructural_amd import _lib, synthetic
     from loopstructural_amd import fd_system as fs
 
-    ens = synthetic.ensemble(12, 77 + nodes[1], nodes=nodes, n_values=(100, 400),
+    ens = synthetic.ensemble(count, 77 + nodes[1], nodes=nodes, n_values=(100, 400),
                              n_normals=(20, 80))
     grids, blocks = [], []
     for nn, v, m in ens:
