@@ -69,6 +69,11 @@ def parse():
     ap.add_argument("--p1-probe", action="store_true",
                     help="only time the P1 stencil at 129^3 and 257^3 nodes (every tuned form, "
                          "the workload of the P1 kernel traces)")
+    ap.add_argument("--ensemble-probe", action="store_true",
+                    help="only prepare and solve the 256-system ensemble by batched multigrid "
+                         "(the workload of its kernel trace)")
+    ap.add_argument("--workers", type=int, default=12,
+                    help="host threads of the ensemble drivers")
     return ap.parse_args()
 
 
@@ -134,6 +139,42 @@ def p1_probe():
     print(json.dumps({"p1_probe": out}), flush=True)
 
 
+def ensemble_probe(args):
+    """The 256-system ensemble (32^3..96^3 nodes) by batched multigrid: grid
+    hierarchies, then the timed rows, row-dependent hierarchy parts and
+    batched solve (as in the bench's ensemble measurement)."""
+    import torch
+
+    from loopstructural_amd import _lib, synthetic
+    from loopstructural_amd import fd_system as fs
+
+    dev = _lib.device()
+    ens = synthetic.ensemble(args.ensemble, 1004)
+    grids, ns = [], []
+    for nnode, _, _ in ens:
+        grids.append(_lib.make_grid((0, 0, 0), (1.0 / (nnode - 1),) * 3, (nnode,) * 3))
+        ns.append(nnode ** 3)
+    reg = fs.reg_struct(W, (32, 32, 32), 1e-8)
+    vd = [(torch.from_numpy(v).to(dev), torch.from_numpy(m).to(dev)) for _, v, m in ens]
+    blocks = [fd_rows(fs, _lib, g, v, m, k) for g, (v, m), k in zip(grids, vd, ns)]
+    bm = fs.BatchedMultigrid(grids, reg, workers=args.workers)
+    t0 = time.perf_counter()
+    bm.prepare(blocks)
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, infos = bm.solve(blocks, rtol=args.rtol)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"systems": len(ens), "workers": args.workers, "setup_seconds": round(t_prep, 3),
+           "seconds": round(dt, 3), "solves_per_s": round(len(ens) / dt, 2),
+           "iterations_max": max(i["iterations"] for i in infos),
+           "converged": sum(i["converged"] for i in infos),
+           "phase_seconds": {k: round(v, 3) for k, v in bm.last_timing.items()}}
+    bm.close()
+    print(json.dumps({"ensemble_probe": out}), flush=True)
+
+
 def stencil_traffic(nn, prec, zc):
     """HBM bytes per dispatch of the stencil from the committed counter
     summary (profiles/stencil_traffic.json), or None."""
@@ -166,6 +207,9 @@ def main():
         return
     if args.p1_probe:
         p1_probe()
+        return
+    if args.ensemble_probe:
+        ensemble_probe(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched and "
@@ -505,7 +549,7 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
     log(f"ensemble (batched Jacobi PCG): {dt:.2f} s")
     # the same ensemble by per-system multigrid, shape groups on several streams
     blocks = [fd_rows(fs, _lib, g, v, m, k) for g, (v, m), k in zip(grids, vd, ns)]
-    es = fs.EnsembleSolver(grids, reg, workers=12)
+    es = fs.EnsembleSolver(grids, reg, workers=args.workers)
     t0 = time.perf_counter()
     es.prepare(blocks, args.rtol)
     t_prep = time.perf_counter() - t0
@@ -519,7 +563,7 @@ def ensemble_c4(args, torch, _lib, fs, synthetic):
     del es
     # the same ensemble by batched multigrid: one hierarchy per system, every
     # system advanced by one launch sequence per iteration
-    bm = fs.BatchedMultigrid(grids, reg, workers=12)
+    bm = fs.BatchedMultigrid(grids, reg, workers=args.workers)
     t0 = time.perf_counter()
     bm.prepare(blocks)
     t_bprep = time.perf_counter() - t0
