@@ -9,7 +9,8 @@ exact minimiser of its realisation (spsolve on the oracle's normal equations,
 sides are left, else the smallest width that holds them): 3 -> 4, 8 -> 8,
 11 -> 16, 21 -> 16 + 8, 27 -> 32 (5 idle lanes), 37 -> 32 + 8, which covers
 every width, padded groups, and two widths in one call (a work per width is
-kept on the handle).  The right-hand side of a realisation from its row values
+kept on the handle); a 128^3 case takes the fine-level and transfer kernels of
+large grids.  The right-hand side of a realisation from its row values
 is checked against the system's own rhs after set_rows with those rows.  A
 failure while the lane work is allocated leaves no half-built work behind: the
 next call builds it afresh and solves correctly.
@@ -78,6 +79,28 @@ def test_multi_rhs_lanes_match_single_solves(nreal):
         ref = ofd.solve_exact(ofd.assemble_fd(og, {"value": v.copy(), "normal": m.copy()},
                                               regularisation=0.1))
         assert np.linalg.norm(c[r].cpu().numpy() - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def test_multi_rhs_at_128_cubed():
+    """At 128^3 nodes the fine level has a covered list (from 2^17 nodes): its
+    covered nodes are smoothed from the interleaved residual over the list and
+    every node by the whole-line pass, and level 1 (65^3) is reached by the
+    z-streamed restriction and prolongation.  Four realisations against the
+    single-system multigrid solve of each (1e-9); iteration counts agree
+    within one captured chunk of 8 (the multi-rhs outer iteration keeps its
+    direction and residual in FP32, the residual replaced every 8
+    iterations)."""
+    s, rhs, reals, g, n = _setup(127, 4)
+    print("multi-rhs at 128^3: solving 4 lanes", flush=True)
+    c, infos = s.solve_multi(rhs, rtol=1e-12)
+    assert all(i["converged"] for i in infos), infos
+    for r in range(4):
+        c1, i1 = s.solve(rtol=1e-12, rhs=rhs[r], multigrid=True, check_every=8)
+        d = float(torch.linalg.vector_norm(c[r] - c1) / torch.linalg.vector_norm(c1))
+        print(f"lane {r}: {infos[r]['iterations']} vs {i1['iterations']} iterations, "
+              f"rel. difference {d:.2e}", flush=True)
+        assert d <= 1e-9, (r, d)
+        assert abs(infos[r]["iterations"] - i1["iterations"]) <= 8, (infos[r], i1)
 
 
 def test_multi_rhs_after_new_rows_and_zero_rhs():
