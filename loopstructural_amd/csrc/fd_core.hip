@@ -1937,6 +1937,12 @@ struct Mg {
   std::vector<MgLevel> lv;
   int deg = 4;            // Chebyshev degree of the coarse levels
   int deg0 = 2;           // Chebyshev degree of the fine level (<= deg)
+  // coarse-level degree of the multi-rhs V-cycle: its fine level is HBM-bound
+  // over R lanes, so coarse steps are cheap against the iterations they save
+  // (at 128^3, 32 lanes: degree 4 101 iterations 757 ms per batch, 5 88 and
+  // 713, 6 80 and 708, 7 75 and 726); the single-system and batched solves,
+  // launch-bound, are fastest at 4
+  int deg_mr = 6;
   bool schwarz = true;    // Schwarz blocks in the smoother (else point Jacobi)
   bool stale = true;      // data rows changed: row-dependent parts to rebuild
   float* cinv = nullptr;  // dense inverse of the coarsest operator
@@ -2578,6 +2584,8 @@ int mg_build_static(lsamd_fd_system* s, Mg* m, hipStream_t st) {
   if (const char* dg = getenv("LSAMD_MG_DEGREE")) m->deg = std::max(1, std::min(kMgDegMax, atoi(dg)));
   if (const char* dg = getenv("LSAMD_MG_DEGREE0")) m->deg0 = std::max(1, atoi(dg));
   m->deg0 = std::min(m->deg0, m->deg);
+  if (const char* dg = getenv("LSAMD_MR_DEGREE"))
+    m->deg_mr = std::max(1, std::min(kMgDegMax, atoi(dg)));
   MgLevel L0;
   L0.sys = s;
   L0.g = s->gd;
@@ -2814,7 +2822,9 @@ int mg_build_rows(lsamd_fd_system* s, Mg* m, hipStream_t st) {
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
     L.c0 = (float)(1.0 / theta);
-    for (int k = 1; k < m->deg; ++k) {
+    // the Chebyshev recursion does not depend on the final degree: every
+    // coefficient up to the largest degree, each smoother takes its prefix
+    for (int k = 1; k < kMgDegMax; ++k) {
       const double rn = 1.0 / (2.0 * sigma - rho);
       L.c1[k] = (float)(rn * rho);
       L.c2[k] = (float)(2.0 * rn / delta);
@@ -3909,7 +3919,7 @@ void mr_vcycle(lsamd_fd_system* s, MrWork* w, int l, hipStream_t st) {
     }
   };
   auto smooth = [&](int from_zero) {
-    const int deg = l == 0 ? m->deg0 : m->deg;
+    const int deg = l == 0 ? m->deg0 : m->deg_mr;
     step((from_zero ? 0 : 1) | (deg == 1 ? kMrLast : 0), 0.f, 0.f);
     for (int k = 1; k < deg; ++k) {
       mr_apply<R>(s, w, l, w->x[l], w->q[l], st);

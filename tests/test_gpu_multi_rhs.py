@@ -72,7 +72,9 @@ def test_multi_rhs_lanes_match_single_solves(nreal):
         c1, i1 = s.solve(rtol=1e-12, rhs=rhs[r], multigrid=True, check_every=8)
         d = float(torch.linalg.vector_norm(c[r] - c1) / torch.linalg.vector_norm(c1))
         assert d <= 1e-9, (r, d)
-        assert abs(infos[r]["iterations"] - i1["iterations"]) <= 2, (infos[r], i1)
+        # the multi-rhs V-cycle smooths the coarse levels with degree 6 (4 in
+        # the single-system one): never more than one chunk of 8 beyond it
+        assert infos[r]["iterations"] <= i1["iterations"] + 8, (infos[r], i1)
     og = ofd.Grid((0, 0, 0), (23,) * 3, (1 / 23,) * 3)
     for r in (0, nreal - 1):
         v, m = reals[r]
@@ -86,10 +88,10 @@ def test_multi_rhs_at_128_cubed():
     covered nodes are smoothed from the interleaved residual over the list and
     every node by the whole-line pass, and level 1 (65^3) is reached by the
     z-streamed restriction and prolongation.  Four realisations against the
-    single-system multigrid solve of each (1e-9); iteration counts agree
-    within one captured chunk of 8 (the multi-rhs outer iteration keeps its
-    direction and residual in FP32, the residual replaced every 8
-    iterations)."""
+    single-system multigrid solve of each (1e-9); the multi-rhs solve takes
+    at most one captured chunk of 8 iterations more (its outer iteration
+    keeps direction and residual in FP32, the residual replaced every 8
+    iterations; its coarse levels smooth with degree 6 against 4)."""
     s, rhs, reals, g, n = _setup(127, 4)
     print("multi-rhs at 128^3: solving 4 lanes", flush=True)
     c, infos = s.solve_multi(rhs, rtol=1e-12)
@@ -100,7 +102,7 @@ def test_multi_rhs_at_128_cubed():
         print(f"lane {r}: {infos[r]['iterations']} vs {i1['iterations']} iterations, "
               f"rel. difference {d:.2e}", flush=True)
         assert d <= 1e-9, (r, d)
-        assert abs(infos[r]["iterations"] - i1["iterations"]) <= 8, (infos[r], i1)
+        assert infos[r]["iterations"] <= i1["iterations"] + 8, (infos[r], i1)
 
 
 def test_multi_rhs_after_new_rows_and_zero_rhs():
