@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--values", type=int, default=5000)
     ap.add_argument("--normals", type=int, default=1000)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--batch", type=int, default=32, help="realisations in the batched form")
+    ap.add_argument("--batch", type=int, default=64, help="realisations in the batched form")
     ap.add_argument("--ensemble", type=int, default=256, help="systems in the uneven ensemble")
     ap.add_argument("--forms", default="plain,mixed,mg,batched")
     ap.add_argument("--ref-solver", default="jacobi", choices=("jacobi", "mg", "none"),
@@ -367,17 +367,19 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
     dev = _lib.device()
     reals = [synthetic.perturb(values, normals, 2000 + b) for b in range(B)]
     vd = [(torch.from_numpy(v).to(dev), torch.from_numpy(m).to(dev)) for v, m in reals]
-    # per-realisation FP64 references (single-system solves, rtol 1e-13)
+    # FP64 references (single-system solves, rtol 1e-13) of 16 realisations
+    # spread over the batch (every one up to 16)
+    ref_idx = sorted(set(np.linspace(0, B - 1, min(B, 16)).astype(int).tolist()))
     refs = []
     rows0 = None
-    for v, m in vd:
+    for b, (v, m) in enumerate(vd):
         rows = fd_rows(fs, _lib, grid, v, m, n)
         if rows0 is None:
             rows0 = rows
         elif not (torch.equal(rows.cols, rows0.cols) and torch.equal(rows.coef, rows0.coef)
                   and torch.equal(rows.w, rows0.w)):
             raise RuntimeError("realisations do not share one matrix")
-        if args.ref_solver == "none":
+        if args.ref_solver == "none" or b not in ref_idx:
             continue
         single.set_rows(rows)
         if args.ref_solver == "mg":
@@ -401,23 +403,27 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
         c, infos = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    errs = [float(torch.linalg.vector_norm(c[b] - refs[b]) / torch.linalg.vector_norm(refs[b]))
-            for b in range(len(refs))] or [float("nan")]
+    errs = [float(torch.linalg.vector_norm(c[b] - r) / torch.linalg.vector_norm(r))
+            for b, r in zip(ref_idx, refs)] or [float("nan")]
     res = {"ms_per_solve": dt * 1e3 / B, "solves_per_s": B / dt, "batch": B,
            "ms_per_batch": dt * 1e3, "method": "multi-rhs multigrid, all lanes per launch",
            "iterations_max": max(i["iterations"] for i in infos), "refinement_sweeps_max": 0,
-           "rel_err_vs_fp64_max": max(errs), "meets_1e-6": max(errs) <= 1e-6,
-           "dtype": "f32+f64"}
+           "rel_err_vs_fp64_max": max(errs), "rel_err_checked": len(refs),
+           "meets_1e-6": max(errs) <= 1e-6, "dtype": "f32+f64"}
     log(f"batched x{B}: {dt * 1e3:.0f} ms/batch = {B / dt:.2f} solves/s, max relerr "
         f"{max(errs):.2e}")
-    if B == 32:
-        # fine-level kernels of the 32-lane group, HIP-event timed on the
-        # launch stream (after the timed region: the group's work is reused)
+    if B >= 24:
+        # fine-level kernels of the widest lane group (64 lanes from 48
+        # realisations, else 32), HIP-event timed on the launch stream (after
+        # the timed region: the group's work is reused)
+        lanes = 64 if B >= 48 else 32
         res["roofline_batched"] = {}
-        for kname, kfull in (("cheb", "k_mr_cheb_all<4> (Chebyshev step over every fine "
-                                      "node, 32 lanes)"),
-                             ("update", "k_mr_upd<32> (FP64 x, FP32 r update, 32 lanes)"),
-                             ("stencil", "k_b_tiles<float, zc> over 32 lanes (regulariser)")):
+        for kname, kfull in (("cheb", f"k_mr_cheb_all<4> (Chebyshev step over every fine "
+                                      f"node, {lanes} lanes)"),
+                             ("update", f"k_mr_upd<{lanes}> (FP64 x, FP32 r update, {lanes} "
+                                        f"lanes)"),
+                             ("stencil", f"k_b_tiles<float, zc> over {lanes} lanes "
+                                         f"(regulariser)")):
             ms, nbytes = sysb.time_multi(kname, iters=20)
             ach = nbytes / (ms * 1e-3) / 1e9
             res["roofline_batched"][kname] = {

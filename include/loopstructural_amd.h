@@ -246,12 +246,13 @@ int lsamd_fd_systems_solve_mg(lsamd_fd_system* const* systems, int32_t nsys,
                               const lsamd_solve_opts* opts, double* const* c,
                               lsamd_solve_info* infos, void* stream);
 
-/* Times one kernel of the 32-lane multi-right-hand-side multigrid on the fine
- * level, reusing the work of this handle's last 32-lane
- * lsamd_fd_system_solve_multi group (LSAMD_ERR_ARG without one) and leaving
- * its contents undefined: kernel 0 = the Chebyshev smoothing step of the
- * nodes without Schwarz blocks (k_mr_cheb), 1 = the FP64 update of x and r
- * (k_mr_upd), 2 = the 32-lane regulariser stencil.  One warm launch, then
+/* Times one kernel of the multi-right-hand-side multigrid on the fine level,
+ * reusing the work of this handle's widest lsamd_fd_system_solve_multi group
+ * (64 lanes, else 32; LSAMD_ERR_ARG without one) and leaving its contents
+ * undefined: kernel 0 = a later Chebyshev smoothing step over every node
+ * (k_mr_cheb_all; the split step of the uncovered nodes, k_mr_cheb, on grids
+ * without a covered list), 1 = the update of the FP64 x and the FP32 r
+ * (k_mr_upd), 2 = the regulariser stencil over the lanes.  One warm launch, then
  * `iters` launches between two HIP events on `stream`: *avg_ms per launch,
  * *bytes = algorithmic bytes per launch (every input and output element
  * once).  A measurement hook with no reference counterpart. */

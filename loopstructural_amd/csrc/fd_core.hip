@@ -1166,7 +1166,7 @@ struct lsamd_fd_system {
   lsamd::Mg* mg = nullptr;  // multigrid hierarchy (built on first MG solve after set_rows)
   lsamd::Ls* ls = nullptr;  // LSMR work (built on the first LSMR solve after set_rows)
   // multi-rhs multigrid work per group width 4/8/16/32 (kept across set_rows)
-  lsamd::MrWork* mr[4] = {nullptr, nullptr, nullptr, nullptr};
+  lsamd::MrWork* mr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void* tzero = nullptr;  // zeroed control blocks of the timing entry points
   // buffers of the row set (set_rows), kept across calls: (pointer, bytes)
   std::vector<std::pair<void*, size_t>> rpool;
@@ -3625,7 +3625,7 @@ int mr_rows(lsamd_fd_system* s, MrWork* w) {
 // work of R lanes (R = 4, 8, 16 or 32), one per width, kept on the handle
 // only when complete: a failed build or row resize frees it
 int mr_ensure(lsamd_fd_system* s, int R, MrWork** out) {
-  const int slot = R == 4 ? 0 : R == 8 ? 1 : R == 16 ? 2 : 3;
+  const int slot = R == 4 ? 0 : R == 8 ? 1 : R == 16 ? 2 : R == 32 ? 3 : 4;
   MrWork* w = s->mr[slot];
   if (!w) {
     w = new MrWork();
@@ -4101,9 +4101,9 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
   }
   for (int r0 = 0; r0 < nrhs;) {
     const int left = nrhs - r0;
-    // lanes per group: 32 only when at least 24 are left (an idle lane costs
+    // lanes per group: 32 from 24 left, 64 from 48 left (an idle lane costs
     // as much as a busy one)
-    const int R = left <= 4 ? 4 : (left <= 8 ? 8 : (left < 24 ? 16 : 32));
+    const int R = left <= 4 ? 4 : (left <= 8 ? 8 : (left < 24 ? 16 : (left < 48 ? 32 : 64)));
     const int nr = std::min(left, R);
     MrWork* w = nullptr;
     if (int rc = mr_ensure(s, R, &w)) return rc;
@@ -4113,7 +4113,8 @@ int mr_solve(lsamd_fd_system* s, const lsamd_solve_opts* o, int nrhs, const doub
     const int rc = R == 4    ? mr_solve_group<4>(s, w, o, nr, rr, cc, infos + r0, st)
                    : R == 8  ? mr_solve_group<8>(s, w, o, nr, rr, cc, infos + r0, st)
                    : R == 16 ? mr_solve_group<16>(s, w, o, nr, rr, cc, infos + r0, st)
-                             : mr_solve_group<32>(s, w, o, nr, rr, cc, infos + r0, st);
+                   : R == 32 ? mr_solve_group<32>(s, w, o, nr, rr, cc, infos + r0, st)
+                             : mr_solve_group<64>(s, w, o, nr, rr, cc, infos + r0, st);
     if (rc) return rc;
     r0 += nr;
   }
@@ -4402,6 +4403,65 @@ int eval_adjoint_into(const lsamd_grid* g, const double* pts, int64_t n, const d
 }
 
 }  // namespace lsamd
+
+// lsamd_fd_system_time_multi for a group of R lanes
+template <int R>
+int time_multi(lsamd_fd_system* s, MrWork* w, int32_t kernel, int32_t iters, double* avg_ms,
+               double* bytes, hipStream_t st) {
+  if (!s->tzero) {  // no lane done, all-done flag clear (room for the widest group)
+    if (int rc = dmalloc(s, &s->tzero, 64 * sizeof(CgCtl))) return rc;
+    LSAMD_HIP(legacy_memset(s->tzero, 0, 64 * sizeof(CgCtl)));
+  }
+  const int* zflag = (const int*)s->tzero;
+  const MgLevel& L = s->mg->lv[0];
+  const int64_t n = s->n;
+  const int split = L.cov ? 1 : 0;
+  // the all-node step (covered nodes read M r from the list) where the
+  // level has one, else the split step over the uncovered nodes
+  const bool all = split && L.cpos && w->zcov.size() > 0 && w->zcov[0];
+  const int64_t nu = split ? n - L.ncov : n;  // uncovered nodes
+  auto launch = [&]() {
+    if (kernel == 0 && all)
+      hipLaunchKernelGGL(k_mr_cheb_all<4>, dim3((unsigned)ceil_div(n, 256), R / 4), dim3(256), 0,
+                         st, (const float*)w->b[0], (const float*)w->q[0], (const float*)L.sm,
+                         (const int32_t*)L.cpos, (const float*)w->zcov[0], L.ncov, w->x[0],
+                         w->d[0], n, 2, L.c0, L.c1[1], L.c2[1], zflag);
+    else if (kernel == 0)
+      hipLaunchKernelGGL(k_mr_cheb, dim3((unsigned)ceil_div(n, 256), R), dim3(256), 0, st,
+                         (const float*)w->b[0], (const float*)w->q[0], mg_sm(L), w->x[0],
+                         w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
+    else if (kernel == 1)
+      hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n), R / mr_outer_lanes(R)), dim3(256), 0,
+                         st, w->x64, (const float*)w->p32, (const float*)w->q32,
+                         (const double*)s->minv64, w->b[0], n, w->part, (const MrCtl*)w->ctl,
+                         zflag);
+    else
+      mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], st, (const CgCtl*)s->tzero);
+  };
+  // algorithmic bytes: every input and output element once
+  // (all-node step: 24 B per uncovered node-lane, 20 B per covered one (M r,
+  // d, x in; d, x out), the tags and the covered nodes' list positions)
+  *bytes = kernel == 0 && all ? 24.0 * R * (double)nu + 20.0 * R * (double)L.ncov +
+                                    4.0 * (double)n + 4.0 * (double)L.ncov
+           : kernel == 0      ? 24.0 * R * (double)nu + 4.0 * (double)n
+           : kernel == 1 ? 32.0 * R * (double)n + 8.0 * (double)n
+                         : 8.0 * R * (double)n;
+  hipEvent_t e0, e1;
+  LSAMD_HIP(hipEventCreate(&e0));
+  LSAMD_HIP(hipEventCreate(&e1));
+  launch();
+  LSAMD_HIP(hipEventRecord(e0, st));
+  for (int it = 0; it < iters; ++it) launch();
+  LSAMD_HIP(hipEventRecord(e1, st));
+  LSAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  LSAMD_CHECK_LAUNCH("time_multi");
+  *avg_ms = ms / iters;
+  return LSAMD_OK;
+}
 
 extern "C" {
 
@@ -4873,65 +4933,17 @@ int lsamd_fd_system_time_multi(lsamd_fd_system* s, int32_t kernel, int32_t iters
                                double* avg_ms, double* bytes, void* stream) {
   if (!s || !avg_ms || !bytes || iters < 1 || kernel < 0 || kernel > 2)
     return fail(LSAMD_ERR_ARG, "time_multi: bad arguments");
-  MrWork* w = s->mr[3];
-  if (!w || w->R != 32 || !s->mg || !s->regular)
-    return fail(LSAMD_ERR_ARG, "time_multi: needs a 32-lane solve_multi on this handle first");
+  if (!s->mg || !s->regular)
+    return fail(LSAMD_ERR_ARG, "time_multi: needs a 32- or 64-lane solve_multi on this handle");
   hipStream_t st = (hipStream_t)stream;
-  constexpr int R = 32;
-  if (!s->tzero) {  // no lane done, all-done flag clear
-    if (int rc = dmalloc(s, &s->tzero, R * sizeof(CgCtl))) return rc;
-    LSAMD_HIP(legacy_memset(s->tzero, 0, R * sizeof(CgCtl)));
-  }
-  const int* zflag = (const int*)s->tzero;
-  const MgLevel& L = s->mg->lv[0];
-  const int64_t n = s->n;
-  const int split = L.cov ? 1 : 0;
-  // the all-node step (covered nodes read M r from the list) where the
-  // level has one, else the split step over the uncovered nodes
-  const bool all = split && L.cpos && w->zcov.size() > 0 && w->zcov[0];
-  const int64_t nu = split ? n - L.ncov : n;  // uncovered nodes
-  auto launch = [&]() {
-    if (kernel == 0 && all)
-      hipLaunchKernelGGL(k_mr_cheb_all<4>, dim3((unsigned)ceil_div(n, 256), R / 4), dim3(256), 0,
-                         st, (const float*)w->b[0], (const float*)w->q[0], (const float*)L.sm,
-                         (const int32_t*)L.cpos, (const float*)w->zcov[0], L.ncov, w->x[0],
-                         w->d[0], n, 2, L.c0, L.c1[1], L.c2[1], zflag);
-    else if (kernel == 0)
-      hipLaunchKernelGGL(k_mr_cheb, dim3((unsigned)ceil_div(n, 256), R), dim3(256), 0, st,
-                         (const float*)w->b[0], (const float*)w->q[0], mg_sm(L), w->x[0],
-                         w->d[0], n, split, 2, L.c0, L.c1[1], L.c2[1], zflag);
-    else if (kernel == 1)
-      hipLaunchKernelGGL(k_mr_upd<R>, dim3(vec_blocks(n), R / mr_outer_lanes(R)), dim3(256), 0,
-                         st, w->x64, (const float*)w->p32, (const float*)w->q32,
-                         (const double*)s->minv64, w->b[0], n, w->part, (const MrCtl*)w->ctl,
-                         zflag);
-    else
-      mr_tiles<float>(s, w, (const float*)w->x[0], w->q[0], st, (const CgCtl*)s->tzero);
-  };
-  // algorithmic bytes: every input and output element once
-  // (all-node step: 24 B per uncovered node-lane, 20 B per covered one (M r,
-  // d, x in; d, x out), the tags and the covered nodes' list positions)
-  *bytes = kernel == 0 && all ? 24.0 * R * (double)nu + 20.0 * R * (double)L.ncov +
-                                    4.0 * (double)n + 4.0 * (double)L.ncov
-           : kernel == 0      ? 24.0 * R * (double)nu + 4.0 * (double)n
-           : kernel == 1 ? 32.0 * R * (double)n + 8.0 * (double)n
-                         : 8.0 * R * (double)n;
-  hipEvent_t e0, e1;
-  LSAMD_HIP(hipEventCreate(&e0));
-  LSAMD_HIP(hipEventCreate(&e1));
-  launch();
-  LSAMD_HIP(hipEventRecord(e0, st));
-  for (int it = 0; it < iters; ++it) launch();
-  LSAMD_HIP(hipEventRecord(e1, st));
-  LSAMD_HIP(hipEventSynchronize(e1));
-  float ms = 0.f;
-  LSAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  LSAMD_CHECK_LAUNCH("time_multi");
-  *avg_ms = ms / iters;
-  return LSAMD_OK;
+  // the widest group this handle has built
+  if (MrWork* w = s->mr[4]; w && w->R == 64)
+    return time_multi<64>(s, w, kernel, iters, avg_ms, bytes, st);
+  if (MrWork* w = s->mr[3]; w && w->R == 32)
+    return time_multi<32>(s, w, kernel, iters, avg_ms, bytes, st);
+  return fail(LSAMD_ERR_ARG, "time_multi: needs a 32- or 64-lane solve_multi on this handle");
 }
+
 
 int lsamd_fd_system_time_stencil(lsamd_fd_system* s, int32_t precision, int32_t iters,
                                  double* avg_ms, void* stream) {

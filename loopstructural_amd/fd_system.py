@@ -257,9 +257,9 @@ class FdSystem:
 
     def time_multi(self, kernel: str, iters=20):
         """Average launch time (ms) and algorithmic bytes of one fine-level
-        kernel of the 32-lane multi-rhs multigrid ('cheb', 'update' or
-        'stencil'), after a solve_multi with a 32-lane group; the group's work
-        is left undefined."""
+        kernel of the multi-rhs multigrid ('cheb', 'update' or 'stencil') on
+        the widest lane group built (64, else 32), after a solve_multi with
+        such a group; the group's work is left undefined."""
         k = {"cheb": 0, "update": 1, "stencil": 2}[kernel]
         ms, nb = C.c_double(0.0), C.c_double(0.0)
         _lib.call("lsamd_fd_system_time_multi", self._h, k, int(iters), C.byref(ms),

@@ -5,10 +5,11 @@ observed values and normal directions).
 Each lane is checked against the single-system multigrid solve of the same
 right-hand side (both to rtol 1e-12: agreement 1e-9 relative) and against the
 exact minimiser of its realisation (spsolve on the oracle's normal equations,
-1e-8).  Groups are 4, 8, 16 or 32 lanes wide (32 once at least 24 right-hand
-sides are left, else the smallest width that holds them): 3 -> 4, 8 -> 8,
-11 -> 16, 21 -> 16 + 8, 27 -> 32 (5 idle lanes), 37 -> 32 + 8, which covers
-every width, padded groups, and two widths in one call (a work per width is
+1e-8).  Groups are 4, 8, 16, 32 or 64 lanes wide (32 once at least 24 right-hand
+sides are left, 64 once at least 48, else the smallest width that holds
+them): 3 -> 4, 8 -> 8, 11 -> 16, 21 -> 16 + 8, 27 -> 32 (5 idle lanes),
+37 -> 32 + 8, 50 -> 64 (14 idle), which covers every width, padded groups,
+and two widths in one call (a work per width is
 kept on the handle); a 128^3 case takes the fine-level and transfer kernels of
 large grids.  The right-hand side of a realisation from its row values
 is checked against the system's own rhs after set_rows with those rows.  A
@@ -63,7 +64,7 @@ def test_rhs_for_matches_set_rows():
     assert float(torch.linalg.vector_norm(rhs[1] - ref) / torch.linalg.vector_norm(ref)) <= 1e-14
 
 
-@pytest.mark.parametrize("nreal", [3, 8, 11, 21, 27, 37])
+@pytest.mark.parametrize("nreal", [3, 8, 11, 21, 27, 37, 50])
 def test_multi_rhs_lanes_match_single_solves(nreal):
     s, rhs, reals, g, n = _setup(23, nreal)
     c, infos = s.solve_multi(rhs, rtol=1e-12)
