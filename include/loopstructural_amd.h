@@ -181,7 +181,7 @@ int lsamd_fd_system_solve(lsamd_fd_system* s, const lsamd_solve_opts* opts, cons
  * 2): rhs and c are (nrhs, nnodes) FP64, one vector after the other; infos
  * (nrhs) host memory.  Realisations that share the constraint locations and
  * differ in observed values have one matrix and nrhs right-hand sides; up to
- * 32 of them are solved per launch sequence (their vectors one after the
+ * 64 of them are solved per launch sequence (their vectors one after the
  * other), with the single-system hierarchy and stopping test per rhs.  No
  * reference counterpart (the reference solves realisations one by one). */
 int lsamd_fd_system_solve_multi(lsamd_fd_system* s, const lsamd_solve_opts* opts, int32_t nrhs,

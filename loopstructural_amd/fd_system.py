@@ -168,7 +168,7 @@ class FdSystem:
 
     def solve_multi(self, rhs: torch.Tensor, rtol=1e-5, atol=0.0, maxiter=None, check_every=8):
         """Multigrid solves of N c_r = rhs[r] for every row of rhs (R, n) at
-        once: the V-cycle and outer-iteration launches serve up to 32 right-hand
+        once: the V-cycle and outer-iteration launches serve up to 64 right-hand
         sides each.  Same stopping test per right-hand side as solve()."""
         rhs = rhs.to(self.dev, F64).contiguous()
         if rhs.dim() != 2 or rhs.shape[1] != self.n:
