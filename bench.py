@@ -64,7 +64,7 @@ def parse():
                     help="skip ensemble, P1, evaluation and adjoint measurements")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a 512 MiB copy, the stencil at 128^3 and 512^3 and the "
-                         "256-system ensemble stencil on both tile widths (the workload of the "
+                         "256-system ensemble stencil on both tile widths and z-chunks (the workload of the "
                          "FETCH_SIZE / WRITE_SIZE counter runs)")
     ap.add_argument("--p1-probe", action="store_true",
                     help="only time the P1 stencil at 129^3 and 257^3 nodes (every tuned form, "
@@ -98,21 +98,21 @@ def traffic_probe():
             ms = s.time_stencil(prec, iters=3)
             out[f"{n1}:{prec}"] = {"avg_us": round(ms * 1e3, 2), **s.stencil_info(prec)}
         s.close()
-    # the 256-system ensemble (32^3..96^3 nodes), FP32 stencil on each tile width
+    # the 256-system ensemble (32^3..96^3 nodes), FP32 stencil in each form
     from loopstructural_amd import synthetic
 
     ens = synthetic.ensemble(256, 1004)
     grids = [_lib.make_grid((0, 0, 0), (1.0 / (k - 1),) * 3, (k,) * 3) for k, _, _ in ens]
     tot = sum(k ** 3 for k, _, _ in ens)
-    for tile in (64, 32):
-        os.environ["LSAMD_BATCH_TILE"] = str(tile)
+    for tile, zc in ((64, 16), (32, 16), (64, 32), (32, 32)):
+        os.environ["LSAMD_BATCH_TILE"], os.environ["LSAMD_BATCH_ZC"] = str(tile), str(zc)
         bs = fs.BatchSystem(grids, fs.reg_struct(W, (32, 32, 32), 1e-8))
-        bs.solve(rtol=1e-5, maxiter=1, precision="fp32", check_every=1)  # fixes the tile width
+        bs.solve(rtol=1e-5, maxiter=1, precision="fp32", check_every=1)  # fixes the form
         ms = bs.time_stencil("fp32", iters=3)
-        out[f"ensemble:fp32:{tile}"] = {"avg_us": round(ms * 1e3, 2), "dof": tot,
-                                        "GBs": round(tot * 8 / (ms * 1e-3) / 1e9, 1)}
+        out[f"ensemble:fp32:{tile}:z{zc}"] = {"avg_us": round(ms * 1e3, 2), "dof": tot,
+                                              "GBs": round(tot * 8 / (ms * 1e-3) / 1e9, 1)}
         bs.close()
-        del os.environ["LSAMD_BATCH_TILE"]
+        del os.environ["LSAMD_BATCH_TILE"], os.environ["LSAMD_BATCH_ZC"]
     print(json.dumps({"traffic_probe": out}), flush=True)
 
 

@@ -1197,12 +1197,12 @@ struct lsamd_fd_system {
   BSys* bsys = nullptr;
   int* blk_sys = nullptr;
   int nblk = 0;
-  int4* btiles = nullptr;       // stencil tiles of all systems (64 x 8 tiles)
-  int nbtile = 0;
-  int4* btiles32 = nullptr;     // the same on 32 x 16 tiles
-  int nbtile32 = 0;
-  std::vector<int2> trange64, trange32;  // per system: its range in each table
-  int btx = 0;                  // tile width in use (0: not yet tuned, 64 until then)
+  // stencil tiles of all systems, one table per batch form f: 64 x 8 tiles
+  // (f & 1 == 0) or 32 x 16, z-chunk 16 (f & 2 == 0) or 32
+  int4* btab[4] = {};
+  int nbt[4] = {};
+  std::vector<int2> trange[4];  // per system: its range in each table
+  int bform = -1;               // form in use (-1: not yet tuned, form 0 until then)
   Coef25<double> bku64;         // bulk coefficients (shared by all systems)
   Coef25<float> bku32;
   CgCtl* bctl = nullptr;        // per-system PCG control
@@ -1410,7 +1410,8 @@ int64_t p1_max_partials(const lsamd_fd_system* s) {
 }
 
 int64_t stencil_partials(const lsamd_fd_system* s, int zc) {
-  if (s->op == 2) return std::max(s->nblk, std::max(s->nbtile, s->nbtile32));
+  if (s->op == 2)
+    return std::max<int64_t>(s->nblk, *std::max_element(s->nbt, s->nbt + 4));
   if (s->op == 1) {
     if (zc == kP1Node) return ceil_div(s->n, 256);
     return p1_blocks(s, (zc >= 1 && zc <= kP1NCfg) ? zc - 1 : kP1Default - 1);
@@ -1438,33 +1439,52 @@ void launch_b_tiles(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, hipS
   Coef25<T> ku;
   for (int q = 0; q < kTaps; ++q) ku.k[q] = f32 ? (T)s->bku32.k[q] : (T)s->bku64.k[q];
   const T* tab = f32 ? (const T*)s->tab32 : (const T*)s->tab64;
-  if (s->btx == 32)
-    hipLaunchKernelGGL((k_b_tiles<T, kBZc, false, 32>), dim3((unsigned)s->nbtile32), dim3(NT), 0,
-                       st, x, y, (const BSys*)s->bsys, (const int4*)s->btiles32, tab, ku, s->part,
-                       ctl);
-  else
-    hipLaunchKernelGGL((k_b_tiles<T, kBZc>), dim3((unsigned)s->nbtile), dim3(NT), 0, st, x, y,
-                       (const BSys*)s->bsys, (const int4*)s->btiles, tab, ku, s->part, ctl);
+  const int f = std::max(0, s->bform);
+  const dim3 grid((unsigned)s->nbt[f]);
+  const BSys* sys = (const BSys*)s->bsys;
+  const int4* tb = (const int4*)s->btab[f];
+  switch (f) {
+    case 0:
+      hipLaunchKernelGGL((k_b_tiles<T, kBZc>), grid, dim3(NT), 0, st, x, y, sys, tb, tab, ku,
+                         s->part, ctl);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_b_tiles<T, kBZc, false, 32>), grid, dim3(NT), 0, st, x, y, sys, tb,
+                         tab, ku, s->part, ctl);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_b_tiles<T, 2 * kBZc>), grid, dim3(NT), 0, st, x, y, sys, tb, tab, ku,
+                         s->part, ctl);
+      break;
+    default:
+      hipLaunchKernelGGL((k_b_tiles<T, 2 * kBZc, false, 32>), grid, dim3(NT), 0, st, x, y, sys,
+                         tb, tab, ku, s->part, ctl);
+  }
 }
 
-// Tile width of a batch: the 64 x 8 and 32 x 16 tables timed on the FP32
-// stencil (HIP events, best of three rounds of five launches), once; the
-// systems' partial ranges follow the chosen table.  LSAMD_BATCH_TILE=64|32
-// pins it.
+// Stencil form of a batch: the tables of 64 x 8 and 32 x 16 tiles with
+// z-chunks 16 and 32 timed on the FP32 stencil (HIP events, best of three
+// rounds of five launches), once; the systems' partial ranges follow the
+// chosen table.  LSAMD_BATCH_TILE=64|32 pins the tile width and
+// LSAMD_BATCH_ZC=16|32 the z-chunk.
 int ensure_batch_tuned(lsamd_fd_system* s, hipStream_t st) {
-  if (s->btx) return LSAMD_OK;
-  int pick = 64;
+  if (s->bform >= 0) return LSAMD_OK;
   const char* e = getenv("LSAMD_BATCH_TILE");
-  if (e && (atoi(e) == 64 || atoi(e) == 32)) {
-    pick = atoi(e);
-  } else {
+  const char* ez = getenv("LSAMD_BATCH_ZC");
+  const int tx = e ? atoi(e) : 0, zc = ez ? atoi(ez) : 0;
+  std::vector<int> cand;
+  for (int f = 0; f < 4; ++f)
+    if ((tx != 64 && tx != 32) || tx == ((f & 1) ? 32 : 64))
+      if ((zc != 16 && zc != 32) || zc == ((f & 2) ? 2 * kBZc : kBZc)) cand.push_back(f);
+  int pick = cand[0];
+  if (cand.size() > 1) {
     hipEvent_t e0, e1;
     LSAMD_HIP(hipEventCreate(&e0));
     LSAMD_HIP(hipEventCreate(&e1));
-    float best[2] = {1e30f, 1e30f};
+    float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
     for (int round = 0; round < 3; ++round)
-      for (int c = 0; c < 2; ++c) {
-        s->btx = c == 0 ? 64 : 32;
+      for (int c : cand) {
+        s->bform = c;
         LSAMD_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < 5; ++r) launch_b_tiles<float>(s, s->p32, s->q32, s->bctl_apply, st);
         LSAMD_HIP(hipEventRecord(e1, st));
@@ -1476,10 +1496,11 @@ int ensure_batch_tuned(lsamd_fd_system* s, hipStream_t st) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     LSAMD_CHECK_LAUNCH("batch tile tuning");
-    pick = best[1] < best[0] ? 32 : 64;
+    for (int c : cand)
+      if (best[c] < best[pick]) pick = c;
   }
-  s->btx = pick;
-  const std::vector<int2>& r = pick == 32 ? s->trange32 : s->trange64;
+  s->bform = pick;
+  const std::vector<int2>& r = s->trange[pick];
   for (int i = 0; i < s->nsys; ++i) {
     s->bsys_h[i].t0 = r[i].x;
     s->bsys_h[i].t1 = r[i].y;
@@ -1496,7 +1517,7 @@ int64_t launch_stencil(lsamd_fd_system* s, const T* x, T* y, const CgCtl* ctl, h
                        int zc) {
   if (s->op == 2) {
     launch_b_tiles<T>(s, x, y, s->bctl_apply, st);
-    return s->nbtile;
+    return s->nbt[std::max(0, s->bform)];
   }
   if (s->op == 1) {
     const bool f32 = sizeof(T) == 4;
@@ -5131,7 +5152,6 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   s->op = 2;
   s->nsys = nsys;
   s->nblk = (int)nb;
-  s->nbtile = 0;
   auto bail = [&](int rc) {
     lsamd_fd_system_destroy(s);
     return rc;
@@ -5141,19 +5161,22 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
     for (int b = hs[i].b0; b < hs[i].b1; ++b) bsys_tab[b] = i;
   // stencil tiles: per system, bulk z-chunks of every xy tile (uniform-
   // coefficient tiles flagged) then the two z-boundary parts
-  std::vector<int4> tiles, tiles32;
-  s->trange64.resize(nsys);
-  s->trange32.resize(nsys);
-  for (int i = 0; i < nsys; ++i) {
-    hs[i].t0 = (int)tiles.size();
-    stencil_tiles(i, hs[i].g, kBZc, tiles);
-    hs[i].t1 = (int)tiles.size();
-    s->trange64[i] = make_int2(hs[i].t0, hs[i].t1);
-    const int a = (int)tiles32.size();
-    stencil_tiles(i, hs[i].g, kBZc, tiles32, 32);
-    s->trange32[i] = make_int2(a, (int)tiles32.size());
-    if (tiles.size() > 2000000000ULL || tiles32.size() > 2000000000ULL)
-      return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
+  std::vector<int4> tiles[4];
+  for (int f = 0; f < 4; ++f) {
+    s->trange[f].resize(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      const int a = (int)tiles[f].size();
+      stencil_tiles(i, hs[i].g, (f & 2) ? 2 * kBZc : kBZc, tiles[f], (f & 1) ? 32 : 64);
+      s->trange[f][i] = make_int2(a, (int)tiles[f].size());
+      if (tiles[f].size() > 2000000000ULL) {
+        lsamd_fd_system_destroy(s);
+        return fail(LSAMD_ERR_ARG, "lsamd_batch_create: ensemble too large");
+      }
+    }
+  }
+  for (int i = 0; i < nsys; ++i) {  // form 0 until the batch is tuned
+    hs[i].t0 = s->trange[0][i].x;
+    hs[i].t1 = s->trange[0][i].y;
   }
   if (int rc = dalloc(s, &s->bsys, nsys)) return bail(rc);
   if (int rc = dalloc(s, &s->blk_sys, nb)) return bail(rc);
@@ -5163,13 +5186,12 @@ int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg
   if (int rc = dalloc(s, &s->odone, nsys)) return bail(rc);
   if (int rc = dalloc(s, &s->bres, nsys)) return bail(rc);
   s->bsys_h = hs;  // with the tile ranges
-  s->nbtile = (int)tiles.size();
-  if (int rc = dalloc(s, &s->btiles, tiles.size())) return bail(rc);
-  LSAMD_HIP(legacy_memcpy(s->btiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-  s->nbtile32 = (int)tiles32.size();
-  if (int rc = dalloc(s, &s->btiles32, tiles32.size())) return bail(rc);
-  LSAMD_HIP(legacy_memcpy(s->btiles32, tiles32.data(), tiles32.size() * sizeof(int4),
-                          hipMemcpyHostToDevice));
+  for (int f = 0; f < 4; ++f) {
+    s->nbt[f] = (int)tiles[f].size();
+    if (int rc = dalloc(s, &s->btab[f], tiles[f].size())) return bail(rc);
+    LSAMD_HIP(legacy_memcpy(s->btab[f], tiles[f].data(), tiles[f].size() * sizeof(int4),
+                            hipMemcpyHostToDevice));
+  }
   LSAMD_HIP(legacy_memcpy(s->bsys, hs.data(), nsys * sizeof(BSys), hipMemcpyHostToDevice));
   LSAMD_HIP(legacy_memcpy(s->blk_sys, bsys_tab.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   LSAMD_HIP(legacy_memset(s->bctl, 0, nsys * sizeof(CgCtl)));

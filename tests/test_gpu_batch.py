@@ -43,17 +43,17 @@ def _rows(g, values, normals, n):
     return fs.concat_rows(blocks, n)
 
 
-def _batch():
+def _batch(shapes=SHAPES):
     from loopstructural_amd import fd_system as fs
 
-    cases = [_case(s, 100 + i) for i, s in enumerate(SHAPES)]
+    cases = [_case(s, 100 + i) for i, s in enumerate(shapes)]
     reg = fs.reg_struct(W, (7, 7, 7), 1e-8)
     bs = fs.BatchSystem([c[0] for c in cases], reg)
-    rows = [_rows(c[0], c[2], c[3], int(np.prod(SHAPES[i]))) for i, c in enumerate(cases)]
+    rows = [_rows(c[0], c[2], c[3], int(np.prod(shapes[i]))) for i, c in enumerate(cases)]
     bs.set_rows(rows)
     singles = []
     for i, c in enumerate(cases):
-        s = fs.FdSystem(c[0], fs.reg_struct(W, SHAPES[i], 1e-8))
+        s = fs.FdSystem(c[0], fs.reg_struct(W, shapes[i], 1e-8))
         s.set_rows(rows[i])
         singles.append(s)
     return bs, singles, cases
@@ -70,6 +70,26 @@ def test_batch_operator_equals_single_systems():
         for s, part in enumerate(bs.split(got)):
             ref = getattr(singles[s], name)()
             assert float((part - ref).abs().max()) <= 1e-13 * float(ref.abs().max()), name
+    y32 = bs.apply(x.float()).double()
+    assert float((y32 - y).abs().max()) <= 1e-5 * float(y.abs().max())
+
+
+@pytest.mark.parametrize("tile,zc", [(64, 16), (32, 16), (64, 32), (32, 32)])
+def test_batch_stencil_forms(monkeypatch, tile, zc):
+    """Each stencil form of a batch (64 x 8 or 32 x 16 tiles, z-chunk 16 or
+    32), pinned before the solve that fixes it, against the single systems'
+    operators (1e-13) and its FP32 apply against FP64 (1e-5); the shapes
+    include z-extents of several chunks and partial tiles."""
+    monkeypatch.setenv("LSAMD_BATCH_TILE", str(tile))
+    monkeypatch.setenv("LSAMD_BATCH_ZC", str(zc))
+    shapes = SHAPES + [(21, 18, 45), (36, 20, 71), (70, 9, 40)]
+    bs, singles, _ = _batch(shapes)
+    bs.solve(rtol=1e-5, maxiter=1, precision="fp32", check_every=1)  # fixes the form
+    x = torch.from_numpy(np.random.default_rng(1).standard_normal(bs.n)).cuda()
+    y = bs.apply(x)
+    for s, (ys, xs) in enumerate(zip(bs.split(y), bs.split(x))):
+        ref = singles[s].apply(xs.contiguous())
+        assert float((ys - ref).abs().max()) <= 1e-13 * float(ref.abs().max()), shapes[s]
     y32 = bs.apply(x.float()).double()
     assert float((y32 - y).abs().max()) <= 1e-5 * float(y.abs().max())
 
