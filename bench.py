@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--ensemble-probe", action="store_true",
                     help="only prepare and solve the 256-system ensemble by batched multigrid "
                          "(the workload of its kernel trace)")
+    ap.add_argument("--eval-probe", action="store_true",
+                    help="only run the FD evaluation measurement on a 128^3 grid with seeded "
+                         "coefficients (the workload of its kernel trace and counter runs)")
     ap.add_argument("--workers", type=int, default=12,
                     help="host threads of the ensemble drivers")
     return ap.parse_args()
@@ -137,6 +140,27 @@ def p1_probe():
         s.close()
     torch.cuda.synchronize()
     print(json.dumps({"p1_probe": out}), flush=True)
+
+
+def eval_probe():
+    """FD evaluation (values + gradients) on the 128^3 grid of the headline
+    solve, seeded random coefficients: 1e7 uniform random points and the
+    216^3 linspace points, three timed calls each after one warm call.  A
+    512 MiB device copy first calibrates the counter runs."""
+    import torch
+
+    from loopstructural_amd import _lib
+    from loopstructural_amd import fd_system as fs
+
+    x = torch.rand(512 ** 3, device=_lib.device(), dtype=torch.float32)
+    torch.empty_like(x).copy_(x)
+    del x
+    n1 = 128
+    grid = _lib.make_grid((0, 0, 0), (1 / (n1 - 1),) * 3, (n1,) * 3)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    c = torch.rand(n1 ** 3, dtype=torch.float64, device="cuda", generator=g)
+    res = fd_evaluation(torch, _lib, fs, grid, c, n1 ** 3)
+    print(json.dumps({"eval_probe": res}), flush=True)
 
 
 def ensemble_probe(args):
@@ -210,6 +234,9 @@ def main():
         return
     if args.ensemble_probe:
         ensemble_probe(args)
+        return
+    if args.eval_probe:
+        eval_probe()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched and "
