@@ -211,6 +211,18 @@ def stencil_traffic(nn, prec, zc):
         return None
 
 
+def batched_traffic(key):
+    """HBM bytes per dispatch of a batched-form kernel from the committed
+    counter summary (profiles/fd_batched_traffic.json), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "fd_batched_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic"].get(key)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -335,6 +347,24 @@ def main():
                         f"figure is roofline_hbm_512",
                 "autotune": info_st}
     log(f"stencil {prec}: {avg_ms * 1e3:.2f} us, {achieved:.0f} GB/s")
+    roofline_128 = roofline
+    rb = forms.get("batched", {}).get("roofline_batched")
+    if best == "batched" and rb and "cheb" in rb:
+        # the batched form's largest kernel share: the fine-level Chebyshev step
+        c = rb["cheb"]
+        roofline = {"bound": "hbm", "achieved": c["achieved"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": c["frac"], "traffic": c["traffic"],
+                    "traffic_source": "profiles/fd_batched_traffic.json (FETCH_SIZE x2 + "
+                                      "WRITE_SIZE of the same timed launches, copy-kernel "
+                                      "calibrated; profiles/fd_batched_traffic_pmc.md)",
+                    "kernel": c["kernel"], "avg_us": c["avg_us"],
+                    "algorithmic_bytes": c["algorithmic_bytes"],
+                    "profile": "profiles/fd_batched_128_kernel_trace.md (rocprofv3 per-dispatch "
+                               "durations by kernel, grid and multigrid level, and of these "
+                               "timed launches)",
+                    "note": "largest kernel share of the batched form's time; the other "
+                            "fine-level kernels are in roofline_batched, the single-system "
+                            "128^3 stencil in roofline_stencil_128"}
     hbm = None if args.no_hbm_sweep else hbm_sweep(torch, _lib, fs)
 
     extras = {}
@@ -377,6 +407,7 @@ def main():
         "fp64_reference_iterations": info_ref["iterations"],
         "roofline": roofline,
         "roofline_batched": forms.get("batched", {}).get("roofline_batched"),
+        "roofline_stencil_128": roofline_128,
         "roofline_hbm_512": hbm,
         "extras": extras,
         "cpu_baseline": cpu,
@@ -456,7 +487,8 @@ def batched_form(args, torch, _lib, fs, synthetic, grid, reg, values, normals, n
             res["roofline_batched"][kname] = {
                 "kernel": kfull, "bound": "hbm", "avg_us": round(ms * 1e3, 2),
                 "algorithmic_bytes": int(nbytes), "achieved": round(ach, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "lanes": lanes, "traffic": batched_traffic(f"{kname}:{lanes}")}
             log(f"batched {kname}: {ms * 1e3:.1f} us, {ach:.0f} GB/s")
     sysb.close()
     del sysb
