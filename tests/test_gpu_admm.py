@@ -191,3 +191,45 @@ def test_constant_norm_polish_runs_and_cleans_up():
     g1 = np.linalg.norm(sup.evaluate_gradient(sup.barycentre, it.c), axis=1)
     print(f"gradient-norm spread {np.std(g0) / np.mean(g0):.3f} -> {np.std(g1) / np.mean(g1):.3f}")
     assert np.std(g1) / np.mean(g1) < np.std(g0) / np.mean(g0)
+
+
+def test_constant_norm_polish_fixed_point_of_a_linear_field():
+    """A linear field f = a.x is the exact regularised minimiser of value data
+    sampled from it (its second derivatives vanish), its gradient is a in every
+    element, and the polish rows g_hat . grad f = |a| (g_hat the normalised
+    current gradient, target the median norm |a|) hold for it exactly: the
+    field is a fixed point of the polish.  A wrong sign, a missing
+    normalisation or a wrong target in the polish rows would conflict with the
+    data rows and move it; a target of 2|a| does move it.  ADMM needs an
+    inequality: two value bounds 100 away from the field stay inactive.  ADMM
+    stops at its default primal / dual tolerances (about 2e-5 from the field
+    here), so the fixed point is checked to 1e-4 against a move of 0.2 for the
+    wrong target."""
+    a = np.array([0.3, -0.2, 0.9])
+    rng = np.random.default_rng(11)
+    p = rng.uniform(0, 1, (200, 3))
+    values = np.column_stack([p, p @ a, np.ones(200)])
+    nx, ny, nz = (k + 1 for k in CELLS)
+    i = np.arange(nx * ny * nz)
+    xyz = np.column_stack([(i % nx) * STEP[0], ((i // nx) % ny) * STEP[1],
+                           (i // (nx * ny)) * STEP[2]])
+    lin = xyz @ a
+    q = np.array([[0.3, 0.4, 0.5], [0.7, 0.2, 0.6]])
+    ineq = np.column_stack([q, q @ a - 100.0, q @ a + 100.0])
+    kw = {"admm_weight": 1.0, "nmajor": 20}
+    it = _interp(values, ineq)
+    assert it.solve_system("admm", solver_kwargs=dict(kw))
+    base = _rel(it.c, lin)
+    polished = dict(kw, constant_norm_iterations=2, constant_norm_weight=0.5)
+    it = _interp(values, ineq)
+    assert it.solve_system("admm", solver_kwargs=dict(polished))
+    err = _rel(it.c, lin)
+    it2 = _interp(values, ineq)
+    assert it2.solve_system("admm", solver_kwargs=dict(
+        polished, constant_norm_target=2.0 * float(np.linalg.norm(a))))
+    moved = _rel(it2.c, lin)
+    print(f"linear field: rel err {base:.2e} unpolished, {err:.2e} polished; with target 2|a|: "
+          f"{moved:.2e}")
+    assert base <= 1e-4
+    assert err <= 1e-4
+    assert moved >= 1e-2
