@@ -104,7 +104,9 @@ typedef struct lsamd_solve_opts {
                            Jacobi, LSAMD_MG_DEGREE=1..8 for the Chebyshev
                            degree of the coarse levels (4),
                            LSAMD_MG_DEGREE0 for the fine level's (2, at most
-                           the coarse one), LSAMD_MG_TIMING=1 prints set-up
+                           the coarse one), LSAMD_MR_DEGREE=1..8 for the
+                           coarse levels of the multi-rhs V-cycle (6),
+                           LSAMD_MG_TIMING=1 prints set-up
                            times, LSAMD_MR_COV_LANES=1|2|4|8 sets the
                            right-hand sides per thread of the multi-rhs
                            covered-node smoothing (2)).
@@ -384,7 +386,10 @@ int lsamd_p1_system_create(const lsamd_grid* g, double cgw, double ridge2,
  * systems drop out.  No reference counterpart (the reference solves one
  * system per call, LI/_discrete_interpolator.py:1221-1319); each system's
  * result equals its single-system solve.  The handle also accepts
- * lsamd_fd_system_apply / _rhs / _diag / _time_stencil / _destroy. */
+ * lsamd_fd_system_apply / _rhs / _diag / _time_stencil / _destroy.  The first
+ * solve fixes the stencil form by timing four tile tables (64 x 8 or 32 x 16
+ * tiles, z-chunk 16 or 32); LSAMD_BATCH_TILE=64|32 and LSAMD_BATCH_ZC=16|32
+ * pin the tile width and the z-chunk. */
 int lsamd_batch_create(int32_t nsys, const lsamd_grid* grids, const lsamd_fd_reg* reg,
                        lsamd_fd_system** out);
 /* node_off (nsys+1, host): first node of each system, total last. */
